@@ -1,0 +1,214 @@
+/*
+ * pic3d_amd.h -- C ABI of the MI355X-native PIC3D hot loop.
+ *
+ * One context = one single-domain simulation resident on one GPU: the six
+ * Yee field components, the current/charge accumulators and the particle
+ * species, all in HBM.  Every entry point returns an int status
+ * (PIC_OK == 0, negative on error; the message is in pic_last_error()).
+ * No exception crosses this boundary.  A context owns one HIP stream; calls
+ * on one context are not re-entrant.  All calls are asynchronous on that
+ * stream except *_get, pic_diag, pic_num_particles, pic_particle_counts and
+ * pic_sync, which synchronise.
+ *
+ * Coordinates follow the reference exactly (include/particle.h:9-29,
+ * src/simulation.cpp:23-63 of vvoidpointer/PIC3D):
+ *   - particle positions are in GLOBAL CELL UNITS, x in [0, nx];
+ *   - field arrays are [z][y][x] with x fastest; E and B have extents
+ *     [0, n+2*mem-1] per axis, J and rho have [-1, n+2*mem] (stored from 0,
+ *     i.e. shifted by +1);
+ *   - a particle at global x sits at buffer coordinate x - (1 - mem);
+ *   - Ex, Jx, By, Bz live on the dual (i+1/2) x grid, etc. (standard Yee).
+ *
+ * Parity-critical functions cite the reference file:line they replace.
+ */
+#ifndef PIC3D_AMD_H
+#define PIC3D_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIC_OK 0
+#define PIC_ERR_ARG (-1)       /* bad argument or parameter set            */
+#define PIC_ERR_HIP (-2)       /* a HIP runtime call failed                 */
+#define PIC_ERR_NOMEM (-3)     /* device allocation failed                  */
+#define PIC_ERR_STATE (-4)     /* call not valid in the context's state     */
+#define PIC_ERR_CAPACITY (-5)  /* particle count exceeds species capacity   */
+
+#define PIC_MAX_SPECIES 4
+
+enum pic_precision {
+    PIC_FP64 = 0, /* reference arithmetic: doubles everywhere                  */
+    PIC_FP32 = 1  /* fp32 fields/momenta/deposit, int cell + fp32 offset       */
+};
+
+/* Field component ids for pic_field_get/pic_field_set. */
+enum pic_field {
+    PIC_EX = 0, PIC_EY, PIC_EZ, PIC_BX, PIC_BY, PIC_BZ, /* E/B extents       */
+    PIC_JX, PIC_JY, PIC_JZ,                             /* J extents (+2)    */
+    PIC_RHO0, PIC_RHO1, PIC_RHO2, PIC_RHO3              /* per-species rho   */
+};
+
+/*
+ * The constants.cpp parameter set (src/constants.cpp:6-85,
+ * include/constants.h:16-99) as a runtime struct for one single domain.
+ * pic_params_default() fills the reference's shipped values.
+ */
+typedef struct pic_params {
+    int32_t nx, ny, nz;        /* global cells per axis (global_nx..nz)         */
+    int32_t mem;               /* 2: quadratic shapes, 4: quartic; = ghost width */
+    int32_t periodic[3];       /* rank_period: 1 periodic, 0 open/absorbing      */
+    int32_t precision;         /* enum pic_precision                             */
+    double dx, dy, dz, dt;     /* grid spacing and time step (c/omega units)     */
+    double pi;                 /* Pi as the reference truncates it (3.1415926536)*/
+    double omega;              /* laser angular frequency (1 in the reference)   */
+    double l0;                 /* laser wavelength 2*Pi*c/omega                  */
+    double lx;                 /* Lx, used only for the x-max laser source       */
+    /* laser at x-min, boundary.cpp:8-99 */
+    int32_t laser_polarization;  /* 1 S, 2 P, 3 both; 0 no source              */
+    int32_t laser_right;         /* is_right of radiating_boundary (ref: 0)    */
+    double laser_amplitude;      /* a0                                          */
+    double laser_pulse_duration; /* T of the sin(pi t/T) envelope               */
+    double laser_start_time;
+    double waist, xfocus, yfocus, zfocus, phase;
+} pic_params;
+
+/* Per-species boundary accounting from one pic_particle_boundary call
+ * (src/mpi_utils.cpp:62-230 on a single domain).  Face order:
+ * x-, x+, y-, y+, z-, z+.  "lost" = removed at an open face; "wrapped" =
+ * crossed a periodic face (re-entered, wrapped at the next push). */
+typedef struct pic_counts {
+    int64_t lost[6];
+    int64_t wrapped[6];
+    int64_t remaining;
+} pic_counts;
+
+/* FP64 diagnostics (north_star: energies, spectrum, Gauss residual). */
+#define PIC_SPECTRUM_BINS 64
+typedef struct pic_diag_out {
+    double field_energy;       /* 0.5 * sum_interior (E^2 + B^2) * dV          */
+    double e_energy, b_energy; /* the two halves of field_energy              */
+    double kinetic[PIC_MAX_SPECIES]; /* sum_p m * w * (gamma - 1) * dV          */
+    double charge[PIC_MAX_SPECIES];  /* sum of rho over interior nodes         */
+    double gauss_l2;           /* || div E - rho_total - rho_background ||_2  */
+    double gauss_max;          /* max |div E - rho_total - rho_background|    */
+    double spectrum_emax;      /* upper edge of the spectrum, in m c^2 units  */
+    double spectrum[PIC_SPECTRUM_BINS]; /* species-0 weight histogram of gamma-1 */
+    int64_t out_of_bounds;     /* particles outside the domain at a push       */
+} pic_diag_out;
+
+/* Seeded initial-state generator.  Identical streams on host and device:
+ * a splitmix64 hash of (seed, particle id, draw index). */
+enum pic_profile {
+    PIC_PROFILE_UNIFORM = 0,     /* density everywhere                         */
+    PIC_PROFILE_SLAB = 1,        /* density for x in [x0, x1)                  */
+    PIC_PROFILE_RAMP_TARGET = 2  /* exp ramp to x0, flat target on [x0, x1)    */
+};
+typedef struct pic_loader {
+    int32_t profile;     /* enum pic_profile                                  */
+    int32_t ppc;         /* particles per cell at density                     */
+    double density;      /* n in n_c at full density (weight = density/ppc)   */
+    double x0, x1;       /* slab or target extent, global cells               */
+    double scale_len;    /* ramp e-folding length, cells                       */
+    double cutoff;       /* ramp: no particles below this density             */
+    double u_th;         /* Maxwellian: each momentum component ~ N(0, u_th^2) */
+    uint64_t seed;
+    int64_t id_offset;   /* id of the first particle                           */
+} pic_loader;
+
+/* ---- parameter set and loader (host only, no GPU needed) ---- */
+
+/* Reference defaults (src/constants.cpp:6-65) for an nx x ny x nz domain. */
+void pic_params_default(pic_params* p, int nx, int ny, int nz);
+/* Validate a parameter set; returns PIC_OK or PIC_ERR_ARG and, when msg is
+ * not NULL, writes a reason into msg[0..msg_len). */
+int pic_params_check(const pic_params* p, char* msg, int msg_len);
+/* Number of particles the loader produces for this domain. */
+int64_t pic_loader_count(const pic_params* p, const pic_loader* l);
+/* Generate particles [first, first+n) of the loader on the host. */
+int pic_loader_host(const pic_params* p, const pic_loader* l, int64_t first, int64_t n,
+                    double* x, double* y, double* z, double* ux, double* uy,
+                    double* uz, double* w, int64_t* id);
+/* Library version string. */
+const char* pic_version(void);
+
+/* ---- context ---- */
+
+typedef struct pic_ctx pic_ctx;
+
+/* Allocates the fields (initialize_domain, src/simulation.cpp:132-151) on
+ * device 'device' and zeroes them. */
+int pic_ctx_create(const pic_params* p, int device, pic_ctx** out);
+int pic_ctx_destroy(pic_ctx* ctx);
+const char* pic_last_error(const pic_ctx* ctx);
+int pic_sync(pic_ctx* ctx);
+
+/* Declare species s (charge q, mass m in reference units; an immobile
+ * species is never pushed nor deposited and acts as a uniform background
+ * of charge density rho_background).  capacity = max particles kept. */
+int pic_species_set(pic_ctx* ctx, int s, double charge, double mass, int immobile,
+                    int64_t capacity);
+int pic_set_background(pic_ctx* ctx, double rho_background);
+
+/* Host SoA -> device (positions in global cell units, u = p/(m c)). */
+int pic_particles_set(pic_ctx* ctx, int s, int64_t n, const double* x, const double* y,
+                      const double* z, const double* ux, const double* uy,
+                      const double* uz, const double* w, const int64_t* id);
+/* Device -> host; x_old may be NULL.  *n in: array capacity, out: count.
+ * Order is the device order (compare particle sets by id). */
+int pic_particles_get(pic_ctx* ctx, int s, int64_t* n, double* x, double* y, double* z,
+                      double* ux, double* uy, double* uz, double* w, int64_t* id,
+                      double* x_old, double* y_old, double* z_old);
+/* Run the seeded loader on the device (no host staging). */
+int pic_particles_load(pic_ctx* ctx, int s, const pic_loader* l);
+int pic_num_particles(pic_ctx* ctx, int s, int64_t* n);
+
+/* ---- per-phase hot path (one call = one reference call) ---- */
+
+/* radiating_boundary(rank, 1, laser_right, nprocs, step), boundary.cpp:42-99 */
+int pic_laser_boundary(pic_ctx* ctx, int step);
+/* update_electromagnetic_fields_first, field.cpp:56-86 (+ halo wrap) */
+int pic_fields_first(pic_ctx* ctx);
+/* update_electromagnetic_fields_second, field.cpp:88-117 (+ halo wrap) */
+int pic_fields_second(pic_ctx* ctx);
+/* clear_currents, particle.cpp:576-592 */
+int pic_clear_currents(pic_ctx* ctx);
+/* update_particle_position_momentum(particles[s], n, rank, s) without the
+ * spin part, particle.cpp:113-316 + currents_particle 372-515 */
+int pic_push(pic_ctx* ctx, int s);
+/* exchange_particle_data on one domain, mpi_utils.cpp:62-230; counts may be
+ * NULL (then nothing synchronises). */
+int pic_particle_boundary(pic_ctx* ctx, int s, pic_counts* counts);
+/* exchange_currents on one domain, particle.cpp:594-601 and
+ * mpi_utils.cpp:232-417 */
+int pic_fold_currents(pic_ctx* ctx);
+/* The pre-loop push of main.cpp:75-79: push + boundary of every species with
+ * no current clear and no fold. */
+int pic_preloop(pic_ctx* ctx);
+/* Steps first_step .. first_step+nsteps-1 of run_main_time_loop
+ * (simulation.cpp:159-183) without output. */
+int pic_step(pic_ctx* ctx, int first_step, int nsteps);
+
+/* ---- readback / diagnostics ---- */
+
+/* Full ghosted array of component comp, as doubles in the reference layout. */
+int pic_field_get(pic_ctx* ctx, int comp, double* host);
+int pic_field_set(pic_ctx* ctx, int comp, const double* host);
+/* Number of doubles pic_field_get writes for comp. */
+int64_t pic_field_size(const pic_params* p, int comp);
+int pic_diag(pic_ctx* ctx, pic_diag_out* out);
+/* Cumulative boundary counts of species s since the last pic_particles_set. */
+int pic_particle_counts(pic_ctx* ctx, int s, pic_counts* out);
+
+/* ---- timing helpers for measurement (HIP events on the context stream) ---- */
+int pic_timer_start(pic_ctx* ctx);
+/* Milliseconds since pic_timer_start, after synchronising the stream. */
+int pic_timer_stop(pic_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PIC3D_AMD_H */
