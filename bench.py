@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the PIC hot loop on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32]
+                    [--workload laser_slab|ramp_target|thermal] [--no-cpu-baseline]
+
+Workload (default): the laser-slab case, 256x128x128 cells, lambda = 20
+cells, a0 = 5 Gaussian pulse injected at x-min, absorbing x / periodic y,z,
+0.1 n_c electron slab over x in [64,224) with 16 particles per cell
+(41.9M electrons), Maxwellian u_th = 0.005 c, quartic shapes (mem = 4),
+synthetic seeded particles generated on the device.
+
+A step is one full pass of the reference's time loop (laser boundary,
+E/B half steps, current clear, push = gather + Boris + Esirkepov deposit,
+particle boundary, current fold, E/B half steps).  W untimed warm-up steps,
+then K timed steps bracketed by stream synchronisation.  Prints one JSON
+line: value = particle pushes per second over whole steps; Yee
+cell-updates/s and the push kernel's roofline come from HIP events recorded
+around every phase inside the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def push_bytes(precision: str, ppc: float) -> float:
+    """Compulsory HBM bytes per particle push (gather + push + deposit):
+    particle state read + written once, field and current tiles amortised
+    over the particles of a cell.
+      FP64: read x,y,z,ux,uy,uz,w (56 B), write x,y,z,ux,uy,uz (48 B);
+            per cell 6 field words read + 4 (J, rho) words read-modify-written
+            = 14 words = 112 B, shared by ppc particles.
+      FP32: read cell + 3 offsets + 3 u + w (32 B), write 28 B; per cell
+            14 words of 4 B = 56 B / ppc."""
+    if precision == "fp64":
+        return 104.0 + 112.0 / ppc
+    return 60.0 + 56.0 / ppc
+
+
+def cell_bytes(precision: str) -> float:
+    """Compulsory bytes per Yee cell-update (E and B advanced one dt): read
+    E, B, J (9 words), write E, B (6 words)."""
+    return 15.0 * (8.0 if precision == "fp64" else 4.0)
+
+
+def load_traffic(name: str):
+    """HBM bytes per push-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
+    None when that profile is absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(wl_name: str, precision: str, threads: int) -> dict:
+    """The CPU oracle (FP64 restatement of the reference loop, OpenMP) on a
+    bounded sample of the same workload: the full grid with a thinner
+    electron slab, pre-loop push + 2 timed steps."""
+    from oracle.oracle import Oracle, load_particles
+    from pic3d_amd import workloads as W
+    if wl_name == "laser_slab":
+        wl = W.laser_slab(slab=(64.0, 72.0))
+        sample = "laser_slab grid 256x128x128, mem 4, slab x in [64,72) 16 ppc (2.1M e-), 2 steps"
+    elif wl_name == "thermal":
+        wl = W.thermal()
+        sample = "thermal 32^3, 8 ppc, mem 2, 2 steps"
+    else:
+        wl = W.ramp_target(nx=128, ny=64, nz=64, ppc_target=228)
+        sample = "ramp_target at 128x64x64 (1/64 of the cells), 2 steps"
+    p = wl.params.copy()
+    p.precision = 0
+    parts = load_particles(p, wl.species[0].loader)
+    o = Oracle(p, threads)
+    for s, sp in enumerate(wl.species):
+        o.species_set(s, sp.charge, sp.mass, sp.immobile)
+    o.set_background(wl.rho_background)
+    o.particles_set(0, parts)
+    o.preloop()
+    n0 = o.num_particles(0)
+    t = time.perf_counter()
+    o.step(1, 2)
+    dt = time.perf_counter() - t
+    n1 = o.num_particles(0)
+    o.close()
+    return {"value": 2 * 0.5 * (n0 + n1) / dt, "unit": "particle pushes/s", "cores": threads,
+            "kind": "port", "sample": sample, "seconds": round(dt, 3)}
+
+
+def run_gpu(args, precision: str) -> dict:
+    from pic3d_amd import Simulation, workloads as W
+    from pic3d_amd._abi import PIC_FP32, PIC_FP64, PIC_OPT_TIMING
+    prec = PIC_FP32 if precision == "fp32" else PIC_FP64
+    if args.workload == "laser_slab":
+        wl = W.laser_slab(precision=prec)
+        ppc = 16.0
+    elif args.workload == "thermal":
+        wl = W.thermal(precision=prec)
+        ppc = 8.0
+    else:
+        wl = W.ramp_target(precision=prec)
+        ppc = 32.0
+    p = wl.params
+    sim = Simulation(p, push_kernel=args.kernel, sort_interval=args.sort_interval)
+    from pic3d_amd.simulation import setup
+    setup(sim, wl, device_loader=True)
+    n_start = sim.num_particles(0)
+    sim.preloop()
+    if args.warmup:
+        sim.step(1, args.warmup)
+    sim.sync()
+    n0 = sim.num_particles(0)
+    sim.set_option(PIC_OPT_TIMING, 1)
+    sim.timing(reset=True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(1 + args.warmup, args.steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    tm = sim.timing(reset=True)
+    n1 = sim.num_particles(0)
+    wall = t1 - t0
+    pushes = args.steps * 0.5 * (n0 + n1)
+    cells = p.nx * p.ny * p.nz
+    push_ms, push_launches = tm["push"]
+    fields_ms = tm["fields"][0]
+    avg_push_s = push_ms / max(push_launches, 1) / 1e3
+    bpp = push_bytes(precision, ppc)
+    achieved = bpp * 0.5 * (n0 + n1) / avg_push_s / 1e9
+    traffic = load_traffic(f"push_{precision}_{args.workload}_pmc.txt")
+    out = {
+        "precision": precision,
+        "pushes_per_s": pushes / wall,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "push_kernel_pushes_per_s": 0.5 * (n0 + n1) / avg_push_s,
+        "yee_cell_updates_per_s": cells * args.steps / (fields_ms / 1e3) if fields_ms > 0 else None,
+        "phase_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in tm.items()},
+        "particles": int(n0), "particles_loaded": int(n_start), "cells": cells,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "push (gather+Boris+Esirkepov)",
+                     "bytes_per_push": bpp,
+                     "avg_launch_ms": avg_push_s * 1e3},
+        "yee_roofline": {"bytes_per_cell_update": cell_bytes(precision),
+                         "achieved_GBs": (cell_bytes(precision) * cells * args.steps /
+                                          (fields_ms / 1e3) / 1e9) if fields_ms > 0 else None},
+        "workload": wl.notes,
+    }
+    sim.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "both"])
+    ap.add_argument("--workload", default="laser_slab",
+                    choices=["laser_slab", "ramp_target", "thermal"])
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 direct, 2 tiled")
+    ap.add_argument("--sort-interval", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
+                          "design-only (see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    precs = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
+    res = {pr: run_gpu(args, pr) for pr in precs}
+    head = res[precs[-1]]
+    line = {
+        "metric": "particle pushes/s (gather+push+deposit) & Yee cell-updates/s, FP32 vs FP64",
+        "value": head["pushes_per_s"],
+        "unit": "particle pushes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if precs[-1] == "fp64" else "f32",
+        "data": "synthetic (seeded device loader)",
+        "config": {"workload": args.workload, "detail": head["workload"],
+                   "cells": head["cells"], "particles": head["particles"],
+                   "parallelism": "single GPU"},
+        "yee_cell_updates_per_s": head["yee_cell_updates_per_s"],
+        "roofline": head["roofline"],
+        "phase_ms_per_step": head["phase_ms_per_step"],
+        "by_precision": {pr: {k: v for k, v in r.items() if k not in ("workload",)}
+                         for pr, r in res.items()},
+    }
+    if "fp64" in res and "fp32" in res:
+        line["fp32_over_fp64"] = res["fp32"]["pushes_per_s"] / res["fp64"]["pushes_per_s"]
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.workload, precs[-1], args.cpu_threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -145,6 +145,29 @@ int pic_ctx_destroy(pic_ctx* ctx);
 const char* pic_last_error(const pic_ctx* ctx);
 int pic_sync(pic_ctx* ctx);
 
+/* Tuning / debugging options (no counterpart in the reference). */
+enum pic_option {
+    PIC_OPT_PUSH_KERNEL = 1,   /* 0 auto, 1 direct (global atomics), 2 tiled (LDS) */
+    PIC_OPT_KEEP_OLD = 2,      /* 1: keep x_old/y_old/z_old per particle (FP64)    */
+    PIC_OPT_SORT_INTERVAL = 3, /* re-sort particles by tile every N pushes (>= 1)  */
+    PIC_OPT_TIMING = 4         /* 1: time every phase with HIP events              */
+};
+int pic_set_option(pic_ctx* ctx, int option, int64_t value);
+
+/* Phase categories reported by pic_timing_get. */
+enum pic_timing_cat {
+    PIC_T_FIELDS = 0,   /* Yee half steps + halo wrap + laser boundary          */
+    PIC_T_CLEAR = 1,    /* current/charge clear                                 */
+    PIC_T_PUSH = 2,     /* push kernel (gather + Boris + deposit)               */
+    PIC_T_SORT = 3,     /* tile sort + work list                                 */
+    PIC_T_BOUNDARY = 4, /* particle boundary                                     */
+    PIC_T_FOLD = 5,     /* current ghost fold                                    */
+    PIC_TIMING_CATS = 6
+};
+/* Accumulated milliseconds and launch counts per category since the last
+ * reset (synchronises); reset != 0 clears the accumulators afterwards. */
+int pic_timing_get(pic_ctx* ctx, double* ms, int64_t* count, int reset);
+
 /* Declare species s (charge q, mass m in reference units; an immobile
  * species is never pushed nor deposited and acts as a uniform background
  * of charge density rho_background).  capacity = max particles kept. */

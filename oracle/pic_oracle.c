@@ -640,6 +640,8 @@ static int push_one(const orc_ctx* c, orc_species* S, int64_t p, double* jx, dou
     S->xo[p] = x; S->yo[p] = y; S->zo[p] = z;
     S->x[p] = xn; S->y[p] = yn; S->z[p] = zn;
     S->ux[p] = ux; S->uy[p] = uy; S->uz[p] = uz;
+    /* a non-finite momentum would put the deposit box outside the arrays */
+    if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) return 1;
     deposit_one(c, S->q, S->w[p], x, y, z, xn, yn, zn, jx, jy, jz, rho);
     return 0;
 }

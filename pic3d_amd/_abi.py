@@ -18,6 +18,8 @@ PIC_FP64, PIC_FP32 = 0, 1
 FIELD_NAMES = ["ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz",
                "rho0", "rho1", "rho2", "rho3"]
 PIC_PROFILE_UNIFORM, PIC_PROFILE_SLAB, PIC_PROFILE_RAMP_TARGET = 0, 1, 2
+PIC_OPT_PUSH_KERNEL, PIC_OPT_KEEP_OLD, PIC_OPT_SORT_INTERVAL, PIC_OPT_TIMING = 1, 2, 3, 4
+TIMING_CATS = ["fields", "clear", "push", "sort", "boundary", "fold"]
 
 
 class PicParams(C.Structure):
@@ -117,6 +119,8 @@ SIGNATURES = {
     "pic_field_size": (C.c_int64, [C.POINTER(PicParams), C.c_int]),
     "pic_diag": (C.c_int, [C.c_void_p, C.POINTER(PicDiag)]),
     "pic_particle_counts": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PicCounts)]),
+    "pic_set_option": (C.c_int, [C.c_void_p, C.c_int, C.c_int64]),
+    "pic_timing_get": (C.c_int, [C.c_void_p, _dp, _ip, C.c_int]),
     "pic_timer_start": (C.c_int, [C.c_void_p]),
     "pic_timer_stop": (C.c_int, [C.c_void_p, _dp]),
 }

@@ -1,0 +1,489 @@
+// k_part.hip -- particle bookkeeping on the device: the single-domain
+// particle boundary (exchange_particle_data, mpi_utils.cpp:62-230), the
+// tile counting sort that keeps species grouped by tile (and compacts away
+// removed particles), the tiled-push work list, the seeded loader and the
+// FP64 diagnostics.
+#include "pic_internal.h"
+
+#define PIC_LD_FN __host__ __device__ static inline
+#include "pic_loader_impl.h"
+
+namespace pic {
+
+// ---------------------------------------------------------------------
+// wave-level helpers (64-wide wavefronts)
+// ---------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// For lanes holding a key (valid), group consecutive lanes with equal keys:
+// returns the lane of the run head and the run length (at the head).
+__device__ __forceinline__ void wave_runs(int64_t key, bool valid, int& head_lane, int& run_len) {
+    const int lane = lane_id();
+    const int64_t prev = __shfl_up(key, 1);
+    const bool prev_valid = __shfl_up((int)valid, 1) != 0;
+    const bool head = valid && (lane == 0 || !prev_valid || prev != key);
+    const uint64_t heads = __ballot(head);
+    const uint64_t vmask = __ballot(valid);
+    // head of my run: highest head lane <= lane
+    const uint64_t below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+    head_lane = below ? 63 - __clzll(below) : lane;
+    // run length at the head: up to the next head or the next invalid lane
+    const uint64_t stop = heads | ~vmask;
+    const uint64_t higher = (lane == 63) ? 0ull : (stop & (~0ull << (lane + 1)));
+    const int next = higher ? __ffsll((unsigned long long)higher) - 1 : 64;
+    run_len = next - lane;
+}
+
+// ---------------------------------------------------------------------
+// FP64 tile of a particle, from its wrapped position (same wrap as the push)
+// ---------------------------------------------------------------------
+struct TileArgs {
+    double N[3];
+    int per[3];
+    int n[3];
+    Tiling tl;
+};
+
+__device__ __forceinline__ int tile_of64(double x, double y, double z, const TileArgs& A) {
+    double p[3] = {x, y, z};
+    const bool out = x < 0.0 || x > A.N[0] || y < 0.0 || y > A.N[1] || z < 0.0 || z > A.N[2];
+    int t[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double v = p[a];
+        if (out && A.per[a]) { if (v < 0.0) v += A.N[a]; if (v > A.N[a]) v -= A.N[a]; }
+        int c = (int)floor(v);
+        c = c < 0 ? 0 : (c >= A.n[a] ? A.n[a] - 1 : c);
+        t[a] = c / A.tl.t[a];
+    }
+    return (t[2] * A.tl.nt[1] + t[1]) * A.tl.nt[0] + t[0];
+}
+
+struct Soa64 {
+    double *x, *y, *z, *ux, *uy, *uz, *w, *xo, *yo, *zo;
+    int64_t* id;
+};
+
+__global__ __launch_bounds__(256) void k_hist64(Soa64 A, const int64_t* __restrict__ cnt,
+                                                 int64_t* __restrict__ tile_cnt, TileArgs T) {
+    const int64_t n = cnt[CNT_SLOTS];
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x;
+    for (int64_t b = base; b < n; b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = b + threadIdx.x;
+        bool valid = false;
+        int key = -1;
+        if (p < n) {
+            const double w = A.w[p];
+            valid = (w == w);
+            if (valid) key = tile_of64(A.x[p], A.y[p], A.z[p], T);
+        }
+        int head, len;
+        wave_runs(key, valid, head, len);
+        if (valid && head == lane_id()) atomicAdd((unsigned long long*)&tile_cnt[key], (unsigned long long)len);
+    }
+}
+
+// exclusive scan of ntiles counts (single workgroup); writes off[0..ntiles],
+// the scatter cursors, and the new slot count.
+__global__ __launch_bounds__(1024) void k_scan_tiles(const int64_t* __restrict__ cnt_in,
+                                                      int64_t* __restrict__ off,
+                                                      int64_t* __restrict__ cur, int ntiles,
+                                                      int64_t* __restrict__ slot_cnt) {
+    __shared__ int64_t part[1024];
+    const int T = blockDim.x, t = threadIdx.x;
+    const int per = (ntiles + T - 1) / T;
+    const int lo = min(ntiles, t * per), hi = min(ntiles, lo + per);
+    int64_t s = 0;
+    for (int i = lo; i < hi; i++) s += cnt_in[i];
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < T; d <<= 1) {
+        int64_t v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = part[t] - s;
+    for (int i = lo; i < hi; i++) {
+        off[i] = run;
+        cur[i] = run;
+        run += cnt_in[i];
+    }
+    if (t == T - 1) {
+        off[ntiles] = part[T - 1];
+        slot_cnt[CNT_SLOTS] = part[T - 1];
+    }
+}
+
+static TileArgs tile_args(const pic_ctx* c) {
+    TileArgs T;
+    for (int a = 0; a < 3; a++) {
+        T.N[a] = c->g.n[a];
+        T.per[a] = c->p.periodic[a];
+        T.n[a] = c->g.n[a];
+    }
+    T.tl = c->tl;
+    return T;
+}
+
+static Soa64 soa64(pic_ctx* c, Species& S, int b) {
+    return Soa64{S.x[b], S.y[b], S.z[b], S.ux[b], S.uy[b], S.uz[b], S.w[b],
+                 S.xo[b], S.yo[b], S.zo[b], S.id[b]};
+}
+
+__global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const int64_t* __restrict__ cnt,
+                                                        int64_t* __restrict__ tile_cur, TileArgs T,
+                                                        int keep_old) {
+    const int64_t n = cnt[CNT_N - 1];
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = b + threadIdx.x;
+        bool valid = false;
+        int key = -1;
+        double x = 0, y = 0, z = 0, w = 0;
+        if (p < n) {
+            w = A.w[p];
+            valid = (w == w);
+            if (valid) {
+                x = A.x[p]; y = A.y[p]; z = A.z[p];
+                key = tile_of64(x, y, z, T);
+            }
+        }
+        int head, len;
+        wave_runs(key, valid, head, len);
+        int64_t base = 0;
+        if (valid && head == lane_id())
+            base = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key], (unsigned long long)len);
+        base = __shfl(base, head);
+        if (valid) {
+            const int64_t q = base + (lane_id() - head);
+            B.x[q] = x; B.y[q] = y; B.z[q] = z; B.w[q] = w;
+            B.ux[q] = A.ux[p]; B.uy[q] = A.uy[p]; B.uz[q] = A.uz[p];
+            B.id[q] = A.id[p];
+            if (keep_old) { B.xo[q] = A.xo[p]; B.yo[q] = A.yo[p]; B.zo[q] = A.zo[p]; }
+        }
+    }
+}
+
+int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int blocks) {
+    k_scatter64_dev<<<blocks, 256, 0, c->stream>>>(A, B, S.d_cnt, c->tile_cur, T,
+                                                   c->keep_old && S.xo[0] && S.xo[1]);
+    PIC_LAUNCH_CHECK(c);
+    S.cur = 1 - S.cur;
+    S.sorted = 1;
+    S.pushes_since_sort = 0;
+    return PIC_OK;
+}
+
+int sort32(pic_ctx* c, int s);  // k_push32.hip
+
+// Counting sort by tile into the scratch buffer; drops removed particles.
+// The slot count on the host side is not needed: kernels read it from
+// d_cnt, and the scatter grid covers the species capacity.
+int launch_sort(pic_ctx* c, int s) {
+    Species& S = c->sp[s];
+    if (c->fp32) return sort32(c, s);
+    int r = launch_fill(c, c->tile_cnt, (int64_t)c->tl.ntiles * 8);
+    if (r) return r;
+    TileArgs T = tile_args(c);
+    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 32);
+    Soa64 A = soa64(c, S, S.cur), B = soa64(c, S, 1 - S.cur);
+    k_hist64<<<blocks, 256, 0, c->stream>>>(A, S.d_cnt, c->tile_cnt, T);
+    PIC_LAUNCH_CHECK(c);
+    // the scatter needs the pre-sort slot count: keep it in d_cnt[CNT_N-1]
+    PIC_HIP(c, hipMemcpyAsync(S.d_cnt + (CNT_N - 1), S.d_cnt + CNT_SLOTS, 8,
+                              hipMemcpyDeviceToDevice, c->stream));
+    k_scan_tiles<<<1, 1024, 0, c->stream>>>(c->tile_cnt, S.tile_off, c->tile_cur, c->tl.ntiles, S.d_cnt);
+    PIC_LAUNCH_CHECK(c);
+    return launch_scatter64(c, S, A, B, T, blocks);
+}
+
+// ---------------------------------------------------------------------
+// work list for the tiled push: (tile << 20 | chunk) for every chunk of
+// 'chunk' particles of every non-empty tile.  Single workgroup.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_work(const int64_t* __restrict__ off, int ntiles,
+                                                int chunk, int64_t* __restrict__ work,
+                                                int64_t* __restrict__ nwork, int64_t cap) {
+    __shared__ int64_t part[1024];
+    const int T = blockDim.x, t = threadIdx.x;
+    const int per = (ntiles + T - 1) / T;
+    const int lo = min(ntiles, t * per), hi = min(ntiles, lo + per);
+    int64_t s = 0;
+    for (int i = lo; i < hi; i++) s += (off[i + 1] - off[i] + chunk - 1) / chunk;
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < T; d <<= 1) {
+        int64_t v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t pos = part[t] - s;
+    for (int i = lo; i < hi; i++) {
+        const int64_t k = (off[i + 1] - off[i] + chunk - 1) / chunk;
+        for (int64_t q = 0; q < k; q++, pos++)
+            if (pos < cap) work[pos] = ((int64_t)i << 20) | q;
+    }
+    if (t == T - 1) *nwork = min(part[T - 1], cap);
+}
+
+int build_work(pic_ctx* c, int s, int chunk) {
+    k_work<<<1, 1024, 0, c->stream>>>(c->sp[s].tile_off, c->tl.ntiles, chunk, c->work, c->nwork,
+                                      c->work_cap);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// particle boundary (FP64): classify X, then Y, then Z like the three
+// phases of exchange_particle_data; remove at open faces, count periodic
+// crossings (those particles stay and are wrapped at the next push).
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_boundary64(Soa64 A, int64_t* __restrict__ cnt, TileArgs T) {
+    __shared__ unsigned long long lc[12];
+    if (threadIdx.x < 12) lc[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t n = cnt[CNT_SLOTS];
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const double w = A.w[p];
+        if (!(w == w)) continue;
+        const double v[3] = {A.x[p], A.y[p], A.z[p]};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const bool lo = v[a] < 0.0, hi = !lo && v[a] >= T.N[a];
+            if (!lo && !hi) continue;
+            if (T.per[a]) {
+                atomicAdd(&lc[6 + 2 * a + (hi ? 1 : 0)], 1ull);
+            } else {
+                atomicAdd(&lc[2 * a + (hi ? 1 : 0)], 1ull);
+                A.w[p] = __longlong_as_double(0x7ff8000000000000ll);  // dead: NaN weight
+                A.id[p] = -1;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 12 && lc[threadIdx.x]) {
+        const unsigned long long v = lc[threadIdx.x];
+        const int i = threadIdx.x;
+        // i < 6: lost, else wrapped; cumulative and per-call counters
+        atomicAdd((unsigned long long*)&cnt[(i < 6 ? CNT_LOST : CNT_WRAP) + i % 6], v);
+        atomicAdd((unsigned long long*)&cnt[(i < 6 ? CNT_LOST_STEP : CNT_WRAP_STEP) + i % 6], v);
+        if (i < 6) atomicAdd((unsigned long long*)&cnt[CNT_LIVE], (unsigned long long)(-(long long)v));
+    }
+}
+
+int boundary32(pic_ctx* c, int s);  // k_push32.hip
+
+int launch_boundary(pic_ctx* c, int s) {
+    Species& S = c->sp[s];
+    // reset the per-call counters
+    int r = launch_fill(c, S.d_cnt + CNT_LOST_STEP, 12 * 8);
+    if (r) return r;
+    if (c->fp32) return boundary32(c, s);
+    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 32);
+    k_boundary64<<<blocks, 256, 0, c->stream>>>(soa64(c, S, S.cur), S.d_cnt, tile_args(c));
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// seeded loader (same generator as the host loader)
+// ---------------------------------------------------------------------
+struct LoaderArgs {
+    uint64_t seed;
+    int64_t id_offset;
+    double u_th, wt;
+    int nx, ny;
+};
+
+__global__ __launch_bounds__(256) void k_loader64(Soa64 A, const int64_t* __restrict__ prefix,
+                                                   int64_t n, LoaderArgs L) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        int32_t ix, iy, iz;
+        pic_locate(p, prefix, L.nx, L.ny, &ix, &iy, &iz);
+        const uint64_t pid = (uint64_t)(L.id_offset + p);
+        double x, y, z, ux, uy, uz;
+        pic_load_one(L.seed, pid, ix, iy, iz, L.u_th, &x, &y, &z, &ux, &uy, &uz);
+        A.x[p] = x; A.y[p] = y; A.z[p] = z;
+        A.ux[p] = ux; A.uy[p] = uy; A.uz[p] = uz;
+        A.w[p] = L.wt;
+        A.id[p] = (int64_t)pid;
+        if (A.xo) { A.xo[p] = x; A.yo[p] = y; A.zo[p] = z; }
+    }
+}
+
+int loader32(pic_ctx* c, int s, const pic_loader* l, int64_t n, const int64_t* d_prefix);
+
+int launch_loader(pic_ctx* c, int s, const pic_loader* l, int64_t n, const int64_t* d_prefix) {
+    if (c->fp32) return loader32(c, s, l, n, d_prefix);
+    Species& S = c->sp[s];
+    LoaderArgs L{l->seed, l->id_offset, l->u_th, l->density / l->ppc, c->g.n[0], c->g.n[1]};
+    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(n, 1), 256), 256 * 64);
+    Soa64 A = soa64(c, S, S.cur);
+    if (!c->keep_old) A.xo = A.yo = A.zo = nullptr;
+    k_loader64<<<blocks, 256, 0, c->stream>>>(A, d_prefix, n, L);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// diagnostics (FP64 reductions)
+//   buf[0] sum E^2, [1] sum B^2, [2] sum res^2, [3] max |res| (bits),
+//   [4..7] charge per species, [8..11] kinetic per species,
+//   [12..12+bins) species-0 spectrum
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_fields(const T* __restrict__ ex, const T* __restrict__ ey,
+                                                     const T* __restrict__ ez, const T* __restrict__ bx,
+                                                     const T* __restrict__ by, const T* __restrict__ bz,
+                                                     const T* r0, const T* r1, const T* r2, const T* r3,
+                                                     int mobile_mask, double rho_bg, Grid g, double dx,
+                                                     double dy, double dz, double* __restrict__ buf) {
+    const int i = g.m + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = g.m + blockIdx.y, k = g.m + blockIdx.z;
+    double ee = 0, be = 0, r2s = 0, rmax = 0, ch[4] = {0, 0, 0, 0};
+    if (i < g.m + g.n[0]) {
+        const int64_t o = g.fi(k, j, i);
+        const double a = ex[o], b = ey[o], c = ez[o], d = bx[o], e = by[o], f = bz[o];
+        ee = a * a + b * b + c * c;
+        be = d * d + e * e + f * f;
+        const double div = (a - (double)ex[o - 1]) / dx + (b - (double)ey[o - g.fe[0]]) / dy +
+                           (c - (double)ez[o - (int64_t)g.fe[0] * g.fe[1]]) / dz;
+        const int64_t q = g.ji(k, j, i);
+        const T* rr[4] = {r0, r1, r2, r3};
+        double rho = rho_bg;
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+            if (rr[s]) {
+                ch[s] = rr[s][q];
+                if (mobile_mask & (1 << s)) rho += ch[s];
+            }
+        const double res = div - rho;
+        r2s = res * res;
+        rmax = fabs(res);
+    }
+    // block reduction
+    __shared__ double red[8][256 / 64];
+    double vals[8] = {ee, be, r2s, rmax, ch[0], ch[1], ch[2], ch[3]};
+#pragma unroll
+    for (int v = 0; v < 8; v++) {
+        double x = vals[v];
+        for (int off = 32; off > 0; off >>= 1) {
+            const double y = __shfl_down(x, off);
+            x = (v == 3) ? fmax(x, y) : x + y;
+        }
+        if (lane_id() == 0) red[v][threadIdx.x / 64] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        const int v = threadIdx.x;
+        double x = red[v][0];
+        for (int w = 1; w < (int)(blockDim.x / 64); w++) x = (v == 3) ? fmax(x, red[v][w]) : x + red[v][w];
+        if (v == 3) atomicMax((unsigned long long*)&buf[3], (unsigned long long)__double_as_longlong(x));
+        else if (v < 3) unsafeAtomicAdd(&buf[v], x);
+        else unsafeAtomicAdd(&buf[4 + (v - 4)], x);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_diag_p64(const double* __restrict__ ux, const double* __restrict__ uy,
+                                                   const double* __restrict__ uz, const double* __restrict__ w,
+                                                   const int64_t* __restrict__ cnt, double mass, int s,
+                                                   double emax, double* __restrict__ buf) {
+    __shared__ double hist[PIC_SPECTRUM_BINS];
+    __shared__ double ke_s;
+    for (int b = threadIdx.x; b < PIC_SPECTRUM_BINS; b += blockDim.x) hist[b] = 0;
+    if (threadIdx.x == 0) ke_s = 0;
+    __syncthreads();
+    const int64_t n = cnt[CNT_SLOTS];
+    double ke = 0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const double wp = w[p];
+        if (!(wp == wp)) continue;
+        const double u2 = ux[p] * ux[p] + uy[p] * uy[p] + uz[p] * uz[p];
+        const double gm1 = u2 / (sqrt(1.0 + u2) + 1.0);
+        ke += mass * wp * gm1;
+        if (s == 0) {
+            const int b = (int)(gm1 / emax * PIC_SPECTRUM_BINS);
+            if (b >= 0 && b < PIC_SPECTRUM_BINS) atomicAdd(&hist[b], wp);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off);
+    if (lane_id() == 0) atomicAdd(&ke_s, ke);
+    __syncthreads();
+    if (threadIdx.x == 0) unsafeAtomicAdd(&buf[8 + s], ke_s);
+    if (s == 0)
+        for (int b = threadIdx.x; b < PIC_SPECTRUM_BINS; b += blockDim.x)
+            if (hist[b] != 0) unsafeAtomicAdd(&buf[12 + b], hist[b]);
+}
+
+int diag_p32(pic_ctx* c, int s, double emax);  // k_push32.hip
+
+int launch_diag(pic_ctx* c, pic_diag_out* out) {
+    const int nbuf = 12 + PIC_SPECTRUM_BINS;
+    int r = launch_fill(c, c->diag_buf, nbuf * 8);
+    if (r) return r;
+    const double emax = out->spectrum_emax > 0 ? out->spectrum_emax : 1.0;
+    int mobile = 0;
+    for (int s = 0; s < PIC_MAX_SPECIES; s++)
+        if (c->sp[s].declared && !c->sp[s].immobile) mobile |= 1 << s;
+    dim3 grid(blocks_for(c->g.n[0], 256), c->g.n[1], c->g.n[2]);
+    if (c->fp32) {
+        const float* rr[4];
+        for (int s = 0; s < 4; s++) rr[s] = c->sp[s].declared ? (const float*)c->rho[s] : nullptr;
+        k_diag_fields<float><<<grid, 256, 0, c->stream>>>(
+            (const float*)c->F[0], (const float*)c->F[1], (const float*)c->F[2], (const float*)c->F[3],
+            (const float*)c->F[4], (const float*)c->F[5], rr[0], rr[1], rr[2], rr[3], mobile, c->rho_bg,
+            c->g, c->p.dx, c->p.dy, c->p.dz, c->diag_buf);
+    } else {
+        const double* rr[4];
+        for (int s = 0; s < 4; s++) rr[s] = c->sp[s].declared ? (const double*)c->rho[s] : nullptr;
+        k_diag_fields<double><<<grid, 256, 0, c->stream>>>(
+            (const double*)c->F[0], (const double*)c->F[1], (const double*)c->F[2], (const double*)c->F[3],
+            (const double*)c->F[4], (const double*)c->F[5], rr[0], rr[1], rr[2], rr[3], mobile, c->rho_bg,
+            c->g, c->p.dx, c->p.dy, c->p.dz, c->diag_buf);
+    }
+    PIC_LAUNCH_CHECK(c);
+    for (int s = 0; s < PIC_MAX_SPECIES; s++) {
+        Species& S = c->sp[s];
+        if (!S.declared || S.cap == 0) continue;
+        if (c->fp32) {
+            r = diag_p32(c, s, emax);
+            if (r) return r;
+            continue;
+        }
+        const int blocks = (int)std::min<int64_t>(blocks_for(S.cap, 256), 256 * 16);
+        k_diag_p64<<<blocks, 256, 0, c->stream>>>(S.ux[S.cur], S.uy[S.cur], S.uz[S.cur], S.w[S.cur],
+                                                  S.d_cnt, S.m, s, emax, c->diag_buf);
+        PIC_LAUNCH_CHECK(c);
+    }
+    double h[12 + PIC_SPECTRUM_BINS];
+    PIC_HIP(c, hipMemcpyAsync(h, c->diag_buf, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    const double dV = c->p.dx * c->p.dy * c->p.dz;
+    out->spectrum_emax = emax;
+    out->e_energy = 0.5 * h[0] * dV;
+    out->b_energy = 0.5 * h[1] * dV;
+    out->field_energy = out->e_energy + out->b_energy;
+    out->gauss_l2 = sqrt(h[2]);
+    out->gauss_max = h[3];  // max of non-negative doubles, reduced on their bit patterns
+    for (int s = 0; s < PIC_MAX_SPECIES; s++) {
+        out->charge[s] = h[4 + s];
+        out->kinetic[s] = h[8 + s] * dV;
+    }
+    for (int b = 0; b < PIC_SPECTRUM_BINS; b++) out->spectrum[b] = h[12 + b];
+    int64_t oob = 0;
+    for (int s = 0; s < PIC_MAX_SPECIES; s++)
+        if (c->sp[s].declared && c->sp[s].d_cnt) {
+            int64_t v = 0;
+            PIC_HIP(c, hipMemcpy(&v, c->sp[s].d_cnt + CNT_OOB, 8, hipMemcpyDeviceToHost));
+            oob += v;
+        }
+    out->out_of_bounds = oob;
+    return PIC_OK;
+}
+
+}  // namespace pic

@@ -1,0 +1,737 @@
+// pic_api.hip -- the extern "C" boundary of libpic3d_amd.so
+// (include/pic3d_amd.h).  Host code only: argument checks, device
+// allocation, host<->device transfers and the per-phase / fused step
+// drivers that enqueue the kernels on the context's stream.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "pic_internal.h"
+
+#define PIC_LD_FN static inline
+#include "pic_loader_impl.h"
+
+using namespace pic;
+
+namespace pic {
+
+int set_err(pic_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+int launch_push(pic_ctx* c, int s) {
+    extern int launch_push64(pic_ctx*, int);
+    extern int launch_push32(pic_ctx*, int);
+    return c->fp32 ? launch_push32(c, s) : launch_push64(c, s);
+}
+
+PhaseTimer::PhaseTimer(pic_ctx* ctx, int cat) : c(ctx) {
+    if (!c->timing) return;
+    if (2 * (c->tused + 1) > (int)c->tev.size()) timing_flush(c);
+    k = c->tused++;
+    c->tcat[k] = cat;
+    hipEventRecord(c->tev[2 * k], c->stream);
+}
+
+PhaseTimer::~PhaseTimer() {
+    if (k >= 0) hipEventRecord(c->tev[2 * k + 1], c->stream);
+}
+
+int timing_flush(pic_ctx* c) {
+    if (c->tused == 0) return PIC_OK;
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->tused; k++) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, c->tev[2 * k], c->tev[2 * k + 1]) == hipSuccess) {
+            c->tacc[c->tcat[k]] += ms;
+            c->tcount[c->tcat[k]]++;
+        }
+    }
+    c->tused = 0;
+    return PIC_OK;
+}
+
+}  // namespace pic
+
+namespace {
+
+template <typename T> int dmalloc(pic_ctx* c, T** p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) return PIC_OK;
+    // round up to 256 bytes so fill kernels may write whole 16-byte words
+    size_t bytes = ((size_t)count * sizeof(T) + 255) & ~(size_t)255;
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_err(c, PIC_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return PIC_OK;
+}
+
+template <typename T> void dfree(T*& p) {
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_species(pic_ctx* c, Species& S) {
+    for (int b = 0; b < 2; b++) {
+        dfree(S.x[b]); dfree(S.y[b]); dfree(S.z[b]); dfree(S.ux[b]); dfree(S.uy[b]);
+        dfree(S.uz[b]); dfree(S.w[b]); dfree(S.id[b]); dfree(S.cell[b]); dfree(S.fx[b]);
+        dfree(S.fy[b]); dfree(S.fz[b]); dfree(S.vx[b]); dfree(S.vy[b]); dfree(S.vz[b]);
+        dfree(S.fw[b]); dfree(S.xo[b]); dfree(S.yo[b]); dfree(S.zo[b]);
+    }
+    dfree(S.d_cnt);
+    dfree(S.tile_off);
+    S = Species();
+    (void)c;
+}
+
+int alloc_old(pic_ctx* c, Species& S) {
+    if (c->fp32 || S.cap == 0 || S.xo[0]) return PIC_OK;
+    for (int b = 0; b < 2; b++) {
+        int r;
+        if ((r = dmalloc(c, &S.xo[b], S.cap)) || (r = dmalloc(c, &S.yo[b], S.cap)) ||
+            (r = dmalloc(c, &S.zo[b], S.cap)))
+            return r;
+    }
+    return PIC_OK;
+}
+
+bool valid_species(pic_ctx* c, int s) {
+    return c && s >= 0 && s < PIC_MAX_SPECIES && c->sp[s].declared;
+}
+
+int64_t read_cnt(pic_ctx* c, Species& S, int idx) {
+    int64_t v = 0;
+    hipMemcpyAsync(&v, S.d_cnt + idx, 8, hipMemcpyDeviceToHost, c->stream);
+    hipStreamSynchronize(c->stream);
+    return v;
+}
+
+}  // namespace
+
+// FP32 particle transfer (k_push32.hip)
+int particles_set32(pic_ctx* c, int s, int64_t n, const double* x, const double* y,
+                    const double* z, const double* ux, const double* uy, const double* uz,
+                    const double* w, const int64_t* id);  // k_push32.hip
+
+int particles_get32(pic_ctx* c, int s, int64_t slots, std::vector<double>* a, std::vector<int64_t>& id);
+
+extern "C" {
+
+const char* pic_version(void) { return "pic3d_amd 0.1 (gfx950)"; }
+
+// src/constants.cpp:6-65 on an nx x ny x nz single domain
+void pic_params_default(pic_params* p, int nx, int ny, int nz) {
+    std::memset(p, 0, sizeof(*p));
+    p->nx = nx; p->ny = ny; p->nz = nz;
+    p->mem = 4;
+    p->periodic[0] = p->periodic[1] = p->periodic[2] = 0;
+    p->precision = PIC_FP64;
+    const double pi = 3.1415926536, c1 = 1.0, omega = 1.0;
+    const double l0 = 2.0 * pi * c1 / omega, t0 = 2.0 * pi / omega;
+    p->pi = pi; p->omega = omega; p->l0 = l0;
+    p->lx = 20.0 * l0;
+    p->dx = l0 / 20; p->dy = l0 / 20; p->dz = l0 / 20;
+    p->dt = t0 / 40;
+    p->laser_polarization = 1;
+    p->laser_right = 0;
+    p->laser_amplitude = 1.0 * 1.0 * c1 * omega / (-(-1.0));
+    p->laser_pulse_duration = 10.0 * t0;
+    p->laser_start_time = 0.0;
+    p->waist = 3.0 * l0;
+    p->xfocus = 10.0 * l0;
+    p->phase = 0.0;
+    p->yfocus = 10.0 * l0;
+    p->zfocus = 10.0 * l0;
+}
+
+int pic_params_check(const pic_params* p, char* msg, int msg_len) {
+    auto fail = [&](const char* why) {
+        if (msg && msg_len > 0) snprintf(msg, msg_len, "%s", why);
+        return PIC_ERR_ARG;
+    };
+    if (!p) return fail("null parameter set");
+    if (p->mem != 2 && p->mem != 4) return fail("mem must be 2 (quadratic) or 4 (quartic)");
+    const int n[3] = {p->nx, p->ny, p->nz};
+    for (int a = 0; a < 3; a++) {
+        if (n[a] < 2 * p->mem + 2) return fail("each axis needs at least 2*mem+2 cells");
+        if (n[a] > 1 << 16) return fail("axis longer than 65536 cells");
+    }
+    if (!(p->dx > 0 && p->dy > 0 && p->dz > 0 && p->dt > 0)) return fail("dx, dy, dz, dt must be > 0");
+    // particles (|v| < c) then move less than one cell per step, which keeps
+    // every deposit inside the current arrays' (mem+1)-deep ghost layers
+    if (!(p->dt < p->dx && p->dt < p->dy && p->dt < p->dz)) return fail("dt must be < min(dx, dy, dz)");
+    if (p->precision != PIC_FP64 && p->precision != PIC_FP32) return fail("unknown precision");
+    for (int a = 0; a < 3; a++)
+        if (p->periodic[a] != 0 && p->periodic[a] != 1) return fail("periodic flags must be 0 or 1");
+    if (msg && msg_len > 0) msg[0] = 0;
+    return PIC_OK;
+}
+
+int64_t pic_field_size(const pic_params* p, int comp) {
+    if (!p) return 0;
+    const int g = comp <= PIC_BZ ? 2 * p->mem : 2 * p->mem + 2;
+    return (int64_t)(p->nx + g) * (p->ny + g) * (p->nz + g);
+}
+
+static std::vector<int64_t> loader_prefix(const pic_params* p, const pic_loader* l) {
+    std::vector<int64_t> pre(p->nx + 1, 0);
+    for (int ix = 0; ix < p->nx; ix++)
+        pre[ix + 1] = pre[ix] + pic_profile_count(l->profile, l->ppc, l->density, l->x0, l->x1,
+                                                  l->scale_len, l->cutoff, ix);
+    return pre;
+}
+
+int64_t pic_loader_count(const pic_params* p, const pic_loader* l) {
+    if (!p || !l || l->ppc <= 0 || p->nx <= 0) return 0;
+    auto pre = loader_prefix(p, l);
+    return pre[p->nx] * (int64_t)p->ny * p->nz;
+}
+
+int pic_loader_host(const pic_params* p, const pic_loader* l, int64_t first, int64_t n, double* x,
+                    double* y, double* z, double* ux, double* uy, double* uz, double* w,
+                    int64_t* id) {
+    if (!p || !l || l->ppc <= 0 || first < 0 || n < 0) return PIC_ERR_ARG;
+    if (first + n > pic_loader_count(p, l)) return PIC_ERR_ARG;
+    auto pre = loader_prefix(p, l);
+    const double wt = l->density / l->ppc;
+    for (int64_t q = 0; q < n; q++) {
+        int32_t ix, iy, iz;
+        pic_locate(first + q, pre.data(), p->nx, p->ny, &ix, &iy, &iz);
+        const uint64_t pid = (uint64_t)(l->id_offset + first + q);
+        pic_load_one(l->seed, pid, ix, iy, iz, l->u_th, &x[q], &y[q], &z[q], &ux[q], &uy[q], &uz[q]);
+        w[q] = wt;
+        id[q] = (int64_t)pid;
+    }
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------
+
+int pic_ctx_create(const pic_params* p, int device, pic_ctx** out) {
+    if (!out) return PIC_ERR_ARG;
+    *out = nullptr;
+    char msg[256];
+    if (pic_params_check(p, msg, sizeof(msg)) != PIC_OK) {
+        fprintf(stderr, "pic_ctx_create: %s\n", msg);
+        return PIC_ERR_ARG;
+    }
+    pic_ctx* c = new (std::nothrow) pic_ctx();
+    if (!c) return PIC_ERR_NOMEM;
+    c->p = *p;
+    c->device = device;
+    c->fp32 = p->precision == PIC_FP32;
+    int r = PIC_OK;
+    auto fail = [&](int code) {
+        fprintf(stderr, "pic_ctx_create: %s\n", c->err.c_str());
+        pic_ctx_destroy(c);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) {
+        set_err(c, PIC_ERR_HIP, "hipSetDevice(%d) failed", device);
+        return fail(PIC_ERR_HIP);
+    }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        set_err(c, PIC_ERR_HIP, "stream/event creation failed");
+        return fail(PIC_ERR_HIP);
+    }
+    Grid& g = c->g;
+    g.n[0] = p->nx; g.n[1] = p->ny; g.n[2] = p->nz;
+    g.m = p->mem;
+    g.hm = p->mem / 2;
+    for (int a = 0; a < 3; a++) {
+        g.fe[a] = g.n[a] + 2 * g.m;
+        g.je[a] = g.n[a] + 2 * g.m + 2;
+    }
+    g.fsize = (int64_t)g.fe[0] * g.fe[1] * g.fe[2];
+    g.jsize = (int64_t)g.je[0] * g.je[1] * g.je[2];
+    // tiles: the LDS accumulator box of the tiled push is (t + m + 3)^3
+    Tiling& tl = c->tl;
+    tl.t[0] = 8; tl.t[1] = 8; tl.t[2] = g.m == 2 ? 8 : 4;
+    tl.ntiles = 1;
+    for (int a = 0; a < 3; a++) {
+        tl.nt[a] = (g.n[a] + tl.t[a] - 1) / tl.t[a];
+        tl.ntiles *= tl.nt[a];
+    }
+    const int es = c->fp32 ? 4 : 8;
+    for (int f = 0; f < 6 && !r; f++) {
+        char* q = nullptr;
+        r = dmalloc(c, &q, g.fsize * es);
+        c->F[f] = q;
+        if (!r) r = launch_fill(c, q, g.fsize * es);
+    }
+    for (int f = 0; f < 3 && !r; f++) {
+        char* q = nullptr;
+        r = dmalloc(c, &q, g.jsize * es);
+        c->J[f] = q;
+        if (!r) r = launch_fill(c, q, g.jsize * es);
+    }
+    if (!r) r = dmalloc(c, &c->tile_cnt, tl.ntiles + 1);
+    if (!r) r = dmalloc(c, &c->tile_cur, tl.ntiles + 1);
+    if (!r) r = dmalloc(c, &c->nwork, 8);
+    if (!r) r = dmalloc(c, &c->diag_buf, 12 + PIC_SPECTRUM_BINS + 8);
+    if (r) return fail(r);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_err(c, PIC_ERR_HIP, "initial fill failed");
+        return fail(PIC_ERR_HIP);
+    }
+    *out = c;
+    return PIC_OK;
+}
+
+int pic_ctx_destroy(pic_ctx* c) {
+    if (!c) return PIC_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int s = 0; s < PIC_MAX_SPECIES; s++) {
+        free_species(c, c->sp[s]);
+        if (c->rho[s]) hipFree(c->rho[s]);
+    }
+    for (int f = 0; f < 6; f++) if (c->F[f]) hipFree(c->F[f]);
+    for (int f = 0; f < 3; f++) if (c->J[f]) hipFree(c->J[f]);
+    dfree(c->tile_cnt); dfree(c->tile_cur); dfree(c->work); dfree(c->nwork); dfree(c->diag_buf);
+    for (auto& e : c->tev) hipEventDestroy(e);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return PIC_OK;
+}
+
+const char* pic_last_error(const pic_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int pic_sync(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_set_option(pic_ctx* c, int option, int64_t value) {
+    if (!c) return PIC_ERR_ARG;
+    switch (option) {
+        case PIC_OPT_PUSH_KERNEL:
+            if (value < 0 || value > 2) return set_err(c, PIC_ERR_ARG, "push kernel must be 0, 1 or 2");
+            c->push_kernel = (int)value;
+            return PIC_OK;
+        case PIC_OPT_KEEP_OLD:
+            if (c->fp32 && value) return set_err(c, PIC_ERR_ARG, "x_old is kept only in FP64");
+            c->keep_old = value != 0;
+            for (int s = 0; s < PIC_MAX_SPECIES && c->keep_old; s++)
+                if (c->sp[s].declared) {
+                    int r = alloc_old(c, c->sp[s]);
+                    if (r) return r;
+                }
+            return PIC_OK;
+        case PIC_OPT_TIMING:
+            c->timing = value != 0;
+            if (c->timing && c->tev.empty()) {
+                c->tev.resize(2 * 2048);
+                c->tcat.resize(2048);
+                for (auto& e : c->tev) PIC_HIP(c, hipEventCreate(&e));
+            }
+            return PIC_OK;
+        case PIC_OPT_SORT_INTERVAL:
+            if (value < 1) return set_err(c, PIC_ERR_ARG, "sort interval must be >= 1");
+            c->sort_interval = (int)value;
+            return PIC_OK;
+        default:
+            return set_err(c, PIC_ERR_ARG, "unknown option %d", option);
+    }
+}
+
+int pic_species_set(pic_ctx* c, int s, double charge, double mass, int immobile, int64_t capacity) {
+    if (!c || s < 0 || s >= PIC_MAX_SPECIES) return PIC_ERR_ARG;
+    if (!(mass > 0)) return set_err(c, PIC_ERR_ARG, "mass must be > 0");
+    if (capacity < 0) return set_err(c, PIC_ERR_ARG, "negative capacity");
+    PIC_HIP(c, hipSetDevice(c->device));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    free_species(c, c->sp[s]);
+    Species& S = c->sp[s];
+    S.declared = 1;
+    S.immobile = immobile != 0;
+    S.q = charge;
+    S.m = mass;
+    S.cap = immobile ? 0 : capacity;
+    int r = PIC_OK;
+    if (!r) r = dmalloc(c, &S.d_cnt, CNT_N);
+    if (!r) r = dmalloc(c, &S.tile_off, c->tl.ntiles + 1);
+    if (!r) r = launch_fill(c, S.d_cnt, CNT_N * 8);
+    for (int b = 0; b < 2 && !r && S.cap > 0; b++) {
+        if (c->fp32) {
+            if (!r) r = dmalloc(c, &S.cell[b], S.cap);
+            if (!r) r = dmalloc(c, &S.fx[b], S.cap);
+            if (!r) r = dmalloc(c, &S.fy[b], S.cap);
+            if (!r) r = dmalloc(c, &S.fz[b], S.cap);
+            if (!r) r = dmalloc(c, &S.vx[b], S.cap);
+            if (!r) r = dmalloc(c, &S.vy[b], S.cap);
+            if (!r) r = dmalloc(c, &S.vz[b], S.cap);
+            if (!r) r = dmalloc(c, &S.fw[b], S.cap);
+        } else {
+            if (!r) r = dmalloc(c, &S.x[b], S.cap);
+            if (!r) r = dmalloc(c, &S.y[b], S.cap);
+            if (!r) r = dmalloc(c, &S.z[b], S.cap);
+            if (!r) r = dmalloc(c, &S.ux[b], S.cap);
+            if (!r) r = dmalloc(c, &S.uy[b], S.cap);
+            if (!r) r = dmalloc(c, &S.uz[b], S.cap);
+            if (!r) r = dmalloc(c, &S.w[b], S.cap);
+        }
+        if (!r) r = dmalloc(c, &S.id[b], S.cap);
+    }
+    if (!r && c->keep_old) r = alloc_old(c, S);
+    // current/charge accumulator of this species (rho_particle[s])
+    if (!r && !c->rho[s]) {
+        const int es = c->fp32 ? 4 : 8;
+        char* q = nullptr;
+        r = dmalloc(c, &q, c->g.jsize * es);
+        c->rho[s] = q;
+        if (!r) r = launch_fill(c, q, c->g.jsize * es);
+    }
+    // push work list sized for the largest species
+    if (!r) {
+        const int64_t need = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 64;
+        if (need > c->work_cap) {
+            dfree(c->work);
+            r = dmalloc(c, &c->work, need);
+            c->work_cap = r ? 0 : need;
+        }
+    }
+    if (r) {
+        free_species(c, S);
+        return r;
+    }
+    return PIC_OK;
+}
+
+int pic_set_background(pic_ctx* c, double rho_bg) {
+    if (!c) return PIC_ERR_ARG;
+    c->rho_bg = rho_bg;
+    return PIC_OK;
+}
+
+int pic_particles_set(pic_ctx* c, int s, int64_t n, const double* x, const double* y,
+                      const double* z, const double* ux, const double* uy, const double* uz,
+                      const double* w, const int64_t* id) {
+    if (!valid_species(c, s) || n < 0) return PIC_ERR_ARG;
+    Species& S = c->sp[s];
+    if (S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is immobile (no particle storage)", s);
+    if (n > S.cap) return set_err(c, PIC_ERR_CAPACITY, "%lld particles exceed capacity %lld",
+                                  (long long)n, (long long)S.cap);
+    for (int64_t q = 0; q < n; q++)
+        if (id[q] < 0) return set_err(c, PIC_ERR_ARG, "particle ids must be >= 0");
+    PIC_HIP(c, hipSetDevice(c->device));
+    if (c->fp32) {
+        int r = particles_set32(c, s, n, x, y, z, ux, uy, uz, w, id);
+        if (r) return r;
+    } else {
+        const int b = S.cur;
+        const size_t B = (size_t)n * 8;
+        PIC_HIP(c, hipMemcpyAsync(S.x[b], x, B, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.y[b], y, B, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.z[b], z, B, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.ux[b], ux, B, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.uy[b], uy, B, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.uz[b], uz, B, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.w[b], w, B, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.id[b], id, B, hipMemcpyHostToDevice, c->stream));
+        if (c->keep_old && S.xo[b]) {
+            PIC_HIP(c, hipMemcpyAsync(S.xo[b], x, B, hipMemcpyHostToDevice, c->stream));
+            PIC_HIP(c, hipMemcpyAsync(S.yo[b], y, B, hipMemcpyHostToDevice, c->stream));
+            PIC_HIP(c, hipMemcpyAsync(S.zo[b], z, B, hipMemcpyHostToDevice, c->stream));
+        }
+    }
+    int64_t cnt[CNT_N] = {0};
+    cnt[CNT_SLOTS] = n;
+    cnt[CNT_LIVE] = n;
+    PIC_HIP(c, hipMemcpyAsync(S.d_cnt, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    S.sorted = 0;
+    return PIC_OK;
+}
+
+int pic_particles_get(pic_ctx* c, int s, int64_t* n, double* x, double* y, double* z, double* ux,
+                      double* uy, double* uz, double* w, int64_t* id, double* x_old,
+                      double* y_old, double* z_old) {
+    if (!valid_species(c, s) || !n) return PIC_ERR_ARG;
+    Species& S = c->sp[s];
+    PIC_HIP(c, hipSetDevice(c->device));
+    if (S.immobile) { *n = 0; return PIC_OK; }
+    const int64_t slots = read_cnt(c, S, CNT_SLOTS);
+    std::vector<double> a[10];
+    std::vector<int64_t> ids(slots);
+    const int b = S.cur;
+    if (c->fp32) {
+        int r = particles_get32(c, s, slots, a, ids);
+        if (r) return r;
+    } else {
+        double* src[10] = {S.x[b], S.y[b], S.z[b], S.ux[b], S.uy[b], S.uz[b], S.w[b],
+                           S.xo[b], S.yo[b], S.zo[b]};
+        for (int f = 0; f < 10; f++) {
+            a[f].assign(slots, 0.0);
+            if (slots && src[f] && (f < 7 || c->keep_old))
+                PIC_HIP(c, hipMemcpyAsync(a[f].data(), src[f], slots * 8, hipMemcpyDeviceToHost, c->stream));
+        }
+        if (slots) PIC_HIP(c, hipMemcpyAsync(ids.data(), S.id[b], slots * 8, hipMemcpyDeviceToHost, c->stream));
+        PIC_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    double* dst[10] = {x, y, z, ux, uy, uz, w, x_old, y_old, z_old};
+    int64_t q = 0;
+    for (int64_t p = 0; p < slots; p++) {
+        if (!(a[6][p] == a[6][p])) continue;  // removed
+        if (q >= *n) return set_err(c, PIC_ERR_CAPACITY, "output arrays too small");
+        for (int f = 0; f < 10; f++)
+            if (dst[f]) dst[f][q] = a[f][p];
+        if (id) id[q] = ids[p];
+        q++;
+    }
+    *n = q;
+    return PIC_OK;
+}
+
+int pic_particles_load(pic_ctx* c, int s, const pic_loader* l) {
+    if (!valid_species(c, s) || !l || l->ppc <= 0) return PIC_ERR_ARG;
+    Species& S = c->sp[s];
+    if (S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is immobile", s);
+    const int64_t n = pic_loader_count(&c->p, l);
+    if (n > S.cap) return set_err(c, PIC_ERR_CAPACITY, "loader makes %lld particles, capacity %lld",
+                                  (long long)n, (long long)S.cap);
+    PIC_HIP(c, hipSetDevice(c->device));
+    auto pre = loader_prefix(&c->p, l);
+    int64_t* d_pre = nullptr;
+    int r = dmalloc(c, &d_pre, (int64_t)pre.size());
+    if (r) return r;
+    PIC_HIP(c, hipMemcpyAsync(d_pre, pre.data(), pre.size() * 8, hipMemcpyHostToDevice, c->stream));
+    r = launch_loader(c, s, l, n, d_pre);
+    int64_t cnt[CNT_N] = {0};
+    cnt[CNT_SLOTS] = n;
+    cnt[CNT_LIVE] = n;
+    if (!r) PIC_HIP(c, hipMemcpyAsync(S.d_cnt, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    dfree(d_pre);
+    S.sorted = 0;
+    return r;
+}
+
+int pic_num_particles(pic_ctx* c, int s, int64_t* n) {
+    if (!valid_species(c, s) || !n) return PIC_ERR_ARG;
+    PIC_HIP(c, hipSetDevice(c->device));
+    *n = c->sp[s].immobile ? 0 : read_cnt(c, c->sp[s], CNT_LIVE);
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// per-phase hot path
+// ---------------------------------------------------------------------
+
+int pic_laser_boundary(pic_ctx* c, int step) {
+    if (!c) return PIC_ERR_ARG;
+    PhaseTimer t(c, PIC_T_FIELDS);
+    return launch_laser(c, step);
+}
+
+int pic_fields_first(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    PhaseTimer t(c, PIC_T_FIELDS);
+    int r = launch_e_half(c);
+    return r ? r : launch_b_half(c);
+}
+
+int pic_fields_second(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    PhaseTimer t(c, PIC_T_FIELDS);
+    int r = launch_b_half(c);
+    return r ? r : launch_e_half(c);
+}
+
+int pic_clear_currents(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    PhaseTimer t(c, PIC_T_CLEAR);
+    return launch_clear(c);
+}
+
+int pic_push(pic_ctx* c, int s) {
+    if (!valid_species(c, s)) return PIC_ERR_ARG;
+    Species& S = c->sp[s];
+    if (S.immobile || S.cap == 0) return PIC_OK;
+    return launch_push(c, s);
+}
+
+int pic_particle_boundary(pic_ctx* c, int s, pic_counts* counts) {
+    if (!valid_species(c, s)) return PIC_ERR_ARG;
+    Species& S = c->sp[s];
+    if (S.immobile || S.cap == 0) {
+        if (counts) std::memset(counts, 0, sizeof(*counts));
+        return PIC_OK;
+    }
+    int r;
+    {
+        PhaseTimer t(c, PIC_T_BOUNDARY);
+        r = launch_boundary(c, s);
+    }
+    if (r || !counts) return r;
+    int64_t h[CNT_N];
+    PIC_HIP(c, hipMemcpyAsync(h, S.d_cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    for (int f = 0; f < 6; f++) {
+        counts->lost[f] = h[CNT_LOST_STEP + f];
+        counts->wrapped[f] = h[CNT_WRAP_STEP + f];
+    }
+    counts->remaining = h[CNT_LIVE];
+    return PIC_OK;
+}
+
+int pic_particle_counts(pic_ctx* c, int s, pic_counts* out) {
+    if (!valid_species(c, s) || !out) return PIC_ERR_ARG;
+    Species& S = c->sp[s];
+    std::memset(out, 0, sizeof(*out));
+    if (S.immobile || !S.d_cnt) return PIC_OK;
+    int64_t h[CNT_N];
+    PIC_HIP(c, hipMemcpyAsync(h, S.d_cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    for (int f = 0; f < 6; f++) {
+        out->lost[f] = h[CNT_LOST + f];
+        out->wrapped[f] = h[CNT_WRAP + f];
+    }
+    out->remaining = h[CNT_LIVE];
+    return PIC_OK;
+}
+
+int pic_fold_currents(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    PhaseTimer t(c, PIC_T_FOLD);
+    return launch_fold(c);
+}
+
+int pic_preloop(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    for (int s = 0; s < PIC_MAX_SPECIES; s++)
+        if (c->sp[s].declared) {
+            int r = pic_push(c, s);
+            if (r) return r;
+        }
+    for (int s = 0; s < PIC_MAX_SPECIES; s++)
+        if (c->sp[s].declared) {
+            int r = pic_particle_boundary(c, s, nullptr);
+            if (r) return r;
+        }
+    return PIC_OK;
+}
+
+int pic_step(pic_ctx* c, int first_step, int nsteps) {
+    if (!c || nsteps < 0) return PIC_ERR_ARG;
+    for (int st = first_step; st < first_step + nsteps; st++) {
+        int r;
+        if ((r = pic_laser_boundary(c, st))) return r;
+        if ((r = pic_fields_first(c))) return r;
+        if ((r = pic_clear_currents(c))) return r;
+        for (int s = 0; s < PIC_MAX_SPECIES; s++)
+            if (c->sp[s].declared && !c->sp[s].immobile && c->sp[s].cap > 0)
+                if ((r = launch_push(c, s))) return r;
+        for (int s = 0; s < PIC_MAX_SPECIES; s++)
+            if (c->sp[s].declared && !c->sp[s].immobile && c->sp[s].cap > 0)
+                if ((r = pic_particle_boundary(c, s, nullptr))) return r;
+        if ((r = pic_fold_currents(c))) return r;
+        if ((r = pic_fields_second(c))) return r;
+    }
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// readback
+// ---------------------------------------------------------------------
+
+static void* comp_ptr(pic_ctx* c, int comp, int64_t* n) {
+    if (comp >= PIC_EX && comp <= PIC_BZ) { *n = c->g.fsize; return c->F[comp]; }
+    if (comp >= PIC_JX && comp <= PIC_JZ) { *n = c->g.jsize; return c->J[comp - PIC_JX]; }
+    if (comp >= PIC_RHO0 && comp <= PIC_RHO3) { *n = c->g.jsize; return c->rho[comp - PIC_RHO0]; }
+    *n = 0;
+    return nullptr;
+}
+
+int pic_field_get(pic_ctx* c, int comp, double* host) {
+    if (!c || !host) return PIC_ERR_ARG;
+    int64_t n;
+    void* p = comp_ptr(c, comp, &n);
+    if (comp < PIC_EX || comp > PIC_RHO3) return set_err(c, PIC_ERR_ARG, "bad component %d", comp);
+    PIC_HIP(c, hipSetDevice(c->device));
+    if (!p) {  // species never declared: all zero
+        std::memset(host, 0, (size_t)pic_field_size(&c->p, comp) * 8);
+        return PIC_OK;
+    }
+    if (c->fp32) {
+        std::vector<float> t(n);
+        PIC_HIP(c, hipMemcpyAsync(t.data(), p, n * 4, hipMemcpyDeviceToHost, c->stream));
+        PIC_HIP(c, hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < n; i++) host[i] = t[i];
+    } else {
+        PIC_HIP(c, hipMemcpyAsync(host, p, n * 8, hipMemcpyDeviceToHost, c->stream));
+        PIC_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return PIC_OK;
+}
+
+int pic_field_set(pic_ctx* c, int comp, const double* host) {
+    if (!c || !host) return PIC_ERR_ARG;
+    int64_t n;
+    void* p = comp_ptr(c, comp, &n);
+    if (!p) return set_err(c, PIC_ERR_ARG, "component %d is not allocated", comp);
+    PIC_HIP(c, hipSetDevice(c->device));
+    if (c->fp32) {
+        std::vector<float> t(n);
+        for (int64_t i = 0; i < n; i++) t[i] = (float)host[i];
+        PIC_HIP(c, hipMemcpyAsync(p, t.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        PIC_HIP(c, hipMemcpyAsync(p, host, n * 8, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return PIC_OK;
+}
+
+int pic_diag(pic_ctx* c, pic_diag_out* out) {
+    if (!c || !out) return PIC_ERR_ARG;
+    PIC_HIP(c, hipSetDevice(c->device));
+    return launch_diag(c, out);
+}
+
+int pic_timing_get(pic_ctx* c, double* ms, int64_t* count, int reset) {
+    if (!c) return PIC_ERR_ARG;
+    int r = timing_flush(c);
+    if (r) return r;
+    for (int k = 0; k < PIC_TIMING_CATS; k++) {
+        if (ms) ms[k] = c->tacc[k];
+        if (count) count[k] = c->tcount[k];
+        if (reset) { c->tacc[k] = 0; c->tcount[k] = 0; }
+    }
+    return PIC_OK;
+}
+
+int pic_timer_start(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    PIC_HIP(c, hipEventRecord(c->ev0, c->stream));
+    return PIC_OK;
+}
+
+int pic_timer_stop(pic_ctx* c, double* ms) {
+    if (!c || !ms) return PIC_ERR_ARG;
+    PIC_HIP(c, hipEventRecord(c->ev1, c->stream));
+    PIC_HIP(c, hipEventSynchronize(c->ev1));
+    float f = 0;
+    PIC_HIP(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+    *ms = f;
+    return PIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// pic_internal.h -- device-side state of a pic_ctx and the kernel launchers
+// shared by the translation units of libpic3d_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/pic3d_amd.h"
+
+namespace pic {
+
+// Device layout of one field set.  E/B arrays: [fe2][fe1][fe0], J/rho:
+// [je2][je1][je0] (the reference's ghosted extents, x fastest).
+struct Grid {
+    int n[3];   // cells per axis
+    int m;      // ghost width == shape order (2 or 4)
+    int hm;     // m / 2
+    int fe[3];  // n + 2m
+    int je[3];  // n + 2m + 2
+    int64_t fsize, jsize;
+    __host__ __device__ int64_t fi(int k, int j, int i) const {
+        return ((int64_t)k * fe[1] + j) * fe[0] + i;
+    }
+    // J index for buffer indices in [-1, n+2m]
+    __host__ __device__ int64_t ji(int k, int j, int i) const {
+        return ((int64_t)(k + 1) * je[1] + (j + 1)) * je[0] + (i + 1);
+    }
+};
+
+// Tiling used by the sorted push: cells are grouped in tiles of
+// tx*ty*tz; particles are kept ordered by tile.
+struct Tiling {
+    int t[3];     // tile edge in cells
+    int nt[3];    // tiles per axis
+    int ntiles;
+};
+
+// Per-species state.  FP64: positions in global cell units.  FP32: a linear
+// cell index plus an in-cell offset in [0,1) per axis.
+struct Species {
+    int declared = 0, immobile = 0;
+    double q = 0, m = 1;
+    int64_t cap = 0;
+    // FP64 arrays (slot order), [0] current, [1] scratch for the sort
+    double* x[2] = {}; double* y[2] = {}; double* z[2] = {};
+    double* ux[2] = {}; double* uy[2] = {}; double* uz[2] = {}; double* w[2] = {};
+    // FP32 arrays
+    int32_t* cell[2] = {};
+    float* fx[2] = {}; float* fy[2] = {}; float* fz[2] = {};
+    float* vx[2] = {}; float* vy[2] = {}; float* vz[2] = {}; float* fw[2] = {};
+    int64_t* id[2] = {};
+    double* xo[2] = {}; double* yo[2] = {}; double* zo[2] = {};  // optional x_old (FP64)
+    int pushes_since_sort = 0;
+    int cur = 0;             // which buffer is current
+    // device counters: [0] slots in use, [1] live particles,
+    // [2..7] lost per face, [8..13] wrapped per face (cumulative),
+    // [14..19] lost and [20..25] wrapped since the last boundary call
+    int64_t* d_cnt = nullptr;
+    int64_t* tile_off = nullptr;  // ntiles + 1 tile offsets after the last sort
+    int sorted = 0;               // tile_off valid for the current order
+};
+
+enum { CNT_SLOTS = 0, CNT_LIVE = 1, CNT_LOST = 2, CNT_WRAP = 8, CNT_LOST_STEP = 14,
+       CNT_WRAP_STEP = 20, CNT_OOB = 26, CNT_N = 32 };
+
+}  // namespace pic
+
+struct pic_ctx {
+    pic_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    pic::Grid g{};
+    pic::Tiling tl{};
+    int fp32 = 0;
+    // fields: FP64 or FP32 storage (one of the two sets is allocated)
+    void* F[6] = {};    // ex ey ez bx by bz
+    void* J[3] = {};    // jx jy jz
+    void* rho[PIC_MAX_SPECIES] = {};
+    double rho_bg = 0.0;
+    pic::Species sp[PIC_MAX_SPECIES];
+    // scratch
+    int64_t* tile_cnt = nullptr;  // ntiles histogram
+    int64_t* tile_cur = nullptr;  // ntiles scatter cursors
+    int64_t* work = nullptr;      // push work items (tile << 20 | chunk)
+    int64_t work_cap = 0;
+    int64_t* nwork = nullptr;     // number of work items (device)
+    int sort_interval = 1;        // re-sort the tiled species every N pushes
+    double* diag_buf = nullptr;   // diagnostics reduction buffer
+    int push_kernel = 0;          // 0 auto, 1 naive (global atomics), 2 tiled
+    int keep_old = 0;
+    // per-phase timing (PIC_OPT_TIMING): event pairs around each phase
+    int timing = 0;
+    std::vector<hipEvent_t> tev;      // event pool
+    std::vector<int> tcat;            // category of pair k (events 2k, 2k+1)
+    int tused = 0;                    // pairs recorded since the last flush
+    double tacc[PIC_TIMING_CATS] = {};
+    int64_t tcount[PIC_TIMING_CATS] = {};
+    std::string err;
+};
+
+namespace pic {
+
+// error helpers -------------------------------------------------------
+int set_err(pic_ctx* c, int code, const char* fmt, ...);
+#define PIC_HIP(ctx, call)                                                          \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return ::pic::set_err((ctx), PIC_ERR_HIP, "%s failed: %s (%s:%d)", #call, \
+                                  hipGetErrorString(e_), __FILE__, __LINE__);       \
+    } while (0)
+#define PIC_LAUNCH_CHECK(ctx) PIC_HIP(ctx, hipGetLastError())
+
+// launchers (return PIC_OK or an error code), implemented per TU ------
+int launch_clear(pic_ctx* c);
+int launch_e_half(pic_ctx* c);
+int launch_b_half(pic_ctx* c);
+int launch_wrap(pic_ctx* c, int first_comp);  // wraps comps first..first+2
+int launch_laser(pic_ctx* c, int step);
+int launch_fold(pic_ctx* c);
+int launch_push(pic_ctx* c, int s);
+int launch_boundary(pic_ctx* c, int s);
+int launch_sort(pic_ctx* c, int s);
+int build_work(pic_ctx* c, int s, int chunk);
+int launch_loader(pic_ctx* c, int s, const pic_loader* l, int64_t n, const int64_t* d_prefix);
+int launch_diag(pic_ctx* c, pic_diag_out* out);
+int launch_fill(pic_ctx* c, void* p, int64_t bytes);
+
+// Records an event pair around a phase when timing is on.
+int timing_flush(pic_ctx* c);
+struct PhaseTimer {
+    pic_ctx* c;
+    int k = -1;
+    PhaseTimer(pic_ctx* ctx, int cat);
+    ~PhaseTimer();
+};
+
+constexpr int PIC_PUSH_CHUNK = 4096;  // particles per tiled-push workgroup
+
+inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace pic

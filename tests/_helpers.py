@@ -1,0 +1,70 @@
+"""Shared test helpers: golden fixtures, command replay, comparisons."""
+from __future__ import annotations
+
+import ctypes as C
+import glob
+import os
+
+import numpy as np
+
+from pic3d_amd._abi import FIELD_NAMES, PicLoader, PicParams
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+
+
+def load_golden(name: str) -> dict:
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    p = PicParams.from_buffer_copy(z["params"].tobytes())
+    l = PicLoader.from_buffer_copy(z["loader"].tobytes())
+    parts = {k: z["in_" + k] for k in ("x", "y", "z", "ux", "uy", "uz", "w", "id")}
+    out = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
+    outp = {k[5:]: z[k] for k in z.files if k.startswith("outp_")}
+    return {"params": p, "loader": l, "parts": parts, "cmds": [str(c) for c in z["cmds"]],
+            "fields": out, "particles": outp, "charge": float(z["charge"]),
+            "mass": float(z["mass"]), "rho_background": float(z["rho_background"])}
+
+
+def replay(sim, cmds):
+    """Run a harness command list on an Oracle or a Simulation."""
+    for c in cmds:
+        if c == "pre":
+            sim.preloop()
+        elif c.startswith("steps:"):
+            _, a, n = c.split(":")
+            sim.step(int(a), int(n))
+        elif c.startswith("laser:"):
+            sim.laser_boundary(int(c[6:]))
+        elif c == "first":
+            sim.fields_first()
+        elif c == "second":
+            sim.fields_second()
+        elif c == "clear":
+            sim.clear_currents()
+        elif c.startswith("push:"):
+            sim.push(int(c[5:]))
+        elif c.startswith("bnd:"):
+            sim.particle_boundary(int(c[4:]))
+        elif c == "fold":
+            sim.fold_currents()
+        else:
+            raise ValueError(c)
+
+
+def by_id(parts: dict) -> dict:
+    o = np.argsort(parts["id"], kind="stable")
+    return {k: np.asarray(v)[o] for k, v in parts.items()}
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> float:
+    """max |a - b| / max |b| (norm-wise relative error; 0 if both are 0)."""
+    s = float(np.max(np.abs(b))) if b.size else 0.0
+    d = float(np.max(np.abs(a - b))) if a.size else 0.0
+    return d / s if s > 0 else d
+
+
+def fields_of(sim, comps=range(10)):
+    return {FIELD_NAMES[c]: sim.field_get(c) for c in comps}

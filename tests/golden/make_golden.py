@@ -1,0 +1,90 @@
+"""Write golden vectors from the reference's own hot path.
+
+Runs oracle/_ref/pic_ref_harness (the reference's field.cpp, particle.cpp,
+boundary.cpp, mpi_utils.cpp and simulation.cpp:initialize_domain, compiled
+unmodified; see oracle/ref_harness/Makefile) on small seeded cases and saves
+inputs and outputs as compressed .npz fixtures in this directory.  Needs
+/root/reference; run once in the build container:
+
+    python tests/golden/make_golden.py
+
+Each fixture holds the parameter set (raw pic_params bytes), the loader
+spec, the input particles, the command list, and every output field and
+particle array of the reference run.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import ref_run  # noqa: E402
+from oracle.oracle import load_particles  # noqa: E402
+from pic3d_amd import workloads as W  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def cases():
+    out = []
+    # periodic thermal plasma, quadratic shapes, warm enough to cross faces
+    wl = W.thermal(n=8, ppc=2, seed=11)
+    wl.species[0].loader.u_th = 0.2
+    out.append(("thermal_m2_8", wl, ["pre", "steps:1:6"]))
+    # periodic, quartic shapes
+    wl = W.thermal(n=10, ppc=1, seed=12, mem=4)
+    wl.species[0].loader.u_th = 0.3
+    out.append(("thermal_m4_10", wl, ["pre", "steps:1:5"]))
+    # laser into a slab, quartic shapes, open x (losses), periodic y/z
+    wl = W.laser_slab(nx=16, ny=12, nz=12, ppc=2, slab=(4.0, 12.0), seed=13)
+    wl.species[0].loader.u_th = 0.3
+    out.append(("laser_m4_16x12x12", wl, ["pre", "steps:1:30"]))
+    # laser, quadratic shapes, hot electrons leaving through x
+    wl = W.laser_slab(nx=12, ny=8, nz=8, ppc=2, slab=(0.0, 12.0), seed=14, mem=2)
+    wl.species[0].loader.u_th = 0.6
+    wl.params.laser_amplitude = 2.0
+    out.append(("laser_m2_12x8x8", wl, ["pre", "steps:1:20"]))
+    # single phases: one push of a warm plasma on fields left by a laser run
+    wl = W.laser_slab(nx=12, ny=10, nz=10, ppc=3, slab=(2.0, 10.0), seed=15, mem=2)
+    wl.species[0].loader.u_th = 0.4
+    out.append(("phases_m2_12x10x10", wl,
+                ["laser:1", "first", "clear", "push:0", "bnd:0", "fold", "second",
+                 "laser:2", "first", "clear", "push:0"]))
+    return out
+
+
+def main():
+    if not ref_run.available():
+        if not ref_run.build():
+            raise SystemExit("reference sources not present; cannot make golden vectors")
+    for name, wl, cmds in cases():
+        p = wl.params
+        L = wl.species[0].loader
+        parts = load_particles(p, L)
+        r = ref_run.run(p, [(1, wl.species[0].charge, wl.species[0].mass, parts),
+                            (0, 1.0, 1836.0, None)], cmds)
+        d = {
+            "params": np.frombuffer(bytes(p), dtype=np.uint8),
+            "loader": np.frombuffer(bytes(L), dtype=np.uint8),
+            "charge": wl.species[0].charge, "mass": wl.species[0].mass,
+            "rho_background": wl.rho_background,
+            "cmds": np.array(cmds),
+        }
+        for k in ("x", "y", "z", "ux", "uy", "uz", "w", "id"):
+            d["in_" + k] = parts[k]
+        for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz", "rho0"):
+            d["out_" + k] = r[k]
+        for k, v in r["p0"].items():
+            d["outp_" + k] = v
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **d)
+        print(f"{name}: {len(parts['x'])} -> {len(r['p0']['x'])} particles, "
+              f"{os.path.getsize(path) // 1024} KiB")
+
+
+if __name__ == "__main__":
+    main()
