@@ -101,7 +101,7 @@ def cpu_baseline(wl_name: str, precision: str, threads: int) -> dict:
 
 def run_gpu(args, precision: str) -> dict:
     from pic3d_amd import Simulation, workloads as W
-    from pic3d_amd._abi import PIC_FP32, PIC_FP64, PIC_OPT_TIMING
+    from pic3d_amd._abi import PIC_FP32, PIC_FP64, PIC_OPT_FP32_ACCUM, PIC_OPT_TIMING
     prec = PIC_FP32 if precision == "fp32" else PIC_FP64
     if args.workload == "laser_slab":
         wl = W.laser_slab(precision=prec)
@@ -114,6 +114,8 @@ def run_gpu(args, precision: str) -> dict:
         ppc = 32.0
     p = wl.params
     sim = Simulation(p, push_kernel=args.kernel, sort_interval=args.sort_interval)
+    if precision == "fp32":
+        sim.set_option(PIC_OPT_FP32_ACCUM, args.fp32_accum)
     from pic3d_amd.simulation import setup
     setup(sim, wl, device_loader=True)
     n_start = sim.num_particles(0)
@@ -172,6 +174,8 @@ def main():
                     choices=["laser_slab", "ramp_target", "thermal"])
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 direct, 2 tiled")
     ap.add_argument("--sort-interval", type=int, default=1)
+    ap.add_argument("--fp32-accum", type=int, default=0,
+                    help="FP32 deposit accumulators: 0 FP64 in LDS, 1 int32 fixed point")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     args = ap.parse_args()

@@ -4,35 +4,13 @@
 // removed particles), the tiled-push work list, the seeded loader and the
 // FP64 diagnostics.
 #include "pic_internal.h"
+#include "atomics.h"
+#include "wave.h"
 
 #define PIC_LD_FN __host__ __device__ static inline
 #include "pic_loader_impl.h"
 
 namespace pic {
-
-// ---------------------------------------------------------------------
-// wave-level helpers (64-wide wavefronts)
-// ---------------------------------------------------------------------
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-// For lanes holding a key (valid), group consecutive lanes with equal keys:
-// returns the lane of the run head and the run length (at the head).
-__device__ __forceinline__ void wave_runs(int64_t key, bool valid, int& head_lane, int& run_len) {
-    const int lane = lane_id();
-    const int64_t prev = __shfl_up(key, 1);
-    const bool prev_valid = __shfl_up((int)valid, 1) != 0;
-    const bool head = valid && (lane == 0 || !prev_valid || prev != key);
-    const uint64_t heads = __ballot(head);
-    const uint64_t vmask = __ballot(valid);
-    // head of my run: highest head lane <= lane
-    const uint64_t below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
-    head_lane = below ? 63 - __clzll(below) : lane;
-    // run length at the head: up to the next head or the next invalid lane
-    const uint64_t stop = heads | ~vmask;
-    const uint64_t higher = (lane == 63) ? 0ull : (stop & (~0ull << (lane + 1)));
-    const int next = higher ? __ffsll((unsigned long long)higher) - 1 : 64;
-    run_len = next - lane;
-}
 
 // ---------------------------------------------------------------------
 // FP64 tile of a particle, from its wrapped position (same wrap as the push)
@@ -176,6 +154,12 @@ int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int b
 
 int sort32(pic_ctx* c, int s);  // k_push32.hip
 
+int launch_scan_tiles(pic_ctx* c, Species& S) {
+    k_scan_tiles<<<1, 1024, 0, c->stream>>>(c->tile_cnt, S.tile_off, c->tile_cur, c->tl.ntiles, S.d_cnt);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
 // Counting sort by tile into the scratch buffer; drops removed particles.
 // The slot count on the host side is not needed: kernels read it from
 // d_cnt, and the scatter grid covers the species capacity.
@@ -192,8 +176,8 @@ int launch_sort(pic_ctx* c, int s) {
     // the scatter needs the pre-sort slot count: keep it in d_cnt[CNT_N-1]
     PIC_HIP(c, hipMemcpyAsync(S.d_cnt + (CNT_N - 1), S.d_cnt + CNT_SLOTS, 8,
                               hipMemcpyDeviceToDevice, c->stream));
-    k_scan_tiles<<<1, 1024, 0, c->stream>>>(c->tile_cnt, S.tile_off, c->tile_cur, c->tl.ntiles, S.d_cnt);
-    PIC_LAUNCH_CHECK(c);
+    r = launch_scan_tiles(c, S);
+    if (r) return r;
     return launch_scatter64(c, S, A, B, T, blocks);
 }
 
@@ -383,8 +367,8 @@ __global__ __launch_bounds__(256) void k_diag_fields(const T* __restrict__ ex, c
         double x = red[v][0];
         for (int w = 1; w < (int)(blockDim.x / 64); w++) x = (v == 3) ? fmax(x, red[v][w]) : x + red[v][w];
         if (v == 3) atomicMax((unsigned long long*)&buf[3], (unsigned long long)__double_as_longlong(x));
-        else if (v < 3) unsafeAtomicAdd(&buf[v], x);
-        else unsafeAtomicAdd(&buf[4 + (v - 4)], x);
+        else if (v < 3) hbm_add(&buf[v], x);
+        else hbm_add(&buf[4 + (v - 4)], x);
     }
 }
 
@@ -408,16 +392,16 @@ __global__ __launch_bounds__(256) void k_diag_p64(const double* __restrict__ ux,
         ke += mass * wp * gm1;
         if (s == 0) {
             const int b = (int)(gm1 / emax * PIC_SPECTRUM_BINS);
-            if (b >= 0 && b < PIC_SPECTRUM_BINS) atomicAdd(&hist[b], wp);
+            if (b >= 0 && b < PIC_SPECTRUM_BINS) lds_add(&hist[b], wp);
         }
     }
     for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off);
     if (lane_id() == 0) atomicAdd(&ke_s, ke);
     __syncthreads();
-    if (threadIdx.x == 0) unsafeAtomicAdd(&buf[8 + s], ke_s);
+    if (threadIdx.x == 0) hbm_add(&buf[8 + s], ke_s);
     if (s == 0)
         for (int b = threadIdx.x; b < PIC_SPECTRUM_BINS; b += blockDim.x)
-            if (hist[b] != 0) unsafeAtomicAdd(&buf[12 + b], hist[b]);
+            if (hist[b] != 0) hbm_add(&buf[12 + b], hist[b]);
 }
 
 int diag_p32(pic_ctx* c, int s, double emax);  // k_push32.hip

@@ -1,25 +1,828 @@
-// k_push32.hip -- FP32 form (placeholder until the FP32 kernels land).
+// k_push32.hip -- the FP32 form of the particle hot path.
+//
+// Particles: linear cell index (int32, -1 = removed) + in-cell offsets
+// fx, fy, fz in [0,1) (fp32) + momenta u = p/mc (fp32) + weight (fp32) + id.
+// Fields, currents and charge are fp32.  Positions relative to the cell keep
+// the shape-function arguments small, so fp32 keeps ~1e-7 of a cell of
+// resolution anywhere in the domain.
+//
+// Same algorithm as the FP64 path (and the reference, particle.cpp:113-515):
+// periodic wrap, staggered gather with quadratic/quartic B-splines,
+// relativistic Boris, Esirkepov deposit of J and rho.  Differences, all
+// exact in exact arithmetic: the gather is evaluated separably (x, then y,
+// then z sums) with fused multiply-adds; the last point of each cumulative
+// current sum (which is sum(S1) - sum(S0) = 0) is not deposited; periodic
+// positions are wrapped right after the move instead of at the next push,
+// and boundary crossings are classified by the push itself (the deposit
+// still uses the unwrapped position, so the ghost fold carries the current).
+#include <cmath>
 #include <vector>
 
 #include "pic_internal.h"
+#include "atomics.h"
+#include "shapes.h"
+#include "wave.h"
+
+#define PIC_LD_FN __host__ __device__ static inline
+#include "pic_loader_impl.h"
 
 namespace pic {
 
-static int fp32_missing(pic_ctx* c) {
-    return set_err(c, PIC_ERR_STATE, "the FP32 form is not built yet");
+struct Soa32 {
+    int32_t* cell;
+    float *fx, *fy, *fz, *ux, *uy, *uz, *w;
+    int64_t* id;
+};
+
+static Soa32 soa32(Species& S, int b) {
+    return Soa32{S.cell[b], S.fx[b], S.fy[b], S.fz[b], S.vx[b], S.vy[b], S.vz[b], S.fw[b], S.id[b]};
 }
-int launch_push32(pic_ctx* c, int) { return fp32_missing(c); }
-int sort32(pic_ctx* c, int) { return fp32_missing(c); }
-int boundary32(pic_ctx* c, int) { return fp32_missing(c); }
-int loader32(pic_ctx* c, int, const pic_loader*, int64_t, const int64_t*) { return fp32_missing(c); }
-int diag_p32(pic_ctx* c, int, double) { return fp32_missing(c); }
+
+struct Consts32 {
+    // LDS accumulation is int32 fixed point: component c (Jx, Jy, Jz, rho)
+    // in units ufix[c] = (bound on one particle's contribution) * 2^-18, so
+    // one chunk (<= 4096 = 2^12 particles, one contribution each per node)
+    // stays below 2^30 and cannot overflow; rounding error per deposit is
+    // <= 2^-19 of that bound.  (Float atomic adds to LDS are an order of
+    // magnitude slower than integer ones on this part.)
+    float ufix[4], inv_ufix[4];
+    float qm;          // q/m * dt/2
+    float q;
+    float cfx, cfy, cfz;  // dt/dx, dt/dy, dt/dz
+    float cjx, cjy, cjz;  // dx/dt, dy/dt, dz/dt (current scale per q w)
+    int n[3];
+    int per[3];
+    int m;
+};
+
+// quadratic/quartic weights for an offset delta = x - round(x) in [-0.5, 0.5)
+template <int M> __device__ __forceinline__ void shape32(float d, float* W) {
+    if (M == 2) shape2_delta<float>(d, W);
+    else shape4_d<float>(d + 2.0f, W);
+}
+
+template <int L>
+__device__ __forceinline__ void shift_down32(float* A) {
+#pragma unroll
+    for (int t = 0; t < L - 1; t++) A[t] = A[t + 1];
+}
+
+// ---------------------------------------------------------------------
+// gather (fp32): E and B at the particle.  FR(comp, k, j, i) reads a field
+// value at buffer indices.  Separable sums (x, then y, then z) with FMAs;
+// for quartic shapes the z loop is rolled (register rotation of the z
+// weights) to keep the kernel inside the instruction cache.
+// ---------------------------------------------------------------------
+template <int M, typename FR>
+__device__ __forceinline__ void gather32(int Cx, int Cy, int Cz, float fx, float fy, float fz,
+                                         float* E, float* B, FR fr) {
+    constexpr int H = M / 2;
+    float Wxp[M + 1], Wyp[M + 1], Wzp[M + 1], Wxd[M + 1], Wyd[M + 1], Wzd[M + 1];
+    const int ox = fx >= 0.5f, oy = fy >= 0.5f, oz = fz >= 0.5f;
+    shape32<M>(fx - (float)ox, Wxp);
+    shape32<M>(fy - (float)oy, Wyp);
+    shape32<M>(fz - (float)oz, Wzp);
+    shape32<M>(fx - 0.5f, Wxd);
+    shape32<M>(fy - 0.5f, Wyd);
+    shape32<M>(fz - 0.5f, Wzd);
+    const int ipx = Cx + ox - H, ipy = Cy + oy - H, ipz = Cz + oz - H;  // primal box start
+    const int idx = Cx - H, idy = Cy - H, idz = Cz - H;                 // dual box start
+    // stagger per component (x, y, z): Ex (d,p,p) Ey (p,d,p) Ez (p,p,d)
+    // Bx (p,d,d) By (d,p,d) Bz (d,d,p)
+    auto one = [&](int c, const float* WX, const float* WY, const float* WZ, int bx, int by,
+                   int bz) -> float {
+        float acc = 0.0f;
+        float wz[M + 1];
+#pragma unroll
+        for (int t = 0; t <= M; t++) wz[t] = WZ[t];
+#pragma unroll(M == 2 ? M + 1 : 1)
+        for (int k = 0; k <= M; k++) {
+            float row = 0.0f;
+#pragma unroll
+            for (int j = 0; j <= M; j++) {
+                float s = 0.0f;
+#pragma unroll
+                for (int i = 0; i <= M; i++) s = fmaf(WX[i], fr(c, bz + k, by + j, bx + i), s);
+                row = fmaf(WY[j], s, row);
+            }
+            acc = fmaf(wz[0], row, acc);
+            shift_down32<M + 1>(wz);
+        }
+        return acc;
+    };
+    E[0] = one(0, Wxd, Wyp, Wzp, idx, ipy, ipz);
+    E[1] = one(1, Wxp, Wyd, Wzp, ipx, idy, ipz);
+    E[2] = one(2, Wxp, Wyp, Wzd, ipx, ipy, idz);
+    B[0] = one(3, Wxp, Wyd, Wzd, ipx, idy, idz);
+    B[1] = one(4, Wxd, Wyp, Wzd, idx, ipy, idz);
+    B[2] = one(5, Wxd, Wyd, Wzp, idx, idy, ipz);
+}
+
+__device__ __forceinline__ void boris32(float* u, const float* E, const float* B, float qm) {
+    const float e0 = E[0] * qm, e1 = E[1] * qm, e2 = E[2] * qm;
+    const float b0 = B[0] * qm, b1 = B[1] * qm, b2 = B[2] * qm;
+    const float px1 = u[0] + e0, py1 = u[1] + e1, pz1 = u[2] + e2;
+    const float ig1 = rsqrtf(1.0f + (px1 * px1 + py1 * py1 + pz1 * pz1));
+    const float tx = b0 * ig1, ty = b1 * ig1, tz = b2 * ig1;
+    const float tt = 2.0f / (1.0f + tx * tx + ty * ty + tz * tz);
+    const float sx = tx * tt, sy = ty * tt, sz = tz * tt;
+    const float px2 = px1 + (py1 * tz - pz1 * ty);
+    const float py2 = py1 + (pz1 * tx - px1 * tz);
+    const float pz2 = pz1 + (px1 * ty - py1 * tx);
+    u[0] = px1 + (py2 * sz - pz2 * sy) + e0;
+    u[1] = py1 + (pz2 * sx - px2 * sz) + e1;
+    u[2] = pz1 + (px2 * sy - py2 * sx) + e2;
+}
+
+// Esirkepov deposit (fp32) in the frame of the particle's old cell: old
+// offsets f0 in [0,1), new offsets f1 = f0 + d with |d| < 1.  The sink D
+// receives D.base(bz, by, bx) (box origin relative to the cell) and then
+// D(comp, k, j, i, v) with box-relative (k, j, i).  The box is always
+// L = M+2 points per axis; the last point of each cumulative sum (which
+// carries sum(S1) - sum(S0) = 0) is not deposited, and the extra planes
+// along the other axes carry exact zeros.
+template <int M, typename D>
+__device__ __forceinline__ void esirkepov32(float qw, const float* f0, const float* f1,
+                                            const Consts32& K, D& dst) {
+    constexpr int H = M / 2, L = M + 2;
+    constexpr int R = (M == 2) ? L : 1;  // unroll factor of the outer loops
+    float S0[3][M + 1], S1[3][M + 1];
+    int b[3], d0[3], d1[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int i1 = f0[a] >= 0.5f ? 1 : 0;
+        const int i2 = (int)floorf(f1[a] + 0.5f);  // -1 .. 2
+        shape32<M>(f0[a] - (float)i1, S0[a]);
+        shape32<M>(f1[a] - (float)i2, S1[a]);
+        b[a] = min(i1, i2) - H;
+        d0[a] = i1 - H - b[a];
+        d1[a] = i2 - H - b[a];
+    }
+    float X0[L], X1[L], Y0[L], Y1[L], Z0[L], Z1[L];
+    auto place = [](const float* S, int d, float* A) {
+#pragma unroll
+        for (int t = 0; t < L; t++) {
+            const float a0 = (t <= M) ? S[t] : 0.0f;
+            const float a1 = (t >= 1) ? S[t - 1] : 0.0f;
+            A[t] = d ? a1 : a0;
+        }
+    };
+    place(S0[0], d0[0], X0); place(S1[0], d1[0], X1);
+    place(S0[1], d0[1], Y0); place(S1[1], d1[1], Y1);
+    place(S0[2], d0[2], Z0); place(S1[2], d1[2], Z1);
+    const float cx = -qw * K.cjx, cy = -qw * K.cjy, cz = -qw * K.cjz;
+    const float third = 1.0f / 3.0f;
+    dst.base(b[2], b[1], b[0]);
+    float za[L], zb[L];
+#pragma unroll
+    for (int t = 0; t < L; t++) { za[t] = Z0[t]; zb[t] = Z1[t] - Z0[t]; }
+    // Jx: cumulative along x, weights W_yz
+#pragma unroll(R)
+    for (int k = 0; k < L; k++) {
+        const float sz0 = za[0], dsz = zb[0];
+#pragma unroll
+        for (int j = 0; j < L; j++) {
+            const float sy0 = Y0[j], dsy = Y1[j] - sy0;
+            const float wyz = cx * (sy0 * sz0 + 0.5f * dsy * sz0 + 0.5f * sy0 * dsz + third * dsy * dsz);
+            float acc = 0.0f;
+#pragma unroll
+            for (int i = 0; i < L - 1; i++) {
+                acc += X1[i] - X0[i];
+                dst(0, k, j, i, acc * wyz);
+            }
+        }
+        shift_down32<L>(za); shift_down32<L>(zb);
+    }
+    // Jy: cumulative along y, weights W_xz
+#pragma unroll
+    for (int t = 0; t < L; t++) { za[t] = Z0[t]; zb[t] = Z1[t] - Z0[t]; }
+#pragma unroll(R)
+    for (int k = 0; k < L; k++) {
+        const float sz0 = za[0], dsz = zb[0];
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const float sx0 = X0[i], dsx = X1[i] - sx0;
+            const float wxz = cy * (sx0 * sz0 + 0.5f * dsx * sz0 + 0.5f * sx0 * dsz + third * dsx * dsz);
+            float acc = 0.0f;
+#pragma unroll
+            for (int j = 0; j < L - 1; j++) {
+                acc += Y1[j] - Y0[j];
+                dst(1, k, j, i, acc * wxz);
+            }
+        }
+        shift_down32<L>(za); shift_down32<L>(zb);
+    }
+    // Jz: cumulative along z, weights W_xy
+    float ya[L], yb[L];
+#pragma unroll
+    for (int t = 0; t < L; t++) { ya[t] = Y0[t]; yb[t] = Y1[t] - Y0[t]; }
+#pragma unroll(R)
+    for (int j = 0; j < L; j++) {
+        const float sy0 = ya[0], dsy = yb[0];
+#pragma unroll
+        for (int i = 0; i < L; i++) {
+            const float sx0 = X0[i], dsx = X1[i] - sx0;
+            const float wxy = cz * (sx0 * sy0 + 0.5f * dsx * sy0 + 0.5f * sx0 * dsy + third * dsx * dsy);
+            float acc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < L - 1; k++) {
+                acc += Z1[k] - Z0[k];
+                dst(2, k, j, i, acc * wxy);
+            }
+        }
+        shift_down32<L>(ya); shift_down32<L>(yb);
+    }
+    // rho at the new position: nodes i2-H .. i2+H
+    dst.base(b[2] + d1[2], b[1] + d1[1], b[0] + d1[0]);
+    float zr[M + 1];
+#pragma unroll
+    for (int t = 0; t <= M; t++) zr[t] = qw * S1[2][t];
+#pragma unroll(M == 2 ? M + 1 : 1)
+    for (int k = 0; k <= M; k++) {
+#pragma unroll
+        for (int j = 0; j <= M; j++) {
+            const float syz = zr[0] * S1[1][j];
+#pragma unroll
+            for (int i = 0; i <= M; i++) dst(3, k, j, i, syz * S1[0][i]);
+        }
+        shift_down32<M + 1>(zr);
+    }
+}
+
+// deposit sinks (fp32) ---------------------------------------------------
+struct HbmSink32 {  // global atomics; indices relative to the cell (Cz, Cy, Cx)
+    float* jp[4];
+    Grid g;
+    int Cz, Cy, Cx;
+    int bz, by, bx;
+    __device__ __forceinline__ void base(int z, int y, int x) { bz = Cz + z; by = Cy + y; bx = Cx + x; }
+    __device__ __forceinline__ void operator()(int c, int k, int j, int i, float v) {
+        hbm_add(&jp[c][g.ji(bz + k, by + j, bx + i)], v);
+    }
+};
+
+// LDS accumulator box (ACC = double: FP64 sums; ACC = int: fixed point in
+// units ufix[c]); 'cell' = the particle's cell offset in the box.
+template <int AX, int AY, int AVOL, typename ACC>
+struct LdsSink32 {
+    ACC* acc;
+    const float* inv_u;
+    int cell;
+    int off;
+    __device__ __forceinline__ void base(int z, int y, int x) { off = cell + (z * AY + y) * AX + x; }
+    __device__ __forceinline__ void operator()(int c, int k, int j, int i, float v) {
+        if constexpr (sizeof(ACC) == 8)
+            lds_add((double*)&acc[off + c * AVOL + (k * AY + j) * AX + i], (double)v);
+        else
+            lds_add_i32((int*)&acc[off + c * AVOL + (k * AY + j) * AX + i], __float2int_rn(v * inv_u[c]));
+    }
+};
+
+// Move: new offsets, cell shifts, periodic wrap / open-boundary removal.
+// Returns the new linear cell or -1 (removed); counts crossings in cnt12
+// (lost x-,x+,y-,y+,z-,z+ then wrapped x-,...).
+__device__ __forceinline__ int move32(int cx, int cy, int cz, float* f, const float* d,
+                                      const Consts32& K, unsigned* cnt12) {
+    int c[3] = {cx, cy, cz};
+    bool lost = false;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float nf = f[a] + d[a];
+        const float s = floorf(nf);
+        nf -= s;
+        int sh = (int)s;
+        if (nf >= 1.0f) { nf = 0.0f; sh += 1; }  // rounding of a tiny negative offset
+        f[a] = nf;
+        int cc = c[a] + sh;
+        // phases X, Y, Z in order: once removed at an open face, later axes
+        // are not classified (mpi_utils.cpp:62-230)
+        if (!lost && (cc < 0 || cc >= K.n[a])) {
+            const int face = 2 * a + (cc < 0 ? 0 : 1);
+            if (K.per[a]) {
+                cc += cc < 0 ? K.n[a] : -K.n[a];
+                atomicAdd(&cnt12[6 + face], 1u);
+            } else {
+                lost = true;
+                atomicAdd(&cnt12[face], 1u);
+            }
+        }
+        c[a] = cc;
+    }
+    if (lost) return -1;
+    return (c[2] * K.n[1] + c[1]) * K.n[0] + c[0];
+}
+
+// ---------------------------------------------------------------------
+// one particle: gather (through FR), Boris, deposit (into SINK), move.
+// Returns false when the particle was skipped as out of bounds.
+// ---------------------------------------------------------------------
+template <int M, typename FR, typename SINK>
+__device__ __forceinline__ bool push32_core(int cx, int cy, int cz, int m, float* f, float* u,
+                                            float w, const Consts32& K, FR fr, SINK& sink,
+                                            int& newcell, unsigned* cnt12) {
+    const int Cx = cx + m - 1, Cy = cy + m - 1, Cz = cz + m - 1;  // buffer index of the cell
+    float E[3], Bf[3];
+    gather32<M>(Cx, Cy, Cz, f[0], f[1], f[2], E, Bf, fr);
+    boris32(u, E, Bf, K.qm);
+    const float ig = rsqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    float d[3] = {u[0] * ig * K.cfx, u[1] * ig * K.cfy, u[2] * ig * K.cfz};
+    // a non-finite momentum would put the deposit box outside the arrays
+    if (!(fabsf(d[0]) < 1.0f && fabsf(d[1]) < 1.0f && fabsf(d[2]) < 1.0f)) return false;
+    float f1[3] = {f[0] + d[0], f[1] + d[1], f[2] + d[2]};
+    esirkepov32<M>(K.q * w, f, f1, K, sink);
+    newcell = move32(cx, cy, cz, f, d, K, cnt12);
+    return true;
+}
+
+// ---------------------------------------------------------------------
+// tiled push: one workgroup per (tile, chunk) work item; field tile and
+// current/charge accumulators in LDS
+// ---------------------------------------------------------------------
+template <int M, int TX, int TY, int TZ, typename ACC>
+struct Box32 {
+    static constexpr int H = M / 2;
+    // field tile: buffer indices [T0 + m - 1 - H, T0 + T + m - 1 + H]
+    static constexpr int FX = TX + M + 1, FY = TY + M + 1, FZ = TZ + M + 1;
+    static constexpr int FVOL = FX * FY * FZ;
+    // accumulator box: [T0 + m - 2 - H, T0 + T + m + H]
+    static constexpr int AX = TX + M + 3, AY = TY + M + 3, AZ = TZ + M + 3;
+    static constexpr int AVOL = AX * AY * AZ;
+    // accumulators first (8-byte aligned for ACC = double), then fields
+    static constexpr size_t LDS = (size_t)4 * AVOL * sizeof(ACC) + (size_t)6 * FVOL * sizeof(float);
+};
+
+struct Fields32 {
+    const float *ex, *ey, *ez, *bx, *by, *bz;
+    float *jx, *jy, *jz, *rho;
+};
+
+template <int M, int TX, int TY, int TZ, typename ACC>
+__global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid g, Consts32 K,
+                                                      Tiling tl, const int64_t* __restrict__ tile_off,
+                                                      const int64_t* __restrict__ work,
+                                                      const int64_t* __restrict__ nwork, int chunk,
+                                                      int64_t* __restrict__ cnt) {
+    using B = Box32<M, TX, TY, TZ, ACC>;
+    extern __shared__ double lds_d[];
+    ACC* acc = reinterpret_cast<ACC*>(lds_d);             // 4 * AVOL accumulators
+    float* fld = reinterpret_cast<float*>(acc + 4 * B::AVOL);  // 6 * FVOL field tile
+    __shared__ unsigned cnt12[12];
+    const int64_t wi = blockIdx.x;
+    if (wi >= *nwork) return;
+    const int64_t item = work[wi];
+    const int tile = (int)(item >> 20);
+    const int64_t cidx = item & ((1 << 20) - 1);
+    const int64_t t_end = tile_off[tile + 1];
+    const int64_t beg = tile_off[tile] + cidx * chunk;
+    const int64_t end = min(t_end, beg + chunk);
+    const int txi = tile % tl.nt[0], tyi = (tile / tl.nt[0]) % tl.nt[1], tzi = tile / (tl.nt[0] * tl.nt[1]);
+    const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
+    const int m = g.m;
+    // field tile origin (buffer indices); reads past a partial last tile are clamped
+    const int fox = T0x + m - 1 - B::H, foy = T0y + m - 1 - B::H, foz = T0z + m - 1 - B::H;
+    const float* fsrc[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
+    for (int e = threadIdx.x; e < 6 * B::FVOL; e += blockDim.x) {
+        const int c = e / B::FVOL, r = e % B::FVOL;
+        const int lx = r % B::FX, ly = (r / B::FX) % B::FY, lz = r / (B::FX * B::FY);
+        const int gx = min(fox + lx, g.fe[0] - 1), gy = min(foy + ly, g.fe[1] - 1), gz = min(foz + lz, g.fe[2] - 1);
+        fld[e] = fsrc[c][g.fi(gz, gy, gx)];
+    }
+    for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) acc[e] = ACC(0);
+    if (threadIdx.x < 12) cnt12[threadIdx.x] = 0;
+    __syncthreads();
+    const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
+    int64_t oob = 0;
+    uint32_t strays = 0;  // iterations whose particle was outside the tile
+    int it = 0;
+    const int len = (int)(end - beg), stride = chunk_stride(len);
+    for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
+        const int64_t p = beg + (int)(((int64_t)t * stride) % len);
+        const int cell = P.cell[p];
+        if (cell < 0) continue;
+        const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
+        if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
+            (unsigned)(cz - T0z) >= (unsigned)TZ) {
+            strays |= 1u << it;  // moved out of the tile since the last sort
+            continue;
+        }
+        float f[3] = {P.fx[p], P.fy[p], P.fz[p]};
+        float u[3] = {P.ux[p], P.uy[p], P.uz[p]};
+        // field reads relative to the LDS tile; deposits relative to the
+        // accumulator box (an in-tile particle's boxes lie inside both)
+        const int fcell = ((cz + m - 1 - foz) * B::FY + (cy + m - 1 - foy)) * B::FX + (cx + m - 1 - fox);
+        const float* fb = fld + fcell;
+        auto fr = [&](int c, int k, int j, int i) -> float {
+            return fb[c * B::FVOL + ((k - (cz + m - 1)) * B::FY + (j - (cy + m - 1))) * B::FX + (i - (cx + m - 1))];
+        };
+        LdsSink32<B::AX, B::AY, B::AVOL, ACC> sink{acc, K.inv_ufix,
+                                              ((cz + m - 1 - aoz) * B::AY + (cy + m - 1 - aoy)) * B::AX + (cx + m - 1 - aox), 0};
+        int nc;
+        if (!push32_core<M>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12)) { oob++; continue; }
+        P.cell[p] = nc;
+        P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
+        P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
+    }
+    if (strays) {  // HBM path for the particles outside the tile
+        float* jdst[4] = {F.jx, F.jy, F.jz, F.rho};
+        it = 0;
+        for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
+            if (!(strays & (1u << it))) continue;
+            const int64_t p = beg + (int)(((int64_t)t * stride) % len);
+            const int cell = P.cell[p];
+            const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
+            float f[3] = {P.fx[p], P.fy[p], P.fz[p]};
+            float u[3] = {P.ux[p], P.uy[p], P.uz[p]};
+            auto fr = [&](int c, int k, int j, int i) -> float { return fsrc[c][g.fi(k, j, i)]; };
+            HbmSink32 sink{{jdst[0], jdst[1], jdst[2], jdst[3]}, g, cz + m - 1, cy + m - 1, cx + m - 1, 0, 0, 0};
+            int nc;
+            if (!push32_core<M>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12)) { oob++; continue; }
+            P.cell[p] = nc;
+            P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
+            P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
+        }
+    }
+    __syncthreads();
+    float* jdst[4] = {F.jx, F.jy, F.jz, F.rho};
+    for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) {
+        const ACC a = acc[e];
+        if (a == ACC(0)) continue;
+        const int c = e / B::AVOL, r = e % B::AVOL;
+        const float v = sizeof(ACC) == 8 ? (float)a : (float)a * K.ufix[c];
+        const int gx = aox + r % B::AX, gy = aoy + (r / B::AX) % B::AY, gz = aoz + r / (B::AX * B::AY);
+        if (gx > g.je[0] - 2 || gy > g.je[1] - 2 || gz > g.je[2] - 2) continue;
+        hbm_add(&jdst[c][g.ji(gz, gy, gx)], v);
+    }
+    if (threadIdx.x < 12 && cnt12[threadIdx.x])
+        atomicAdd((unsigned long long*)&cnt[CNT_PEND + threadIdx.x], (unsigned long long)cnt12[threadIdx.x]);
+    if (oob) atomicAdd((unsigned long long*)&cnt[CNT_OOB], (unsigned long long)oob);
+}
+
+// direct push: any particle order, gather from HBM, global atomics
+template <int M>
+__global__ __launch_bounds__(256) void k_push32_direct(Soa32 P, Fields32 F, Grid g, Consts32 K,
+                                                       int64_t* __restrict__ cnt) {
+    __shared__ unsigned cnt12[12];
+    if (threadIdx.x < 12) cnt12[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t n = cnt[CNT_SLOTS];
+    const float* fsrc[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
+    const int m = g.m;
+    int64_t oob = 0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const int cell = P.cell[p];
+        if (cell < 0) continue;
+        const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
+        float f[3] = {P.fx[p], P.fy[p], P.fz[p]};
+        float u[3] = {P.ux[p], P.uy[p], P.uz[p]};
+        auto fr = [&](int c, int k, int j, int i) -> float { return fsrc[c][g.fi(k, j, i)]; };
+        HbmSink32 sink{{F.jx, F.jy, F.jz, F.rho}, g, cz + m - 1, cy + m - 1, cx + m - 1, 0, 0, 0};
+        int nc;
+        if (!push32_core<M>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12)) { oob++; continue; }
+        P.cell[p] = nc;
+        P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
+        P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
+    }
+    __syncthreads();
+    if (threadIdx.x < 12 && cnt12[threadIdx.x])
+        atomicAdd((unsigned long long*)&cnt[CNT_PEND + threadIdx.x], (unsigned long long)cnt12[threadIdx.x]);
+    if (oob) atomicAdd((unsigned long long*)&cnt[CNT_OOB], (unsigned long long)oob);
+}
+
+// ---------------------------------------------------------------------
+// tile sort (fp32 layout): drops removed particles (cell < 0)
+// ---------------------------------------------------------------------
+struct TileArgs32 {
+    int n[3];
+    Tiling tl;
+};
+
+__device__ __forceinline__ int tile_of32(int cell, const TileArgs32& A) {
+    const int cx = cell % A.n[0], cy = (cell / A.n[0]) % A.n[1], cz = cell / (A.n[0] * A.n[1]);
+    return ((cz / A.tl.t[2]) * A.tl.nt[1] + cy / A.tl.t[1]) * A.tl.nt[0] + cx / A.tl.t[0];
+}
+
+__global__ __launch_bounds__(256) void k_hist32(const int32_t* __restrict__ cellv,
+                                                 const int64_t* __restrict__ cnt,
+                                                 int64_t* __restrict__ tile_cnt, TileArgs32 T) {
+    const int64_t n = cnt[CNT_SLOTS];
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = b + threadIdx.x;
+        const int cell = p < n ? cellv[p] : -1;
+        const bool valid = cell >= 0;
+        const int key = valid ? tile_of32(cell, T) : -1;
+        int head, len;
+        wave_runs(key, valid, head, len);
+        if (valid && head == lane_id()) atomicAdd((unsigned long long*)&tile_cnt[key], (unsigned long long)len);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int64_t* __restrict__ cnt,
+                                                    int64_t* __restrict__ tile_cur, TileArgs32 T) {
+    const int64_t n = cnt[CNT_N - 1];
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = b + threadIdx.x;
+        const int cell = p < n ? A.cell[p] : -1;
+        const bool valid = cell >= 0;
+        const int key = valid ? tile_of32(cell, T) : -1;
+        int head, len;
+        wave_runs(key, valid, head, len);
+        int64_t base = 0;
+        if (valid && head == lane_id())
+            base = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key], (unsigned long long)len);
+        base = __shfl(base, head);
+        if (valid) {
+            const int64_t q = base + (lane_id() - head);
+            Bo.cell[q] = cell;
+            Bo.fx[q] = A.fx[p]; Bo.fy[q] = A.fy[p]; Bo.fz[q] = A.fz[p];
+            Bo.ux[q] = A.ux[p]; Bo.uy[q] = A.uy[p]; Bo.uz[q] = A.uz[p];
+            Bo.w[q] = A.w[p];
+            Bo.id[q] = A.id[p];
+        }
+    }
+}
+
+int sort32(pic_ctx* c, int s) {
+    Species& S = c->sp[s];
+    int r = launch_fill(c, c->tile_cnt, (int64_t)c->tl.ntiles * 8);
+    if (r) return r;
+    TileArgs32 T;
+    for (int a = 0; a < 3; a++) T.n[a] = c->g.n[a];
+    T.tl = c->tl;
+    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 32);
+    k_hist32<<<blocks, 256, 0, c->stream>>>(S.cell[S.cur], S.d_cnt, c->tile_cnt, T);
+    PIC_LAUNCH_CHECK(c);
+    PIC_HIP(c, hipMemcpyAsync(S.d_cnt + (CNT_N - 1), S.d_cnt + CNT_SLOTS, 8,
+                              hipMemcpyDeviceToDevice, c->stream));
+    extern int launch_scan_tiles(pic_ctx*, Species&);
+    r = launch_scan_tiles(c, S);
+    if (r) return r;
+    k_scatter32<<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), soa32(S, 1 - S.cur), S.d_cnt,
+                                               c->tile_cur, T);
+    PIC_LAUNCH_CHECK(c);
+    S.cur = 1 - S.cur;
+    S.sorted = 1;
+    S.pushes_since_sort = 0;
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// boundary (fp32): the push already moved and classified every particle;
+// publish its pending counts as this call's counts.
+// ---------------------------------------------------------------------
+__global__ void k_boundary32(int64_t* __restrict__ cnt) {
+    const int i = threadIdx.x;
+    if (i >= 12) return;
+    const int64_t v = cnt[CNT_PEND + i];
+    cnt[CNT_PEND + i] = 0;
+    cnt[(i < 6 ? CNT_LOST_STEP : CNT_WRAP_STEP) + i % 6] = v;
+    cnt[(i < 6 ? CNT_LOST : CNT_WRAP) + i % 6] += v;
+    if (i < 6 && v) atomicAdd((unsigned long long*)&cnt[CNT_LIVE], (unsigned long long)(-v));
+}
+
+int boundary32(pic_ctx* c, int s) {
+    k_boundary32<<<1, 64, 0, c->stream>>>(c->sp[s].d_cnt);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// loader (fp32 layout) from the shared seeded generator
+// ---------------------------------------------------------------------
+struct LoaderArgs32 {
+    uint64_t seed;
+    int64_t id_offset;
+    double u_th, wt;
+    int n[3];
+};
+
+__host__ __device__ inline void to_cell_offset(double v, int n, int& c, float& f) {
+    c = (int)floor(v);
+    double r = v - (double)c;
+    f = (float)r;
+    if (f >= 1.0f) { f = 0.0f; c += 1; }
+    if (c >= n) c = n - 1, f = 1.0f - 5.96e-8f;  // only x == n exactly
+    if (c < 0) c = 0, f = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_loader32(Soa32 A, const int64_t* __restrict__ prefix,
+                                                   int64_t n, LoaderArgs32 L) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        int32_t ix, iy, iz;
+        pic_locate(p, prefix, L.n[0], L.n[1], &ix, &iy, &iz);
+        const uint64_t pid = (uint64_t)(L.id_offset + p);
+        double x, y, z, ux, uy, uz;
+        pic_load_one(L.seed, pid, ix, iy, iz, L.u_th, &x, &y, &z, &ux, &uy, &uz);
+        int cx, cy, cz;
+        float fx, fy, fz;
+        to_cell_offset(x, L.n[0], cx, fx);
+        to_cell_offset(y, L.n[1], cy, fy);
+        to_cell_offset(z, L.n[2], cz, fz);
+        A.cell[p] = (cz * L.n[1] + cy) * L.n[0] + cx;
+        A.fx[p] = fx; A.fy[p] = fy; A.fz[p] = fz;
+        A.ux[p] = (float)ux; A.uy[p] = (float)uy; A.uz[p] = (float)uz;
+        A.w[p] = (float)L.wt;
+        A.id[p] = (int64_t)pid;
+    }
+}
+
+int loader32(pic_ctx* c, int s, const pic_loader* l, int64_t n, const int64_t* d_prefix) {
+    Species& S = c->sp[s];
+    LoaderArgs32 L{l->seed, l->id_offset, l->u_th, l->density / l->ppc, {c->g.n[0], c->g.n[1], c->g.n[2]}};
+    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(n, 1), 256), 256 * 64);
+    k_loader32<<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), d_prefix, n, L);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// diagnostics (fp32 particles, fp64 sums)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_diag_p32(Soa32 A, const int64_t* __restrict__ cnt, double mass,
+                                                   int s, double emax, double* __restrict__ buf) {
+    __shared__ double hist[PIC_SPECTRUM_BINS];
+    __shared__ double ke_s;
+    for (int b = threadIdx.x; b < PIC_SPECTRUM_BINS; b += blockDim.x) hist[b] = 0;
+    if (threadIdx.x == 0) ke_s = 0;
+    __syncthreads();
+    const int64_t n = cnt[CNT_SLOTS];
+    double ke = 0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        if (A.cell[p] < 0) continue;
+        const double ux = A.ux[p], uy = A.uy[p], uz = A.uz[p], wp = A.w[p];
+        const double u2 = ux * ux + uy * uy + uz * uz;
+        const double gm1 = u2 / (sqrt(1.0 + u2) + 1.0);
+        ke += mass * wp * gm1;
+        if (s == 0) {
+            const int b = (int)(gm1 / emax * PIC_SPECTRUM_BINS);
+            if (b >= 0 && b < PIC_SPECTRUM_BINS) lds_add(&hist[b], wp);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off);
+    if (lane_id() == 0) atomicAdd(&ke_s, ke);
+    __syncthreads();
+    if (threadIdx.x == 0) hbm_add(&buf[8 + s], ke_s);
+    if (s == 0)
+        for (int b = threadIdx.x; b < PIC_SPECTRUM_BINS; b += blockDim.x)
+            if (hist[b] != 0) hbm_add(&buf[12 + b], hist[b]);
+}
+
+int diag_p32(pic_ctx* c, int s, double emax) {
+    Species& S = c->sp[s];
+    const int blocks = (int)std::min<int64_t>(blocks_for(S.cap, 256), 256 * 16);
+    k_diag_p32<<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), S.d_cnt, S.m, s, emax, c->diag_buf);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// push launcher
+// ---------------------------------------------------------------------
+static Consts32 consts32(const pic_ctx* c, const Species& S) {
+    Consts32 K;
+    K.qm = (float)(S.q / S.m * (c->p.dt / 2.0));
+    K.q = (float)S.q;
+    K.cfx = (float)(c->p.dt / c->p.dx);
+    K.cfy = (float)(c->p.dt / c->p.dy);
+    K.cfz = (float)(c->p.dt / c->p.dz);
+    K.cjx = (float)(c->p.dx / c->p.dt);
+    K.cjy = (float)(c->p.dy / c->p.dt);
+    K.cjz = (float)(c->p.dz / c->p.dt);
+    for (int a = 0; a < 3; a++) { K.n[a] = c->g.n[a]; K.per[a] = c->p.periodic[a]; }
+    K.m = c->g.m;
+    const double qw = std::fabs(S.q) * (S.wmax > 0 ? S.wmax : 1.0);
+    const double bound[4] = {qw * c->p.dx / c->p.dt, qw * c->p.dy / c->p.dt, qw * c->p.dz / c->p.dt, qw};
+    for (int k = 0; k < 4; k++) {
+        const double u = std::ldexp(bound[k], -18);
+        K.ufix[k] = (float)u;
+        K.inv_ufix[k] = (float)(1.0 / u);
+    }
+    return K;
+}
+
+template <int M, int KIND, typename ACC>
+static int launch_tiled32(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
+    constexpr int TX = TileShape<KIND, M>::X, TY = TileShape<KIND, M>::Y, TZ = TileShape<KIND, M>::Z;
+    using B = Box32<M, TX, TY, TZ, ACC>;
+    const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push32_tiled<M, TX, TY, TZ, ACC>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
+    k_push32_tiled<M, TX, TY, TZ, ACC><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
+        soa32(S, S.cur), Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, S.d_cnt);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+int launch_push32(pic_ctx* c, int s) {
+    Species& S = c->sp[s];
+    Fields32 Fd{(const float*)c->F[0], (const float*)c->F[1], (const float*)c->F[2],
+                (const float*)c->F[3], (const float*)c->F[4], (const float*)c->F[5],
+                (float*)c->J[0], (float*)c->J[1], (float*)c->J[2], (float*)c->rho[s]};
+    Consts32 K = consts32(c, S);
+    const int kernel = c->push_kernel == 0 ? 2 : c->push_kernel;
+    if (kernel == 1) {
+        PhaseTimer t(c, PIC_T_PUSH);
+        const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 64);
+        if (c->g.m == 2) k_push32_direct<2><<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), Fd, c->g, K, S.d_cnt);
+        else k_push32_direct<4><<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), Fd, c->g, K, S.d_cnt);
+        PIC_LAUNCH_CHECK(c);
+        return PIC_OK;
+    }
+    {
+        PhaseTimer t(c, PIC_T_SORT);
+        if (!S.sorted || S.pushes_since_sort >= c->sort_interval) {
+            int r = sort32(c, s);
+            if (r) return r;
+        }
+        int r = build_work(c, s, PIC_PUSH_CHUNK);
+        if (r) return r;
+    }
+    S.pushes_since_sort++;
+    PhaseTimer t(c, PIC_T_PUSH);
+    if (c->fix32) {
+        if (c->g.m == 2) return launch_tiled32<2, 2, int>(c, S, Fd, K);
+        return launch_tiled32<4, 2, int>(c, S, Fd, K);
+    }
+    if (c->g.m == 2) return launch_tiled32<2, 1, double>(c, S, Fd, K);
+    return launch_tiled32<4, 1, double>(c, S, Fd, K);
+}
 
 }  // namespace pic
 
-int particles_set32(pic_ctx* c, int, int64_t, const double*, const double*, const double*,
-                    const double*, const double*, const double*, const double*, const int64_t*) {
-    return pic::set_err(c, PIC_ERR_STATE, "the FP32 form is not built yet");
+// ---------------------------------------------------------------------
+// host-side conversion for pic_particles_set / pic_particles_get
+// ---------------------------------------------------------------------
+using namespace pic;
+
+int particles_set32(pic_ctx* c, int s, int64_t n, const double* x, const double* y, const double* z,
+                    const double* ux, const double* uy, const double* uz, const double* w,
+                    const int64_t* id) {
+    Species& S = c->sp[s];
+    std::vector<int32_t> cell(n);
+    std::vector<float> a[7];
+    for (auto& v : a) v.resize(n);
+    for (int64_t q = 0; q < n; q++) {
+        // a periodic coordinate outside [0, n) is wrapped first (the
+        // reference wraps it at the next push)
+        double v[3] = {x[q], y[q], z[q]};
+        int cc[3];
+        float ff[3];
+        for (int ax = 0; ax < 3; ax++) {
+            const int N = c->g.n[ax];
+            if (c->p.periodic[ax]) {
+                if (v[ax] < 0.0) v[ax] += N;
+                if (v[ax] >= N) v[ax] -= N;
+            }
+            if (!(v[ax] >= 0.0 && v[ax] <= N))
+                return set_err(c, PIC_ERR_ARG, "particle %lld outside the domain", (long long)q);
+            to_cell_offset(v[ax], N, cc[ax], ff[ax]);
+        }
+        cell[q] = (cc[2] * c->g.n[1] + cc[1]) * c->g.n[0] + cc[0];
+        a[0][q] = ff[0]; a[1][q] = ff[1]; a[2][q] = ff[2];
+        a[3][q] = (float)ux[q]; a[4][q] = (float)uy[q]; a[5][q] = (float)uz[q];
+        a[6][q] = (float)w[q];
+    }
+    const int b = S.cur;
+    float* dst[7] = {S.fx[b], S.fy[b], S.fz[b], S.vx[b], S.vy[b], S.vz[b], S.fw[b]};
+    if (n) {
+        PIC_HIP(c, hipMemcpyAsync(S.cell[b], cell.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+        for (int f = 0; f < 7; f++)
+            PIC_HIP(c, hipMemcpyAsync(dst[f], a[f].data(), n * 4, hipMemcpyHostToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.id[b], id, n * 8, hipMemcpyHostToDevice, c->stream));
+    }
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
 }
-int particles_get32(pic_ctx* c, int, int64_t, std::vector<double>*, std::vector<int64_t>&) {
-    return pic::set_err(c, PIC_ERR_STATE, "the FP32 form is not built yet");
+
+int particles_get32(pic_ctx* c, int s, int64_t slots, std::vector<double>* a, std::vector<int64_t>& id) {
+    Species& S = c->sp[s];
+    const int b = S.cur;
+    std::vector<int32_t> cell(slots);
+    std::vector<float> t[7];
+    float* src[7] = {S.fx[b], S.fy[b], S.fz[b], S.vx[b], S.vy[b], S.vz[b], S.fw[b]};
+    if (slots) {
+        PIC_HIP(c, hipMemcpyAsync(cell.data(), S.cell[b], slots * 4, hipMemcpyDeviceToHost, c->stream));
+        for (int f = 0; f < 7; f++) {
+            t[f].resize(slots);
+            PIC_HIP(c, hipMemcpyAsync(t[f].data(), src[f], slots * 4, hipMemcpyDeviceToHost, c->stream));
+        }
+        PIC_HIP(c, hipMemcpyAsync(id.data(), S.id[b], slots * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    for (int f = 0; f < 10; f++) a[f].assign(slots, 0.0);
+    const int nx = c->g.n[0], ny = c->g.n[1];
+    for (int64_t p = 0; p < slots; p++) {
+        if (cell[p] < 0) {  // removed
+            a[6][p] = __builtin_nan("");
+            continue;
+        }
+        const int cx = cell[p] % nx, cy = (cell[p] / nx) % ny, cz = cell[p] / (nx * ny);
+        a[0][p] = cx + (double)t[0][p];
+        a[1][p] = cy + (double)t[1][p];
+        a[2][p] = cz + (double)t[2][p];
+        for (int f = 3; f < 7; f++) a[f][p] = t[f][p];
+    }
+    return PIC_OK;
 }

@@ -15,6 +15,7 @@
 //    particle list; currents are accumulated in LDS (ds_add_f64) over the
 //    tile plus deposit halo and flushed to HBM once per workgroup.
 #include "pic_internal.h"
+#include "atomics.h"
 #include "shapes.h"
 
 namespace pic {
@@ -52,8 +53,24 @@ __device__ __forceinline__ bool wrap_check(double& x, double& y, double& z, cons
     return x >= 0.0 && x <= K.N[0] && y >= 0.0 && y <= K.N[1] && z >= 0.0 && z <= K.N[2];
 }
 
+// Rolled loops over a small per-particle weight array read element 0 of a
+// register copy and shift the copy down by one each iteration: only
+// compile-time indices, so the arrays stay in registers (no scratch).
+template <int L>
+__device__ __forceinline__ void shift_down(double* A) {
+#pragma unroll
+    for (int t = 0; t < L - 1; t++) A[t] = A[t + 1];
+}
+template <int L>
+__device__ __forceinline__ void copy_arr(const double* S, double* D) {
+#pragma unroll
+    for (int t = 0; t < L; t++) D[t] = S[t];
+}
+
 // Gather + Boris + advance for one particle.  Field reads go through the
-// functor FR(comp, k, j, i) so the same code serves HBM and LDS tiles.
+// functor FR(comp, k, j, i).  Each component's sum runs k -> j -> i with
+// terms ((Wx*Wy)*Wz)*F, the reference's order (particle.cpp:207-259); the
+// k loop is rolled to keep the kernel inside the instruction cache.
 template <int M, typename FR>
 __device__ __forceinline__ void gather_boris(double x, double y, double z, double& ux, double& uy,
                                              double& uz, double& xn, double& yn, double& zn,
@@ -67,44 +84,28 @@ __device__ __forceinline__ void gather_boris(double x, double y, double z, doubl
     shape_at<M>(yc - 0.5, Wyd);
     shape_at<M>(zc - 0.5, Wzd);
     const int id = (int)xc, jd = (int)yc, kd = (int)zc;
-    double e0 = 0.0, e1 = 0.0, e2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
-    // particle.cpp:207-259: sum_k sum_j sum_i ((Wx*Wy)*Wz)*F
+    auto comp = [&](int c, const double* WX, const double* WY, const double* WZ, int b0, int b1,
+                    int b2) -> double {
+        double e = 0.0;
+        double wzr[M + 1];
+        copy_arr<M + 1>(WZ, wzr);
+#pragma unroll 1
+        for (int k = 0; k <= M; k++) {
+            const double wz = wzr[0];
 #pragma unroll
-    for (int k = 0; k <= M; k++)
+            for (int j = 0; j <= M; j++)
 #pragma unroll
-        for (int j = 0; j <= M; j++)
-#pragma unroll
-            for (int i = 0; i <= M; i++) e0 += Wxd[i] * Wyp[j] * Wzp[k] * fr(0, kp + k - H, jp + j - H, id + i - H);
-#pragma unroll
-    for (int k = 0; k <= M; k++)
-#pragma unroll
-        for (int j = 0; j <= M; j++)
-#pragma unroll
-            for (int i = 0; i <= M; i++) e1 += Wxp[i] * Wyd[j] * Wzp[k] * fr(1, kp + k - H, jd + j - H, ip + i - H);
-#pragma unroll
-    for (int k = 0; k <= M; k++)
-#pragma unroll
-        for (int j = 0; j <= M; j++)
-#pragma unroll
-            for (int i = 0; i <= M; i++) e2 += Wxp[i] * Wyp[j] * Wzd[k] * fr(2, kd + k - H, jp + j - H, ip + i - H);
-#pragma unroll
-    for (int k = 0; k <= M; k++)
-#pragma unroll
-        for (int j = 0; j <= M; j++)
-#pragma unroll
-            for (int i = 0; i <= M; i++) b0 += Wxp[i] * Wyd[j] * Wzd[k] * fr(3, kd + k - H, jd + j - H, ip + i - H);
-#pragma unroll
-    for (int k = 0; k <= M; k++)
-#pragma unroll
-        for (int j = 0; j <= M; j++)
-#pragma unroll
-            for (int i = 0; i <= M; i++) b1 += Wxd[i] * Wyp[j] * Wzd[k] * fr(4, kd + k - H, jp + j - H, id + i - H);
-#pragma unroll
-    for (int k = 0; k <= M; k++)
-#pragma unroll
-        for (int j = 0; j <= M; j++)
-#pragma unroll
-            for (int i = 0; i <= M; i++) b2 += Wxd[i] * Wyd[j] * Wzp[k] * fr(5, kp + k - H, jd + j - H, id + i - H);
+                for (int i = 0; i <= M; i++) e += WX[i] * WY[j] * wz * fr(c, b2 + k - H, b1 + j - H, b0 + i - H);
+            shift_down<M + 1>(wzr);
+        }
+        return e;
+    };
+    double e0 = comp(0, Wxd, Wyp, Wzp, id, jp, kp);
+    double e1 = comp(1, Wxp, Wyd, Wzp, ip, jd, kp);
+    double e2 = comp(2, Wxp, Wyp, Wzd, ip, jp, kd);
+    double b0 = comp(3, Wxp, Wyd, Wzd, ip, jd, kd);
+    double b1 = comp(4, Wxd, Wyp, Wzd, id, jp, kd);
+    double b2 = comp(5, Wxd, Wyd, Wzp, id, jd, kp);
     // Boris, particle.cpp:271-311
     e0 *= K.qm; e1 *= K.qm; e2 *= K.qm; b0 *= K.qm; b1 *= K.qm; b2 *= K.qm;
     const double px1 = ux + e0, py1 = uy + e1, pz1 = uz + e2;
@@ -129,15 +130,21 @@ __device__ __forceinline__ void gather_boris(double x, double y, double z, doubl
 }
 
 // Esirkepov deposit of one particle moving (xo,yo,zo) -> (xn,yn,zn)
-// (particle.cpp:381-514).  ADD(comp, k, j, i, v) adds v (comp 0..2 = J,
-// 3 = rho).
+// (particle.cpp:381-514).  The sink D receives D.base(bz, by, bx) once
+// (the box origin, buffer indices) and then D(comp, k, j, i, v) with
+// box-relative compile-time (k, j, i); comp 0..2 = J, 3 = rho.
 //
 // The reference walks a box [min(i1,i2)-H, max(i1,i2)+H] per axis (M+1 or
-// M+2 points).  Here the box is always M+2 points with the old/new shape
-// weights placed at compile-time offsets (selected by d0/d1 in {0,1}), so
-// the loops unroll into registers; points past the reference's box are
-// predicated off.  Values and the order of the cumulative sums are the
-// reference's, so each deposited term is bit-identical.
+// M+2 points).  Here the box is always L = M+2 points with the old/new
+// shape weights placed at compile-time offsets (selected by d0/d1 in {0,1}),
+// so the loops unroll into registers with immediate LDS offsets.  Callers
+// guarantee |x_new - x_old| < 1 cell per axis, hence |i2 - i1| <= 1.  Values
+// and the order of each cumulative sum are the reference's, so every term
+// the reference deposits is deposited bit-identically; the extra points are
+// (a) planes outside the reference's box along the non-cumulative axes,
+// whose weights are exactly zero (adding +-0 changes no value), and (b) the
+// last point of a cumulative sum when the particle stays on its node, which
+// is predicated off.
 template <int M>
 __device__ __forceinline__ void place(const double* S, int d, double* A) {
 #pragma unroll
@@ -148,95 +155,30 @@ __device__ __forceinline__ void place(const double* S, int d, double* A) {
     }
 }
 
-template <int M, typename ADD>
-__device__ __noinline__ void esirkepov_generic(double qw, double xo, double yo, double zo,
-                                               double xn, double yn, double zn, double org,
-                                               const PushConsts& K, ADD add) {
-    constexpr int H = M / 2;
-    double Sx0[M + 1], Sx1[M + 1], Sy0[M + 1], Sy1[M + 1], Sz0[M + 1], Sz1[M + 1];
-    const int i1 = shape_at<M>(xo - org, Sx0), i2 = shape_at<M>(xn - org, Sx1);
-    const int j1 = shape_at<M>(yo - org, Sy0), j2 = shape_at<M>(yn - org, Sy1);
-    const int k1 = shape_at<M>(zo - org, Sz0), k2 = shape_at<M>(zn - org, Sz1);
-    const int imin = min(i1, i2) - H, imax = max(i1, i2) + H;
-    const int jmin = min(j1, j2) - H, jmax = max(j1, j2) + H;
-    const int kmin = min(k1, k2) - H, kmax = max(k1, k2) + H;
-    const double cx = qw * K.dx / K.dt, cy = qw * K.dy / K.dt, cz = qw * K.dz / K.dt;
-    auto sv = [](const double* S, int ic, int v) -> double {
-        return (v >= ic - H && v <= ic + H) ? S[v - ic + H] : 0.0;
-    };
-    for (int k = kmin; k <= kmax; k++) {
-        const double sz0 = sv(Sz0, k1, k), dsz = sv(Sz1, k2, k) - sz0;
-        for (int j = jmin; j <= jmax; j++) {
-            const double sy0 = sv(Sy0, j1, j), dsy = sv(Sy1, j2, j) - sy0;
-            const double wyz = sy0 * sz0 + 0.5 * dsy * sz0 + 0.5 * sy0 * dsz + (1.0 / 3.0) * dsy * dsz;
-            double acc = 0.0;
-            for (int i = imin; i <= imax; i++) {
-                acc += sv(Sx1, i2, i) - sv(Sx0, i1, i);
-                add(0, k, j, i, -(cx * acc * wyz));
-            }
-        }
-    }
-    for (int k = kmin; k <= kmax; k++) {
-        const double sz0 = sv(Sz0, k1, k), dsz = sv(Sz1, k2, k) - sz0;
-        for (int i = imin; i <= imax; i++) {
-            const double sx0 = sv(Sx0, i1, i), dsx = sv(Sx1, i2, i) - sx0;
-            const double wxz = sx0 * sz0 + 0.5 * dsx * sz0 + 0.5 * sx0 * dsz + (1.0 / 3.0) * dsx * dsz;
-            double acc = 0.0;
-            for (int j = jmin; j <= jmax; j++) {
-                acc += sv(Sy1, j2, j) - sv(Sy0, j1, j);
-                add(1, k, j, i, -(cy * acc * wxz));
-            }
-        }
-    }
-    for (int j = jmin; j <= jmax; j++) {
-        const double sy0 = sv(Sy0, j1, j), dsy = sv(Sy1, j2, j) - sy0;
-        for (int i = imin; i <= imax; i++) {
-            const double sx0 = sv(Sx0, i1, i), dsx = sv(Sx1, i2, i) - sx0;
-            const double wxy = sx0 * sy0 + 0.5 * dsx * sy0 + 0.5 * sx0 * dsy + (1.0 / 3.0) * dsx * dsy;
-            double acc = 0.0;
-            for (int k = kmin; k <= kmax; k++) {
-                acc += sv(Sz1, k2, k) - sv(Sz0, k1, k);
-                add(2, k, j, i, -(cz * acc * wxy));
-            }
-        }
-    }
-    for (int k = kmin; k <= kmax; k++) {
-        const double sz = sv(Sz1, k2, k);
-        if (sz == 0.0) continue;
-        for (int j = jmin; j <= jmax; j++) {
-            const double sy = sv(Sy1, j2, j);
-            if (sy == 0.0) continue;
-            for (int i = imin; i <= imax; i++) add(3, k, j, i, qw * sv(Sx1, i2, i) * sy * sz);
-        }
-    }
-}
-
-template <int M, typename ADD>
+template <int M, typename D>
 __device__ __forceinline__ void esirkepov(double qw, double xo, double yo, double zo, double xn,
-                                          double yn, double zn, const PushConsts& K, int m,
-                                          ADD add) {
+                                          double yn, double zn, const PushConsts& K, int m, D& dst) {
     constexpr int H = M / 2, L = M + 2;
     const double org = (double)(1 - m);
     double Sx0[M + 1], Sx1[M + 1], Sy0[M + 1], Sy1[M + 1], Sz0[M + 1], Sz1[M + 1];
     const int i1 = shape_at<M>(xo - org, Sx0), i2 = shape_at<M>(xn - org, Sx1);
     const int j1 = shape_at<M>(yo - org, Sy0), j2 = shape_at<M>(yn - org, Sy1);
     const int k1 = shape_at<M>(zo - org, Sz0), k2 = shape_at<M>(zn - org, Sz1);
-    if (abs(i2 - i1) > 1 || abs(j2 - j1) > 1 || abs(k2 - k1) > 1) {
-        // a move of more than one node in a step (dt beyond the Courant limit)
-        esirkepov_generic<M>(qw, xo, yo, zo, xn, yn, zn, org, K, add);
-        return;
-    }
     const int bx = min(i1, i2) - H, by = min(j1, j2) - H, bz = min(k1, k2) - H;
-    const int Lx = abs(i2 - i1) + M + 1, Ly = abs(j2 - j1) + M + 1, Lz = abs(k2 - k1) + M + 1;
+    const bool crx = i1 != i2, cry = j1 != j2, crz = k1 != k2;
     double X0[L], X1[L], Y0[L], Y1[L], Z0[L], Z1[L];
     place<M>(Sx0, i1 - H - bx, X0); place<M>(Sx1, i2 - H - bx, X1);
     place<M>(Sy0, j1 - H - by, Y0); place<M>(Sy1, j2 - H - by, Y1);
     place<M>(Sz0, k1 - H - bz, Z0); place<M>(Sz1, k2 - H - bz, Z1);
     const double cx = qw * K.dx / K.dt, cy = qw * K.dy / K.dt, cz = qw * K.dz / K.dt;
+    dst.base(bz, by, bx);
     // Jx: cumulative sum along x, weights W_yz (particle.cpp:427-450)
-#pragma unroll
+    double za[L], zb[L];
+    copy_arr<L>(Z0, za); copy_arr<L>(Z1, zb);
+#pragma unroll 1
     for (int k = 0; k < L; k++) {
-        const double sz0 = Z0[k], dsz = Z1[k] - sz0;
+        const double sz0 = za[0], dsz = zb[0] - sz0;
+        shift_down<L>(za); shift_down<L>(zb);
 #pragma unroll
         for (int j = 0; j < L; j++) {
             const double sy0 = Y0[j], dsy = Y1[j] - sy0;
@@ -245,14 +187,16 @@ __device__ __forceinline__ void esirkepov(double qw, double xo, double yo, doubl
 #pragma unroll
             for (int i = 0; i < L; i++) {
                 acc += X1[i] - X0[i];
-                if (i < Lx && j < Ly && k < Lz) add(0, bz + k, by + j, bx + i, -(cx * acc * wyz));
+                if (i < L - 1 || crx) dst(0, k, j, i, -(cx * acc * wyz));
             }
         }
     }
     // Jy (particle.cpp:452-475)
-#pragma unroll
+    copy_arr<L>(Z0, za); copy_arr<L>(Z1, zb);
+#pragma unroll 1
     for (int k = 0; k < L; k++) {
-        const double sz0 = Z0[k], dsz = Z1[k] - sz0;
+        const double sz0 = za[0], dsz = zb[0] - sz0;
+        shift_down<L>(za); shift_down<L>(zb);
 #pragma unroll
         for (int i = 0; i < L; i++) {
             const double sx0 = X0[i], dsx = X1[i] - sx0;
@@ -261,14 +205,17 @@ __device__ __forceinline__ void esirkepov(double qw, double xo, double yo, doubl
 #pragma unroll
             for (int j = 0; j < L; j++) {
                 acc += Y1[j] - Y0[j];
-                if (i < Lx && j < Ly && k < Lz) add(1, bz + k, by + j, bx + i, -(cy * acc * wxz));
+                if (j < L - 1 || cry) dst(1, k, j, i, -(cy * acc * wxz));
             }
         }
     }
     // Jz (particle.cpp:477-500)
-#pragma unroll
+    double ya[L], yb[L];
+    copy_arr<L>(Y0, ya); copy_arr<L>(Y1, yb);
+#pragma unroll 1
     for (int j = 0; j < L; j++) {
-        const double sy0 = Y0[j], dsy = Y1[j] - sy0;
+        const double sy0 = ya[0], dsy = yb[0] - sy0;
+        shift_down<L>(ya); shift_down<L>(yb);
 #pragma unroll
         for (int i = 0; i < L; i++) {
             const double sx0 = X0[i], dsx = X1[i] - sx0;
@@ -277,23 +224,71 @@ __device__ __forceinline__ void esirkepov(double qw, double xo, double yo, doubl
 #pragma unroll
             for (int k = 0; k < L; k++) {
                 acc += Z1[k] - Z0[k];
-                if (i < Lx && j < Ly && k < Lz) add(2, bz + k, by + j, bx + i, -(cz * acc * wxy));
+                if (k < L - 1 || crz) dst(2, k, j, i, -(cz * acc * wxy));
             }
         }
     }
-    // rho at the new position (particle.cpp:502-514): nodes i2-H .. i2+H
-#pragma unroll
+    // rho at the new position (particle.cpp:502-514): nodes i2-H .. i2+H.
+    // The reference skips planes whose weight is exactly zero; adding +-0
+    // there changes nothing, so every node is deposited.
+    dst.base(k2 - H, j2 - H, i2 - H);
+    double zr[M + 1];
+    copy_arr<M + 1>(Sz1, zr);
+#pragma unroll 1
     for (int k = 0; k <= M; k++) {
-        const double sz = Sz1[k];
-        if (sz == 0.0) continue;
+        const double sz = zr[0];
+        shift_down<M + 1>(zr);
 #pragma unroll
-        for (int j = 0; j <= M; j++) {
-            const double sy = Sy1[j];
-            if (sy == 0.0) continue;
+        for (int j = 0; j <= M; j++)
 #pragma unroll
-            for (int i = 0; i <= M; i++) add(3, k2 - H + k, j2 - H + j, i2 - H + i, qw * Sx1[i] * sy * sz);
-        }
+            for (int i = 0; i <= M; i++) dst(3, k, j, i, qw * Sx1[i] * Sy1[j] * sz);
     }
+}
+
+// deposit sinks ---------------------------------------------------------
+struct HbmSink {  // global FP64 atomics on the ghosted J/rho arrays
+    double* jp[4];
+    Grid g;
+    int bz, by, bx;
+    __device__ __forceinline__ void base(int z, int y, int x) { bz = z; by = y; bx = x; }
+    __device__ __forceinline__ void operator()(int c, int k, int j, int i, double v) {
+        hbm_add(&jp[c][g.ji(bz + k, by + j, bx + i)], v);
+    }
+};
+
+template <int AX, int AY, int VOL>
+struct LdsSink {  // LDS accumulator box with origin (oz, oy, ox)
+    double* acc;
+    int oz, oy, ox;
+    int off;
+    __device__ __forceinline__ void base(int z, int y, int x) {
+        off = ((z - oz) * AY + (y - oy)) * AX + (x - ox);
+    }
+    __device__ __forceinline__ void operator()(int c, int k, int j, int i, double v) {
+        lds_add(&acc[off + c * VOL + (k * AY + j) * AX + i], v);
+    }
+};
+
+// Full push of one particle against HBM fields and HBM currents (the
+// direct kernel, and the tiled kernel's path for a particle outside its
+// tile).  Returns 1 if the particle was skipped as out of bounds.
+template <int M>
+__device__ __forceinline__ int push_one_hbm(const P64& P, int64_t p, double wp, const Fields64& F,
+                                         const Grid& g, const PushConsts& K) {
+    double x = P.x[p], y = P.y[p], z = P.z[p];
+    if (!wrap_check(x, y, z, K)) return 1;
+    const double* fp[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
+    auto fr = [&](int c, int k, int j, int i) -> double { return fp[c][g.fi(k, j, i)]; };
+    double ux = P.ux[p], uy = P.uy[p], uz = P.uz[p], xn, yn, zn;
+    gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr);
+    P.x[p] = xn; P.y[p] = yn; P.z[p] = zn;
+    P.ux[p] = ux; P.uy[p] = uy; P.uz[p] = uz;
+    if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
+    // a non-finite momentum would put the deposit box outside the arrays
+    if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) return 1;
+    HbmSink d{{F.jx, F.jy, F.jz, F.rho}, g, 0, 0, 0};
+    esirkepov<M>(K.q * wp, x, y, z, xn, yn, zn, K, g.m, d);
+    return 0;
 }
 
 // ---------------------------------------------------------------------
@@ -304,23 +299,10 @@ __global__ __launch_bounds__(256) void k_push64_direct(P64 P, Fields64 F, Grid g
     const int64_t n = P.cnt[CNT_SLOTS];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     int64_t oob = 0;
-    const double* fp[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
-    auto fr = [&](int c, int k, int j, int i) -> double { return fp[c][g.fi(k, j, i)]; };
-    double* jp[4] = {F.jx, F.jy, F.jz, F.rho};
-    auto add = [&](int c, int k, int j, int i, double v) { unsafeAtomicAdd(&jp[c][g.ji(k, j, i)], v); };
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
         const double wp = P.w[p];
         if (!(wp == wp)) continue;
-        double x = P.x[p], y = P.y[p], z = P.z[p];
-        if (!wrap_check(x, y, z, K)) { oob++; continue; }
-        double ux = P.ux[p], uy = P.uy[p], uz = P.uz[p], xn, yn, zn;
-        gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr);
-        P.x[p] = xn; P.y[p] = yn; P.z[p] = zn;
-        P.ux[p] = ux; P.uy[p] = uy; P.uz[p] = uz;
-        if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
-        // a non-finite momentum would put the deposit box outside the arrays
-        if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) { oob++; continue; }
-        esirkepov<M>(K.q * wp, x, y, z, xn, yn, zn, K, g.m, add);
+        oob += push_one_hbm<M>(P, p, wp, F, g, K);
     }
     if (oob) atomicAdd((unsigned long long*)&P.cnt[CNT_OOB], (unsigned long long)oob);
 }
@@ -336,6 +318,10 @@ struct AccBox {
     static constexpr int H = M / 2;
     static constexpr int BX = TX + M + 3, BY = TY + M + 3, BZ = TZ + M + 3;
     static constexpr int VOL = BX * BY * BZ;
+    // field tile: buffer indices [T0 + m - 1 - H, T0 + T + m - 1 + H]
+    static constexpr int FX = TX + M + 1, FY = TY + M + 1, FZ = TZ + M + 1;
+    static constexpr int FVOL = FX * FY * FZ;
+    static constexpr size_t LDS = (size_t)(6 * FVOL + 4 * VOL) * sizeof(double);
 };
 
 template <int M, int TX, int TY, int TZ>
@@ -344,10 +330,12 @@ __global__ __launch_bounds__(256) void k_push64_tiled(P64 P, Fields64 F, Grid g,
                                                       const int64_t* __restrict__ work,
                                                       const int64_t* __restrict__ nwork, int chunk) {
     using B = AccBox<M, TX, TY, TZ>;
-    extern __shared__ double acc[];  // 4 * VOL
+    extern __shared__ double lds64[];
+    double* fld = lds64;               // 6 * FVOL: the tile's fields
+    double* acc = lds64 + 6 * B::FVOL; // 4 * VOL: J and rho accumulators
     const int64_t wi = blockIdx.x;
     if (wi >= *nwork) return;
-    const int64_t item = work[wi];            // tile * 2^20 + chunk index
+    const int64_t item = work[wi];            // tile << 20 | chunk index
     const int tile = (int)(item >> 20);
     const int64_t cidx = item & ((1 << 20) - 1);
     const int64_t t_beg = tile_off[tile], t_end = tile_off[tile + 1];
@@ -356,42 +344,62 @@ __global__ __launch_bounds__(256) void k_push64_tiled(P64 P, Fields64 F, Grid g,
     const int tzi = tile / (tl.nt[0] * tl.nt[1]);
     const int tyi = (tile / tl.nt[0]) % tl.nt[1];
     const int txi = tile % tl.nt[0];
-    // buffer-index origin of the accumulator box
-    const int ox = txi * TX + g.m - 2 - B::H;
-    const int oy = tyi * TY + g.m - 2 - B::H;
-    const int oz = tzi * TZ + g.m - 2 - B::H;
+    const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
+    const double* fp[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
+    const int fox = T0x + g.m - 1 - B::H, foy = T0y + g.m - 1 - B::H, foz = T0z + g.m - 1 - B::H;
+    for (int e = threadIdx.x; e < 6 * B::FVOL; e += blockDim.x) {
+        const int c = e / B::FVOL, r = e % B::FVOL;
+        const int lx = r % B::FX, ly = (r / B::FX) % B::FY, lz = r / (B::FX * B::FY);
+        // reads past a partial last tile are clamped (never used)
+        const int gx = min(fox + lx, g.fe[0] - 1), gy = min(foy + ly, g.fe[1] - 1), gz = min(foz + lz, g.fe[2] - 1);
+        fld[e] = fp[c][g.fi(gz, gy, gx)];
+    }
     for (int e = threadIdx.x; e < 4 * B::VOL; e += blockDim.x) acc[e] = 0.0;
     __syncthreads();
-    const double* fp[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
-    auto fr = [&](int c, int k, int j, int i) -> double { return fp[c][g.fi(k, j, i)]; };
-    int64_t oob = 0, spill = 0;
-    double* jp[4] = {F.jx, F.jy, F.jz, F.rho};
-    auto add = [&](int c, int k, int j, int i, double v) {
-        const int lx = i - ox, ly = j - oy, lz = k - oz;
-        if ((unsigned)lx < (unsigned)B::BX && (unsigned)ly < (unsigned)B::BY &&
-            (unsigned)lz < (unsigned)B::BZ) {
-            atomicAdd(&acc[c * B::VOL + (lz * B::BY + ly) * B::BX + lx], v);
-        } else {  // outside the box (only for a particle outside its tile)
-            unsafeAtomicAdd(&jp[c][g.ji(k, j, i)], v);
-            spill++;
-        }
+    // gather from the LDS field tile (an in-tile particle's stencil lies in it)
+    auto fr = [&](int c, int k, int j, int i) -> double {
+        return fld[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)];
     };
-    for (int64_t p = beg + threadIdx.x; p < end; p += blockDim.x) {
+    LdsSink<B::BX, B::BY, B::VOL> sink{acc, T0z + g.m - 2 - B::H, T0y + g.m - 2 - B::H,
+                                      T0x + g.m - 2 - B::H, 0};
+    int64_t oob = 0;
+    uint32_t strays = 0;  // iterations whose particle was outside the tile
+    int it = 0;
+    const int len = (int)(end - beg), stride = chunk_stride(len);
+    for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
+        const int64_t p = beg + (int)(((int64_t)t * stride) % len);
         const double wp = P.w[p];
         if (!(wp == wp)) continue;
         double x = P.x[p], y = P.y[p], z = P.z[p];
         if (!wrap_check(x, y, z, K)) { oob++; continue; }
+        // cell after the wrap (x == N belongs to the last cell)
+        const int cx = min((int)x, g.n[0] - 1), cy = min((int)y, g.n[1] - 1), cz = min((int)z, g.n[2] - 1);
+        if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
+            (unsigned)(cz - T0z) >= (unsigned)TZ) {
+            strays |= 1u << it;  // left its tile since the last sort: HBM pass below
+            continue;
+        }
         double ux = P.ux[p], uy = P.uy[p], uz = P.uz[p], xn, yn, zn;
         gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr);
         P.x[p] = xn; P.y[p] = yn; P.z[p] = zn;
         P.ux[p] = ux; P.uy[p] = uy; P.uz[p] = uz;
         if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
-        // a non-finite momentum would put the deposit box outside the arrays
         if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) { oob++; continue; }
-        esirkepov<M>(K.q * wp, x, y, z, xn, yn, zn, K, g.m, add);
+        // in-tile particle, move < 1 cell: its whole box lies in the LDS box
+        esirkepov<M>(K.q * wp, x, y, z, xn, yn, zn, K, g.m, sink);
+    }
+    if (strays) {
+        it = 0;
+        for (int t = threadIdx.x; t < len; t += blockDim.x, it++)
+            if (strays & (1u << it)) {
+                const int64_t p = beg + (int)(((int64_t)t * stride) % len);
+                oob += push_one_hbm<M>(P, p, P.w[p], F, g, K);
+            }
     }
     __syncthreads();
     // flush: one global atomic per nonzero accumulator point
+    double* jp[4] = {F.jx, F.jy, F.jz, F.rho};
+    const int ox = T0x + g.m - 2 - B::H, oy = T0y + g.m - 2 - B::H, oz = T0z + g.m - 2 - B::H;
     for (int e = threadIdx.x; e < 4 * B::VOL; e += blockDim.x) {
         const double v = acc[e];
         if (v == 0.0) continue;
@@ -401,10 +409,9 @@ __global__ __launch_bounds__(256) void k_push64_tiled(P64 P, Fields64 F, Grid g,
         // a partial last tile's box reaches past the arrays; those points
         // are never deposited into, but keep the flush in bounds regardless
         if (gx > g.je[0] - 2 || gy > g.je[1] - 2 || gz > g.je[2] - 2) continue;
-        unsafeAtomicAdd(&jp[c][g.ji(gz, gy, gx)], v);
+        hbm_add(&jp[c][g.ji(gz, gy, gx)], v);
     }
     if (oob) atomicAdd((unsigned long long*)&P.cnt[CNT_OOB], (unsigned long long)oob);
-    (void)spill;
 }
 
 // ---------------------------------------------------------------------
@@ -420,12 +427,30 @@ static PushConsts make_consts(const pic_ctx* c, const Species& S) {
     return K;
 }
 
+// Particle pointers of the species' current buffer (read after any sort,
+// which swaps the buffers).
+static P64 p64(pic_ctx* c, Species& S) {
+    const int cur = S.cur;
+    return P64{S.x[cur], S.y[cur], S.z[cur], S.ux[cur], S.uy[cur], S.uz[cur], S.w[cur],
+               c->keep_old ? S.xo[cur] : nullptr, c->keep_old ? S.yo[cur] : nullptr,
+               c->keep_old ? S.zo[cur] : nullptr, S.d_cnt};
+}
+
+template <int M>
+static int launch_tiled64(pic_ctx* c, Species& S, const P64& P, const Fields64& Fd,
+                          const PushConsts& K, int64_t max_items) {
+    using TS = TileShape<0, M>;
+    using B = AccBox<M, TS::X, TS::Y, TS::Z>;
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push64_tiled<M, TS::X, TS::Y, TS::Z>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
+    k_push64_tiled<M, TS::X, TS::Y, TS::Z><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
+        P, Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
 int launch_push64(pic_ctx* c, int s) {
     Species& S = c->sp[s];
-    const int cur = S.cur;
-    P64 P{S.x[cur], S.y[cur], S.z[cur], S.ux[cur], S.uy[cur], S.uz[cur], S.w[cur],
-          c->keep_old ? S.xo[cur] : nullptr, c->keep_old ? S.yo[cur] : nullptr,
-          c->keep_old ? S.zo[cur] : nullptr, S.d_cnt};
     Fields64 Fd{(const double*)c->F[0], (const double*)c->F[1], (const double*)c->F[2],
                 (const double*)c->F[3], (const double*)c->F[4], (const double*)c->F[5],
                 (double*)c->J[0], (double*)c->J[1], (double*)c->J[2], (double*)c->rho[s]};
@@ -434,13 +459,14 @@ int launch_push64(pic_ctx* c, int s) {
     if (kernel == 1) {
         PhaseTimer t(c, PIC_T_PUSH);
         int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 64);
+        const P64 P = p64(c, S);
         if (c->g.m == 2) k_push64_direct<2><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K);
         else k_push64_direct<4><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K);
         PIC_LAUNCH_CHECK(c);
         return PIC_OK;
     }
     // tiled: particles grouped by tile (a stale grouping only costs speed:
-    // deposits outside the tile's accumulator box go straight to HBM)
+    // a particle found outside its tile is pushed through HBM)
     {
         PhaseTimer t(c, PIC_T_SORT);
         if (!S.sorted || S.pushes_since_sort >= c->sort_interval) {
@@ -454,25 +480,9 @@ int launch_push64(pic_ctx* c, int s) {
     PhaseTimer t(c, PIC_T_PUSH);
     S.pushes_since_sort++;
     const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / chunk + 1;
-    int64_t* work = c->work;
-    int64_t* nwork = c->nwork;
-    if (c->g.m == 2) {
-        using B = AccBox<2, 8, 8, 8>;
-        const size_t lds = 4 * B::VOL * sizeof(double);
-        PIC_HIP(c, hipFuncSetAttribute((const void*)k_push64_tiled<2, 8, 8, 8>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        k_push64_tiled<2, 8, 8, 8><<<(unsigned)max_items, 256, lds, c->stream>>>(
-            P, Fd, c->g, K, c->tl, S.tile_off, work, nwork, chunk);
-    } else {
-        using B = AccBox<4, 8, 8, 4>;
-        const size_t lds = 4 * B::VOL * sizeof(double);
-        PIC_HIP(c, hipFuncSetAttribute((const void*)k_push64_tiled<4, 8, 8, 4>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        k_push64_tiled<4, 8, 8, 4><<<(unsigned)max_items, 256, lds, c->stream>>>(
-            P, Fd, c->g, K, c->tl, S.tile_off, work, nwork, chunk);
-    }
-    PIC_LAUNCH_CHECK(c);
-    return PIC_OK;
+    const P64 P = p64(c, S);  // after the sort
+    if (c->g.m == 2) return launch_tiled64<2>(c, S, P, Fd, K, max_items);
+    return launch_tiled64<4>(c, S, P, Fd, K, max_items);
 }
 
 }  // namespace pic

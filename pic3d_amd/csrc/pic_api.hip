@@ -28,6 +28,27 @@ int set_err(pic_ctx* c, int code, const char* fmt, ...) {
     return code;
 }
 
+void set_tiling(pic_ctx* c) {
+    const int kind = c->fp32 ? (c->fix32 ? 2 : 1) : 0;
+    const bool q = c->g.m == 2;
+    int t[3];
+    switch (kind) {
+        case 0: t[0] = q ? TileShape<0, 2>::X : TileShape<0, 4>::X; t[1] = q ? TileShape<0, 2>::Y : TileShape<0, 4>::Y;
+                t[2] = q ? TileShape<0, 2>::Z : TileShape<0, 4>::Z; break;
+        case 1: t[0] = q ? TileShape<1, 2>::X : TileShape<1, 4>::X; t[1] = q ? TileShape<1, 2>::Y : TileShape<1, 4>::Y;
+                t[2] = q ? TileShape<1, 2>::Z : TileShape<1, 4>::Z; break;
+        default: t[0] = q ? TileShape<2, 2>::X : TileShape<2, 4>::X; t[1] = q ? TileShape<2, 2>::Y : TileShape<2, 4>::Y;
+                 t[2] = q ? TileShape<2, 2>::Z : TileShape<2, 4>::Z; break;
+    }
+    Tiling& tl = c->tl;
+    tl.ntiles = 1;
+    for (int a = 0; a < 3; a++) {
+        tl.t[a] = t[a];
+        tl.nt[a] = (c->g.n[a] + t[a] - 1) / t[a];
+        tl.ntiles *= tl.nt[a];
+    }
+}
+
 int launch_push(pic_ctx* c, int s) {
     extern int launch_push64(pic_ctx*, int);
     extern int launch_push32(pic_ctx*, int);
@@ -258,14 +279,7 @@ int pic_ctx_create(const pic_params* p, int device, pic_ctx** out) {
     }
     g.fsize = (int64_t)g.fe[0] * g.fe[1] * g.fe[2];
     g.jsize = (int64_t)g.je[0] * g.je[1] * g.je[2];
-    // tiles: the LDS accumulator box of the tiled push is (t + m + 3)^3
-    Tiling& tl = c->tl;
-    tl.t[0] = 8; tl.t[1] = 8; tl.t[2] = g.m == 2 ? 8 : 4;
-    tl.ntiles = 1;
-    for (int a = 0; a < 3; a++) {
-        tl.nt[a] = (g.n[a] + tl.t[a] - 1) / tl.t[a];
-        tl.ntiles *= tl.nt[a];
-    }
+    set_tiling(c);
     const int es = c->fp32 ? 4 : 8;
     for (int f = 0; f < 6 && !r; f++) {
         char* q = nullptr;
@@ -279,8 +293,8 @@ int pic_ctx_create(const pic_params* p, int device, pic_ctx** out) {
         c->J[f] = q;
         if (!r) r = launch_fill(c, q, g.jsize * es);
     }
-    if (!r) r = dmalloc(c, &c->tile_cnt, tl.ntiles + 1);
-    if (!r) r = dmalloc(c, &c->tile_cur, tl.ntiles + 1);
+    if (!r) r = dmalloc(c, &c->tile_cnt, c->tl.ntiles + 1);
+    if (!r) r = dmalloc(c, &c->tile_cur, c->tl.ntiles + 1);
     if (!r) r = dmalloc(c, &c->nwork, 8);
     if (!r) r = dmalloc(c, &c->diag_buf, 12 + PIC_SPECTRUM_BINS + 8);
     if (r) return fail(r);
@@ -343,6 +357,37 @@ int pic_set_option(pic_ctx* c, int option, int64_t value) {
                 for (auto& e : c->tev) PIC_HIP(c, hipEventCreate(&e));
             }
             return PIC_OK;
+        case PIC_OPT_FP32_ACCUM: {
+            if (value != 0 && value != 1) return set_err(c, PIC_ERR_ARG, "accumulator kind must be 0 or 1");
+            if (!c->fp32) return value ? set_err(c, PIC_ERR_ARG, "fixed-point accumulation is an FP32 option") : PIC_OK;
+            c->fix32 = (int)value;
+            const int old_tiles = c->tl.ntiles;
+            set_tiling(c);
+            for (int s = 0; s < PIC_MAX_SPECIES; s++) c->sp[s].sorted = 0;
+            if (c->tl.ntiles > old_tiles) {  // tile buffers grow with the tile count
+                dfree(c->tile_cnt); dfree(c->tile_cur);
+                int r = dmalloc(c, &c->tile_cnt, c->tl.ntiles + 1);
+                if (!r) r = dmalloc(c, &c->tile_cur, c->tl.ntiles + 1);
+                for (int s = 0; s < PIC_MAX_SPECIES && !r; s++)
+                    if (c->sp[s].declared) {
+                        dfree(c->sp[s].tile_off);
+                        r = dmalloc(c, &c->sp[s].tile_off, c->tl.ntiles + 1);
+                    }
+                if (r) return r;
+            }
+            const int64_t need = (int64_t)c->tl.ntiles + [&] {
+                int64_t m = 0;
+                for (int s = 0; s < PIC_MAX_SPECIES; s++) m = std::max(m, c->sp[s].cap);
+                return m;
+            }() / PIC_PUSH_CHUNK + 64;
+            if (need > c->work_cap) {
+                dfree(c->work);
+                int r = dmalloc(c, &c->work, need);
+                c->work_cap = r ? 0 : need;
+                if (r) return r;
+            }
+            return PIC_OK;
+        }
         case PIC_OPT_SORT_INTERVAL:
             if (value < 1) return set_err(c, PIC_ERR_ARG, "sort interval must be >= 1");
             c->sort_interval = (int)value;
@@ -429,8 +474,13 @@ int pic_particles_set(pic_ctx* c, int s, int64_t n, const double* x, const doubl
     if (S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is immobile (no particle storage)", s);
     if (n > S.cap) return set_err(c, PIC_ERR_CAPACITY, "%lld particles exceed capacity %lld",
                                   (long long)n, (long long)S.cap);
-    for (int64_t q = 0; q < n; q++)
+    double wmax = 0;
+    for (int64_t q = 0; q < n; q++) {
         if (id[q] < 0) return set_err(c, PIC_ERR_ARG, "particle ids must be >= 0");
+        if (!(std::fabs(w[q]) <= 1e300)) return set_err(c, PIC_ERR_ARG, "weights must be finite");
+        wmax = std::max(wmax, std::fabs(w[q]));
+    }
+    S.wmax = wmax;
     PIC_HIP(c, hipSetDevice(c->device));
     if (c->fp32) {
         int r = particles_set32(c, s, n, x, y, z, ux, uy, uz, w, id);
@@ -513,6 +563,7 @@ int pic_particles_load(pic_ctx* c, int s, const pic_loader* l) {
     int r = dmalloc(c, &d_pre, (int64_t)pre.size());
     if (r) return r;
     PIC_HIP(c, hipMemcpyAsync(d_pre, pre.data(), pre.size() * 8, hipMemcpyHostToDevice, c->stream));
+    S.wmax = std::fabs(l->density / l->ppc);
     r = launch_loader(c, s, l, n, d_pre);
     int64_t cnt[CNT_N] = {0};
     cnt[CNT_SLOTS] = n;

@@ -44,6 +44,7 @@ struct Tiling {
 struct Species {
     int declared = 0, immobile = 0;
     double q = 0, m = 1;
+    double wmax = 0;  // largest |weight| loaded (bounds one particle's deposit)
     int64_t cap = 0;
     // FP64 arrays (slot order), [0] current, [1] scratch for the sort
     double* x[2] = {}; double* y[2] = {}; double* z[2] = {};
@@ -64,8 +65,11 @@ struct Species {
     int sorted = 0;               // tile_off valid for the current order
 };
 
+// d_cnt layout.  CNT_PEND: FP32 pushes classify boundary crossings as they
+// move particles and park the counts here until pic_particle_boundary.
+// The last entry is scratch for the sort (pre-sort slot count).
 enum { CNT_SLOTS = 0, CNT_LIVE = 1, CNT_LOST = 2, CNT_WRAP = 8, CNT_LOST_STEP = 14,
-       CNT_WRAP_STEP = 20, CNT_OOB = 26, CNT_N = 32 };
+       CNT_WRAP_STEP = 20, CNT_OOB = 26, CNT_PEND = 27, CNT_N = 40 };
 
 }  // namespace pic
 
@@ -93,6 +97,7 @@ struct pic_ctx {
     double* diag_buf = nullptr;   // diagnostics reduction buffer
     int push_kernel = 0;          // 0 auto, 1 naive (global atomics), 2 tiled
     int keep_old = 0;
+    int fix32 = 0;                // FP32: int32 fixed-point LDS accumulators
     // per-phase timing (PIC_OPT_TIMING): event pairs around each phase
     int timing = 0;
     std::vector<hipEvent_t> tev;      // event pool
@@ -141,6 +146,36 @@ struct PhaseTimer {
 };
 
 constexpr int PIC_PUSH_CHUNK = 4096;  // particles per tiled-push workgroup
+// the tiled pushes track one bit per particle a thread visits (256 threads)
+static_assert(PIC_PUSH_CHUNK <= 32 * 256, "stray mask holds 32 iterations");
+
+// Visiting order of a tiled push chunk of length len: the t-th visit goes
+// to slot (t * stride) % len, a bijection for a prime stride that does not
+// divide len.  Particles sharing a cell sit next to each other in storage;
+// the stride puts a wavefront's 64 lanes in different cells, so their LDS
+// deposits rarely target the same address (same-address LDS atomics
+// serialise).
+__host__ __device__ inline int chunk_stride(int len) {
+    return (len % 61) ? 61 : ((len % 67) ? 67 : 71);
+}
+
+// Tile edges of the sorted push, chosen so two workgroups' LDS (field tile
+// + accumulator box) fit in one CU's 160 KB.  KIND: 0 = FP64 fields and
+// FP64 accumulators; 1 = FP32 fields, FP64 accumulators; 2 = FP32 fields,
+// int32 fixed-point accumulators.
+//   0: quadratic 8x4x4 (60 KB), quartic 4x4x4 (78 KB)
+//   1: quadratic 8x8x4 (69 KB), quartic 4x4x4 (60 KB)
+//   2: quadratic 8x8x8 (67 KB), quartic 8x8x4 (76 KB)
+template <int KIND, int M> struct TileShape;
+template <> struct TileShape<0, 2> { static constexpr int X = 8, Y = 4, Z = 4; };
+template <> struct TileShape<0, 4> { static constexpr int X = 4, Y = 4, Z = 4; };
+template <> struct TileShape<1, 2> { static constexpr int X = 8, Y = 8, Z = 4; };
+template <> struct TileShape<1, 4> { static constexpr int X = 4, Y = 4, Z = 4; };
+template <> struct TileShape<2, 2> { static constexpr int X = 8, Y = 8, Z = 8; };
+template <> struct TileShape<2, 4> { static constexpr int X = 8, Y = 8, Z = 4; };
+
+// Set c->tl for the context's precision, order and accumulator kind.
+void set_tiling(pic_ctx* c);
 
 inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 

@@ -1,0 +1,163 @@
+"""GPU FP32 form against the FP64 oracle (error of the reduced precision).
+
+Bars, stated per test:
+  * one push on identical fields: positions within 2e-6 cells, momenta
+    within 2e-6 relative (of max |u|), J/rho within 1e-5 norm-wise relative
+    (FP64 LDS accumulators, the default) or 5e-4 (int32 fixed-point LDS
+    accumulators: each add is rounded to 2^-19 of the largest single-particle
+    contribution); cell indices identical except for particles closer than
+    1e-5 cells to a cell face.
+  * boundary losses/wraps per face within one particle of the oracle.
+  * 32^3 thermal plasma, 100 steps: total energy within 1e-4 relative,
+    field energy within 1e-2 relative (field energy is ~1e-4 of the total,
+    it is noise-dominated at this resolution).
+  * golden reference runs (up to 30 steps): fields within 1e-3 norm-wise.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, load_particles
+from pic3d_amd import PIC_FP32, Simulation, workloads as W
+from pic3d_amd._abi import FIELD_NAMES, PIC_OPT_FP32_ACCUM
+from pic3d_amd.simulation import field_shape
+
+from _helpers import by_id, golden_names, load_golden, rel_err, replay
+
+pytestmark = pytest.mark.gpu
+
+
+def fp32(p):
+    q = p.copy()
+    q.precision = PIC_FP32
+    return q
+
+
+@pytest.mark.parametrize("kernel,fix", [(1, 0), (2, 0), (2, 1)])
+@pytest.mark.parametrize("mem", [2, 4])
+def test_fp32_single_push(mem, kernel, fix):
+    print(f"fp32 push mem {mem} kernel {kernel} fixed-point {fix}", flush=True)
+    rng = np.random.default_rng(21 + mem)
+    wl = W.laser_slab(nx=24, ny=20, nz=18, ppc=4, slab=(0.0, 24.0), seed=8, mem=mem)
+    wl.params.periodic[0] = 1
+    wl.species[0].loader.u_th = 0.4
+    p = wl.params
+    parts = load_particles(p, wl.species[0].loader)
+    F = {c: rng.normal(0, 0.3, field_shape(p, c)).astype(np.float32).astype(np.float64)
+         for c in range(6)}
+    sim = Simulation(fp32(p), push_kernel=kernel)
+    sim.set_option(PIC_OPT_FP32_ACCUM, fix)
+    sim.species_set(0, -1.0, 1.0, False, len(parts["x"]))
+    # inputs representable in fp32 so both sides start from the same state
+    p32 = {k: (v.astype(np.float32).astype(np.float64) if k in ("ux", "uy", "uz", "w") else v)
+           for k, v in parts.items()}
+    sim.particles_set(0, p32)
+    start = by_id(sim.particles_get(0))  # positions as stored (cell + fp32 offset)
+    orc = Oracle(p, 1)
+    orc.species_set(0, -1.0, 1.0)
+    orc.particles_set(0, start)
+    for c, a in F.items():
+        sim.field_set(c, a)
+        orc.field_set(c, a)
+    sim.push(0)
+    orc.push(0)
+    pg, po = by_id(sim.particles_get(0)), by_id(orc.particles_get(0))
+    assert np.array_equal(pg["id"], po["id"])
+    # the FP32 form wraps periodic positions right after the move, the
+    # reference at the next push: compare modulo the box
+    for k, n in (("x", p.nx), ("y", p.ny), ("z", p.nz)):
+        d = pg[k] - po[k]
+        d -= n * np.round(d / n)
+        assert np.max(np.abs(d)) < 2e-6, k
+        po[k] = pg[k] - d
+    umax = max(np.max(np.abs(po[k])) for k in ("ux", "uy", "uz"))
+    for k in ("ux", "uy", "uz"):
+        assert np.max(np.abs(pg[k] - po[k])) < 2e-6 * umax, k
+    near = np.zeros(len(po["x"]), bool)
+    for k in ("x", "y", "z"):
+        fr = po[k] - np.floor(po[k])
+        near |= (fr < 1e-5) | (fr > 1 - 1e-5)
+    cg = np.floor(np.stack([pg["x"], pg["y"], pg["z"]]))
+    co = np.floor(np.stack([po["x"], po["y"], po["z"]]))
+    assert np.all((cg == co).all(axis=0) | near)
+    bar = 5e-4 if fix else 1e-5
+    for c in (6, 7, 8, 9):
+        a, b = sim.field_get(c), orc.field_get(c)
+        assert rel_err(a, b) < bar, (FIELD_NAMES[c], rel_err(a, b))
+
+
+def test_fp32_boundary_counts():
+    print("fp32 boundary", flush=True)
+    wl = W.laser_slab(nx=16, ny=12, nz=12, ppc=4, slab=(0.0, 16.0), seed=9, mem=2)
+    wl.species[0].loader.u_th = 2.0
+    p = wl.params
+    parts = load_particles(p, wl.species[0].loader)
+    sim = Simulation(fp32(p))
+    sim.species_set(0, -1.0, 1.0, False, len(parts["x"]))
+    sim.particles_set(0, parts)
+    orc = Oracle(p, 1)
+    orc.species_set(0, -1.0, 1.0)
+    orc.particles_set(0, by_id(sim.particles_get(0)))
+    for it in range(3):
+        sim.push(0); orc.push(0)
+        cg, co = sim.particle_boundary(0), orc.particle_boundary(0)
+        for f in range(6):
+            assert abs(cg["lost"][f] - co["lost"][f]) <= 1, (it, cg, co)
+            assert abs(cg["wrapped"][f] - co["wrapped"][f]) <= 1, (it, cg, co)
+    assert abs(sim.num_particles(0) - orc.num_particles(0)) <= 2
+
+
+def test_fp32_loader_matches_host():
+    wl = W.ramp_target(nx=64, ny=16, nz=16, ppc_target=20)
+    p = wl.params
+    host = by_id(load_particles(p, wl.species[0].loader))
+    sim = Simulation(p)
+    sim.species_set(0, -1.0, 1.0, False, len(host["x"]))
+    sim.particles_load(0, wl.species[0].loader)
+    dev = by_id(sim.particles_get(0))
+    assert np.array_equal(dev["id"], host["id"])
+    for k in ("x", "y", "z"):
+        assert np.array_equal(np.floor(dev[k]), np.floor(host[k])), k
+        assert np.max(np.abs(dev[k] - host[k])) <= 6e-8, k
+    for k in ("ux", "uy", "uz"):
+        assert np.max(np.abs(dev[k] - host[k])) <= 1e-7 * np.max(np.abs(host[k])), k
+
+
+@pytest.mark.parametrize("fix", [0, 1])
+def test_fp32_thermal_energy(fix):
+    print(f"fp32 thermal fixed-point {fix}", flush=True)
+    wl = W.thermal()
+    p = wl.params
+    parts = load_particles(p, wl.species[0].loader)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        q = fp32(p) if prec == "fp32" else p
+        sim = Simulation(q)
+        if prec == "fp32":
+            sim.set_option(PIC_OPT_FP32_ACCUM, fix)
+        for s, sp in enumerate(wl.species):
+            sim.species_set(s, sp.charge, sp.mass, sp.immobile, len(parts["x"]) if s == 0 else 0)
+        sim.set_background(1.0)
+        sim.particles_set(0, parts)
+        sim.preloop()
+        sim.step(1, 100)
+        res[prec] = sim.diag(0.01)
+        sim.close()
+    a, b = res["fp32"], res["fp64"]
+    ta = a["field_energy"] + a["kinetic"][0]
+    tb = b["field_energy"] + b["kinetic"][0]
+    assert abs(ta - tb) < 1e-4 * tb, (ta, tb)
+    assert abs(a["field_energy"] - b["field_energy"]) < 1e-2 * b["field_energy"]
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_fp32_golden(name):
+    print(f"fp32 golden {name}", flush=True)
+    g = load_golden(name)
+    sim = Simulation(fp32(g["params"]))
+    sim.species_set(0, g["charge"], g["mass"], False, len(g["parts"]["x"]))
+    sim.set_background(g["rho_background"])
+    sim.particles_set(0, g["parts"])
+    replay(sim, g["cmds"])
+    for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz"):
+        got = sim.field_get(FIELD_NAMES.index(k))
+        assert rel_err(got, g["fields"][k]) < 1e-3, (k, rel_err(got, g["fields"][k]))
