@@ -164,28 +164,49 @@ def run_gpu(args, precision: str) -> dict:
     return out
 
 
+def copy_probe(nbytes: int):
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads n*4 B, writes n*4 B
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "both"])
+    ap.add_argument("--precision", default="both", choices=["fp64", "fp32", "both"])
     ap.add_argument("--workload", default="laser_slab",
                     choices=["laser_slab", "ramp_target", "thermal"])
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 direct, 2 tiled")
     ap.add_argument("--sort-interval", type=int, default=1)
-    ap.add_argument("--fp32-accum", type=int, default=0,
-                    help="FP32 deposit accumulators: 0 FP64 in LDS, 1 int32 fixed point")
+    ap.add_argument("--fp32-accum", type=int, default=1,
+                    help="FP32 deposit accumulators: 1 int32 fixed point (default), 0 FP64 in LDS")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (read BYTES, write BYTES) "
+                         "three times: the known-size kernel that scales rocprofv3 "
+                         "FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                           "design-only (see DESIGN.md)", "n_gpus": args.gpus}))
         sys.exit(2)
     precs = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {pr: run_gpu(args, pr) for pr in precs}
-    head = res[precs[-1]]
+    # headline precision: the ramp-target case is an FP32 configuration, the others FP64
+    want = "fp32" if args.workload == "ramp_target" else "fp64"
+    hp = want if want in res else precs[0]
+    head = res[hp]
     line = {
         "metric": "particle pushes/s (gather+push+deposit) & Yee cell-updates/s, FP32 vs FP64",
         "value": head["pushes_per_s"],
@@ -193,7 +214,7 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if precs[-1] == "fp64" else "f32",
+        "dtype": "f64" if hp == "fp64" else "f32",
         "data": "synthetic (seeded device loader)",
         "config": {"workload": args.workload, "detail": head["workload"],
                    "cells": head["cells"], "particles": head["particles"],
@@ -207,7 +228,7 @@ def main():
     if "fp64" in res and "fp32" in res:
         line["fp32_over_fp64"] = res["fp32"]["pushes_per_s"] / res["fp64"]["pushes_per_s"]
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(args.workload, precs[-1], args.cpu_threads)
+        line["cpu_baseline"] = cpu_baseline(args.workload, hp, args.cpu_threads)
     print(json.dumps(line), flush=True)
 
 

@@ -394,9 +394,9 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
     int64_t oob = 0;
     uint32_t strays = 0;  // iterations whose particle was outside the tile
     int it = 0;
-    const int len = (int)(end - beg), stride = chunk_stride(len);
+    const int len = (int)(end - beg);
     for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
-        const int64_t p = beg + (int)(((int64_t)t * stride) % len);
+        const int64_t p = beg + visit_slot(t, len);
         const int cell = P.cell[p];
         if (cell < 0) continue;
         const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
         it = 0;
         for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
             if (!(strays & (1u << it))) continue;
-            const int64_t p = beg + (int)(((int64_t)t * stride) % len);
+            const int64_t p = beg + visit_slot(t, len);
             const int cell = P.cell[p];
             const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
             float f[3] = {P.fx[p], P.fy[p], P.fz[p]};

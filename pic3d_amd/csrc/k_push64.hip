@@ -365,9 +365,9 @@ __global__ __launch_bounds__(256) void k_push64_tiled(P64 P, Fields64 F, Grid g,
     int64_t oob = 0;
     uint32_t strays = 0;  // iterations whose particle was outside the tile
     int it = 0;
-    const int len = (int)(end - beg), stride = chunk_stride(len);
+    const int len = (int)(end - beg);
     for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
-        const int64_t p = beg + (int)(((int64_t)t * stride) % len);
+        const int64_t p = beg + visit_slot(t, len);
         const double wp = P.w[p];
         if (!(wp == wp)) continue;
         double x = P.x[p], y = P.y[p], z = P.z[p];
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(256) void k_push64_tiled(P64 P, Fields64 F, Grid g,
         it = 0;
         for (int t = threadIdx.x; t < len; t += blockDim.x, it++)
             if (strays & (1u << it)) {
-                const int64_t p = beg + (int)(((int64_t)t * stride) % len);
+                const int64_t p = beg + visit_slot(t, len);
                 oob += push_one_hbm<M>(P, p, P.w[p], F, g, K);
             }
     }

@@ -279,6 +279,7 @@ int pic_ctx_create(const pic_params* p, int device, pic_ctx** out) {
     }
     g.fsize = (int64_t)g.fe[0] * g.fe[1] * g.fe[2];
     g.jsize = (int64_t)g.je[0] * g.je[1] * g.je[2];
+    c->fix32 = c->fp32;  // FP32 default: fixed-point deposit accumulators
     set_tiling(c);
     const int es = c->fp32 ? 4 : 8;
     for (int f = 0; f < 6 && !r; f++) {

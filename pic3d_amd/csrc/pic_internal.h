@@ -149,14 +149,18 @@ constexpr int PIC_PUSH_CHUNK = 4096;  // particles per tiled-push workgroup
 // the tiled pushes track one bit per particle a thread visits (256 threads)
 static_assert(PIC_PUSH_CHUNK <= 32 * 256, "stray mask holds 32 iterations");
 
-// Visiting order of a tiled push chunk of length len: the t-th visit goes
-// to slot (t * stride) % len, a bijection for a prime stride that does not
-// divide len.  Particles sharing a cell sit next to each other in storage;
-// the stride puts a wavefront's 64 lanes in different cells, so their LDS
-// deposits rarely target the same address (same-address LDS atomics
-// serialise).
-__host__ __device__ inline int chunk_stride(int len) {
-    return (len % 61) ? 61 : ((len % 67) ? 67 : 71);
+// Visiting order of a tiled push chunk of length len: a 64 x (len/64)
+// transpose of the slots (the tail past the last multiple of 64 in order).
+// Lane l of the wave at visit row q takes slot l * (len/64) + q, so a
+// wavefront's 64 lanes sit len/64 slots apart -- in different cells when a
+// cell holds no more than len/64 particles, which keeps the LDS deposits of
+// one instruction off a shared address (same-address LDS atomics
+// serialise) -- while the following visit rows reuse the same cache lines
+// (slot + 1), so each line is fetched from HBM once.
+__host__ __device__ inline int visit_slot(int t, int len) {
+    const int l64 = len & ~63;
+    if (t >= l64) return t;
+    return (t & 63) * (l64 >> 6) + (t >> 6);
 }
 
 // Tile edges of the sorted push, chosen so two workgroups' LDS (field tile
