@@ -218,6 +218,34 @@ int pic_preloop(pic_ctx* ctx);
  * (simulation.cpp:159-183) without output. */
 int pic_step(pic_ctx* ctx, int first_step, int nsteps);
 
+/* ---- the rest of the reference's per-call surface ---- */
+
+/* radiating_boundary(rank, is_left, is_right, nprocs, step),
+ * boundary.cpp:42-99: is_left / is_right switch the x-min / x-max laser
+ * sources; the Silver-Mueller writes happen whenever x is open. */
+int pic_radiating_boundary(pic_ctx* ctx, int step, int is_left, int is_right);
+/* update_electromagnetic_fields, field.cpp:24-54: E then B advanced a full
+ * dt, each followed by its halo exchange. */
+int pic_fields_full(pic_ctx* ctx);
+/* exchange_boundary_data(data), mpi_utils.cpp:9-60, for one E/B
+ * component (PIC_EX..PIC_BZ). */
+int pic_field_exchange(pic_ctx* ctx, int comp);
+/* exchange_boundary_current_fixed(data), mpi_utils.cpp:232-417, for one
+ * J or rho array (PIC_JX..PIC_JZ, PIC_RHO0..PIC_RHO3). */
+int pic_current_exchange(pic_ctx* ctx, int comp);
+/* binomial_filter_currents, particle.cpp:520-574: one 1-2-1 pass per axis
+ * (X, Y, Z) over the interior of jx, jy, jz. */
+int pic_filter_currents(pic_ctx* ctx);
+/* currents_particle(p, s), particle.cpp:372-515, for n particles given on
+ * the host by old and new positions and weight (charge from species s):
+ * Esirkepov current and charge deposit into J and rho[s].  FP64 contexts
+ * only.  Particles whose old position is outside [0, n] on an axis, or
+ * that moved a cell or more, are skipped and counted in *skipped (may be
+ * NULL). Synchronises. */
+int pic_deposit(pic_ctx* ctx, int s, int64_t n, const double* x_old, const double* y_old,
+                const double* z_old, const double* x, const double* y, const double* z,
+                const double* w, int64_t* skipped);
+
 /* ---- readback / diagnostics ---- */
 
 /* Full ghosted array of component comp, as doubles in the reference layout. */

@@ -35,6 +35,11 @@ _SIG = {
     "orc_fold_currents": (None, [C.c_void_p]),
     "orc_field_wrap": (None, [C.c_void_p, C.c_int]),
     "orc_preloop": (None, [C.c_void_p]),
+    "orc_radiating_boundary": (None, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "orc_fields_full": (None, [C.c_void_p]),
+    "orc_fold_one": (None, [C.c_void_p, C.c_int]),
+    "orc_filter_currents": (None, [C.c_void_p]),
+    "orc_deposit": (C.c_int64, [C.c_void_p, C.c_int, C.c_int64] + [C.POINTER(C.c_double)] * 7),
     "orc_step": (None, [C.c_void_p, C.c_int, C.c_int]),
     "orc_field_get": (None, [C.c_void_p, C.c_int, _dp]),
     "orc_field_set": (None, [C.c_void_p, C.c_int, _dp]),
@@ -179,6 +184,27 @@ class Oracle:
 
     def preloop(self):
         lib().orc_preloop(self._h)
+
+    def radiating_boundary(self, step, is_left=1, is_right=0):
+        lib().orc_radiating_boundary(self._h, step, is_left, is_right)
+
+    def fields_full(self):
+        lib().orc_fields_full(self._h)
+
+    def field_exchange(self, comp):
+        lib().orc_field_wrap(self._h, comp)
+
+    def current_exchange(self, comp):
+        lib().orc_fold_one(self._h, comp)
+
+    def filter_currents(self):
+        lib().orc_filter_currents(self._h)
+
+    def deposit(self, s, parts) -> int:
+        a = {k: np.ascontiguousarray(parts[k], dtype=np.float64)
+             for k in ("x_old", "y_old", "z_old", "x", "y", "z", "w")}
+        return lib().orc_deposit(self._h, s, len(a["x"]), _d(a["x_old"]), _d(a["y_old"]),
+                                 _d(a["z_old"]), _d(a["x"]), _d(a["y"]), _d(a["z"]), _d(a["w"]))
 
     def step(self, first_step, nsteps):
         lib().orc_step(self._h, first_step, nsteps)

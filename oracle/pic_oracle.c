@@ -270,9 +270,11 @@ void orc_field_wrap(orc_ctx* c, int comp) {
 /* Yee half steps: field.cpp:56-117                                    */
 /* ------------------------------------------------------------------ */
 
-static void e_half(orc_ctx* c) {
+/* E += coef (curl B - J); coef = dt_half (constants.cpp:42) in the split
+ * step, dt in update_electromagnetic_fields (field.cpp:24-54) */
+static void e_step(orc_ctx* c, double coef) {
     const int m = c->m;
-    const double dth = c->p.dt / 2.0; /* dt_half, constants.cpp:42 */
+    const double dth = coef;
     const double dx = c->p.dx, dy = c->p.dy, dz = c->p.dz;
     double *ex = c->F[EX], *ey = c->F[EY], *ez = c->F[EZ];
     const double *bx = c->F[BX], *by = c->F[BY], *bz = c->F[BZ];
@@ -294,9 +296,9 @@ static void e_half(orc_ctx* c) {
     orc_field_wrap(c, EZ);
 }
 
-static void b_half(orc_ctx* c) {
+static void b_step(orc_ctx* c, double coef) {
     const int m = c->m;
-    const double dth = c->p.dt / 2.0;
+    const double dth = coef;
     const double dx = c->p.dx, dy = c->p.dy, dz = c->p.dz;
     const double *ex = c->F[EX], *ey = c->F[EY], *ez = c->F[EZ];
     double *bx = c->F[BX], *by = c->F[BY], *bz = c->F[BZ];
@@ -317,8 +319,10 @@ static void b_half(orc_ctx* c) {
     orc_field_wrap(c, BZ);
 }
 
-void orc_fields_first(orc_ctx* c) { e_half(c); b_half(c); }
-void orc_fields_second(orc_ctx* c) { b_half(c); e_half(c); }
+void orc_fields_first(orc_ctx* c) { e_step(c, c->p.dt / 2.0); b_step(c, c->p.dt / 2.0); }
+void orc_fields_second(orc_ctx* c) { b_step(c, c->p.dt / 2.0); e_step(c, c->p.dt / 2.0); }
+/* update_electromagnetic_fields, field.cpp:24-54 */
+void orc_fields_full(orc_ctx* c) { e_step(c, c->p.dt); b_step(c, c->p.dt); }
 
 /* ------------------------------------------------------------------ */
 /* laser source + Silver-Mueller: boundary.cpp:8-99                    */
@@ -345,7 +349,9 @@ double orc_gpulse(const orc_ctx* c, double time, double x, double y, double z) {
            cos(ph + P->phase);
 }
 
-void orc_laser_boundary(orc_ctx* c, int step) {
+void orc_laser_boundary(orc_ctx* c, int step) { orc_radiating_boundary(c, step, 1, c->p.laser_right); }
+
+void orc_radiating_boundary(orc_ctx* c, int step, int is_left, int is_right) {
     const pic_params* P = &c->p;
     if (P->periodic[0]) return; /* x neighbours exist: rank_x* != -2 */
     const int m = c->m, N = c->n[0];
@@ -362,9 +368,11 @@ void orc_laser_boundary(orc_ctx* c, int step) {
             double yy = ((double)j + 0.0) * dy - P->yfocus;
             double zz = ((double)k + 0.0) * dz - P->zfocus;
             double Sl = 0.0, Pl = 0.0, Sr = 0.0, Pr = 0.0;
-            if (pol == 1 || pol == 3) Sl = orc_gpulse(c, time, 0.0, yy, zz);
-            if (pol == 2 || pol == 3) Pl = orc_gpulse(c, time, 0.0, yy, zz);
-            if (P->laser_right) {
+            if (is_left) {
+                if (pol == 1 || pol == 3) Sl = orc_gpulse(c, time, 0.0, yy, zz);
+                if (pol == 2 || pol == 3) Pl = orc_gpulse(c, time, 0.0, yy, zz);
+            }
+            if (is_right) {
                 if (pol == 1 || pol == 3) Sr = orc_gpulse(c, time, P->lx, yy, zz);
                 if (pol == 2 || pol == 3) Pr = orc_gpulse(c, time, P->lx, yy, zz);
             }
@@ -469,6 +477,48 @@ static void fold_one(const orc_ctx* c, double* D) {
         if (c->p.periodic[a]) fold_accumulate(c, D, a);
     for (int a = 0; a < 3; a++)
         if (c->p.periodic[a]) fold_sync(c, D, a);
+}
+
+/* exchange_boundary_current_fixed of one array: comp 6..8 = jx..jz,
+ * 9.. = rho of species comp-9 */
+void orc_fold_one(orc_ctx* c, int comp) {
+    if (comp >= 6 && comp <= 8) fold_one(c, c->J[comp - 6]);
+    else if (comp >= 9 && comp < 9 + PIC_MAX_SPECIES) fold_one(c, c->rho[comp - 9]);
+}
+
+/* binomial_filter_currents, particle.cpp:520-574: per array jx, jy, jz a
+ * 1-2-1 pass along x over the interior rows, then along y, then along z,
+ * each pass reading the line as it was before that pass (the reference
+ * copies each line into a buffer, ghosts included, before filtering). */
+static void filter_array(orc_ctx* c, double* D) {
+    const int m = c->m;
+    const int nx = c->n[0], ny = c->n[1], nz = c->n[2];
+    int nmax = nx > ny ? nx : ny;
+    if (nz > nmax) nmax = nz;
+    double* buf = (double*)malloc((size_t)(nmax + 2) * sizeof(double));
+    for (int k = m; k < m + nz; k++)
+        for (int j = m; j < m + ny; j++) {
+            for (int i = -1; i <= nx; i++) buf[i + 1] = D[JI(c, k, j, m + i)];
+            for (int i = 0; i < nx; i++)
+                D[JI(c, k, j, m + i)] = 0.25 * buf[i] + 0.5 * buf[i + 1] + 0.25 * buf[i + 2];
+        }
+    for (int k = m; k < m + nz; k++)
+        for (int i = m; i < m + nx; i++) {
+            for (int j = -1; j <= ny; j++) buf[j + 1] = D[JI(c, k, m + j, i)];
+            for (int j = 0; j < ny; j++)
+                D[JI(c, k, m + j, i)] = 0.25 * buf[j] + 0.5 * buf[j + 1] + 0.25 * buf[j + 2];
+        }
+    for (int j = m; j < m + ny; j++)
+        for (int i = m; i < m + nx; i++) {
+            for (int k = -1; k <= nz; k++) buf[k + 1] = D[JI(c, m + k, j, i)];
+            for (int k = 0; k < nz; k++)
+                D[JI(c, m + k, j, i)] = 0.25 * buf[k] + 0.5 * buf[k + 1] + 0.25 * buf[k + 2];
+        }
+    free(buf);
+}
+
+void orc_filter_currents(orc_ctx* c) {
+    for (int f = 0; f < 3; f++) filter_array(c, c->J[f]);
 }
 
 void orc_fold_currents(orc_ctx* c) {
@@ -644,6 +694,26 @@ static int push_one(const orc_ctx* c, orc_species* S, int64_t p, double* jx, dou
     if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) return 1;
     deposit_one(c, S->q, S->w[p], x, y, z, xn, yn, zn, jx, jy, jz, rho);
     return 0;
+}
+
+/* currents_particle(p, s) for explicit old/new positions and weights,
+ * particle.cpp:372-515, in the given order.  Returns the number skipped
+ * (old position outside [0, N] or a move of a cell or more). */
+int64_t orc_deposit(orc_ctx* c, int s, int64_t n, const double* xo, const double* yo,
+                    const double* zo, const double* xn, const double* yn, const double* zn,
+                    const double* w) {
+    int64_t bad = 0;
+    for (int64_t p = 0; p < n; p++) {
+        const int inside = xo[p] >= 0.0 && xo[p] <= c->n[0] && yo[p] >= 0.0 && yo[p] <= c->n[1] &&
+                           zo[p] >= 0.0 && zo[p] <= c->n[2];
+        if (!inside || !(fabs(xn[p] - xo[p]) < 1.0 && fabs(yn[p] - yo[p]) < 1.0 && fabs(zn[p] - zo[p]) < 1.0)) {
+            bad++;
+            continue;
+        }
+        deposit_one(c, c->sp[s].q, w[p], xo[p], yo[p], zo[p], xn[p], yn[p], zn[p], c->J[0], c->J[1],
+                    c->J[2], c->rho[s]);
+    }
+    return bad;
 }
 
 void orc_push(orc_ctx* c, int s) {

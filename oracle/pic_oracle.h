@@ -43,6 +43,13 @@ void orc_particle_boundary(orc_ctx* c, int s, pic_counts* out); /* mpi_utils.cpp
 void orc_fold_currents(orc_ctx* c);             /* mpi_utils.cpp:232-417   */
 void orc_field_wrap(orc_ctx* c, int comp);      /* mpi_utils.cpp:9-60      */
 void orc_preloop(orc_ctx* c);                   /* main.cpp:75-79          */
+void orc_radiating_boundary(orc_ctx* c, int step, int is_left, int is_right); /* boundary.cpp:42-99 */
+void orc_fields_full(orc_ctx* c);               /* field.cpp:24-54         */
+void orc_fold_one(orc_ctx* c, int comp);        /* mpi_utils.cpp:232-417, one array */
+void orc_filter_currents(orc_ctx* c);           /* particle.cpp:520-574    */
+int64_t orc_deposit(orc_ctx* c, int s, int64_t n, const double* xo, const double* yo,
+                    const double* zo, const double* xn, const double* yn, const double* zn,
+                    const double* w);           /* particle.cpp:372-515    */
 void orc_step(orc_ctx* c, int first_step, int nsteps); /* simulation.cpp:159-183 */
 
 void orc_field_get(const orc_ctx* c, int comp, double* out);

@@ -12,6 +12,12 @@
 //        steps:A:N           steps A..A+N-1 of run_main_time_loop
 //                            (simulation.cpp:159-183) without output
 //        laser:S first second clear push:K bnd:K fold   single phases
+//        rad:S:L:R           radiating_boundary(rank, L, R, 1, S)
+//        full                update_electromagnetic_fields
+//        filter              binomial_filter_currents
+//        xfield:C            exchange_boundary_data(ex|ey|ez|bx|by|bz [C])
+//        xcur:C              exchange_boundary_current_fixed(jx|jy|jz|rho0|rho1 [C])
+//        dep:K               currents_particle(p, K) for every particle of kind K
 //
 // MPI_PROC_NULL is -1 under MPICH but the reference tests for OpenMPI's -2
 // (boundary.cpp:87,92; mpi_utils.cpp:16-50,80-222).  On an open x axis the
@@ -44,10 +50,10 @@ static void zero3(double*** a, int lo, int hiz, int hiy, int hix) {
             for (int i = lo; i <= hix; i++) a[k][j][i] = 0.0;
 }
 
-static void laser(int step) {
+static void laser(int step, int left = 1, int right = 0) {
     int xm = rank_xminus, xp = rank_xplus;
     if (!rank_period[0]) rank_xminus = rank_xplus = -2;
-    radiating_boundary(g_rank, 1, 0, 1, step);
+    radiating_boundary(g_rank, left, right, 1, step);
     rank_xminus = xm;
     rank_xplus = xp;
 }
@@ -166,6 +172,23 @@ int main(int argc, char** argv) {
             exchange_particle_data(P[s], n[s], g_rank, g_comm);
         } else if (cmd == "fold") {
             exchange_currents(g_comm);
+        } else if (cmd.rfind("rad:", 0) == 0) {
+            int st = 0, l = 0, r = 0;
+            std::sscanf(cmd.c_str(), "rad:%d:%d:%d", &st, &l, &r);
+            laser(st, l, r);
+        } else if (cmd == "full") {
+            update_electromagnetic_fields(g_rank, 1, g_comm);
+        } else if (cmd == "filter") {
+            binomial_filter_currents();
+        } else if (cmd.rfind("xfield:", 0) == 0) {
+            double*** a[6] = {ex, ey, ez, bx, by, bz};
+            exchange_boundary_data(a[std::atoi(cmd.c_str() + 7)], g_comm);
+        } else if (cmd.rfind("xcur:", 0) == 0) {
+            double*** a[5] = {jx, jy, jz, rho_particle[0], rho_particle[1]};
+            exchange_boundary_current_fixed(a[std::atoi(cmd.c_str() + 5)], g_comm);
+        } else if (cmd.rfind("dep:", 0) == 0) {
+            int s = std::atoi(cmd.c_str() + 4);
+            for (int p = 0; p < n[s]; p++) currents_particle(P[s][p], s);
         } else {
             std::fprintf(stderr, "harness: unknown command %s\n", cmd.c_str());
             return 5;

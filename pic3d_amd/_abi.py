@@ -115,6 +115,12 @@ SIGNATURES = {
     "pic_fold_currents": (C.c_int, [C.c_void_p]),
     "pic_preloop": (C.c_int, [C.c_void_p]),
     "pic_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "pic_radiating_boundary": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "pic_fields_full": (C.c_int, [C.c_void_p]),
+    "pic_field_exchange": (C.c_int, [C.c_void_p, C.c_int]),
+    "pic_current_exchange": (C.c_int, [C.c_void_p, C.c_int]),
+    "pic_filter_currents": (C.c_int, [C.c_void_p]),
+    "pic_deposit": (C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7 + [C.POINTER(C.c_int64)]),
     "pic_field_get": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "pic_field_set": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "pic_field_size": (C.c_int64, [C.POINTER(PicParams), C.c_int]),

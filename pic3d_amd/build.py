@@ -72,7 +72,37 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_compat(verbose)
     return OUT
+
+
+COMPAT_SRC = os.path.join(HERE, "compat", "pic3d_compat.cpp")
+COMPAT_OUT = os.path.join(HERE, "libpic3d_compat.so")
+MPI_INC = "/opt/conda/include"  # the image's MPICH headers (only MPI_Comm is used)
+DRIVER_SRC = os.path.join(ROOT, "tests", "compat", "compat_main.cpp")
+DRIVER_OUT = os.path.join(ROOT, "tests", "compat", "bin", "compat_main")
+
+
+def _gxx(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed:\n{r.stdout}\n{r.stderr}")
+
+
+def build_compat(verbose: bool = False) -> str:
+    """libpic3d_compat.so: the reference's C++ surface on the C ABI (host
+    C++, built with g++; the parameter externs and the HDF5 writers are
+    left for the program that links it).  Also builds the test driver."""
+    inc = [f"-I{os.path.join(ROOT, 'include')}", f"-I{MPI_INC}", "-DMPICH_SKIP_MPICXX"]
+    _gxx(["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-shared", *inc,
+          COMPAT_SRC, f"-L{HERE}", "-lpic3d_amd", "-Wl,-rpath,$ORIGIN", "-o", COMPAT_OUT], verbose)
+    os.makedirs(os.path.dirname(DRIVER_OUT), exist_ok=True)
+    _gxx(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", *inc, DRIVER_SRC,
+          f"-L{HERE}", "-lpic3d_compat", "-lpic3d_amd",
+          "-Wl,-rpath,$ORIGIN/../../../pic3d_amd", "-o", DRIVER_OUT], verbose)
+    return COMPAT_OUT
 
 
 if __name__ == "__main__":

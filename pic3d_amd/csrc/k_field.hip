@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_b_half(const T* __restrict__ ex, const 
 // ---------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_wrap(T* __restrict__ a0, T* __restrict__ a1,
-                                              T* __restrict__ a2, Grid g, int axis) {
+                                              T* __restrict__ a2, Grid g, int axis, int na) {
     const int m = g.m, N = g.n[axis];
     // element (u, v, gi) where gi in [0, 2m) selects the ghost layer
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,19 +84,40 @@ __global__ __launch_bounds__(256) void k_wrap(T* __restrict__ a0, T* __restrict_
     else if (axis == 1) { od = g.fi(v, dst, u); os = g.fi(v, src, u); }
     else { od = g.fi(dst, v, u); os = g.fi(src, v, u); }
     a0[od] = a0[os];
-    a1[od] = a1[os];
-    a2[od] = a2[os];
+    if (na > 1) {
+        a1[od] = a1[os];
+        a2[od] = a2[os];
+    }
 }
 
 // ---------------------------------------------------------------------
-// Laser source + Silver-Mueller x boundary: radiating_boundary with
-// is_left = 1 (boundary.cpp:42-99), Gpulse (29-40), generate_laser_pulse
-// (8-27).  Applied only when x is open (the reference's rank_x* == -2).
+// One pass of the binomial current filter along one axis (filter_field,
+// particle.cpp:520-567): interior nodes get 0.25 a[i-1] + 0.5 a[i] +
+// 0.25 a[i+1] of the array as it was before the pass (src is that copy);
+// the other two axes run over the interior only.
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_filter(const T* __restrict__ src, T* __restrict__ dst,
+                                                Grid g, int axis) {
+    const int i = g.m + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = g.m + blockIdx.y;
+    const int k = g.m + blockIdx.z;
+    if (i >= g.m + g.n[0]) return;
+    const int64_t o = g.ji(k, j, i);
+    const int64_t s = axis == 0 ? 1 : (axis == 1 ? (int64_t)g.je[0] : (int64_t)g.je[0] * g.je[1]);
+    dst[o] = (T)0.25 * src[o - s] + (T)0.5 * src[o] + (T)0.25 * src[o + s];
+}
+
+// ---------------------------------------------------------------------
+// Laser source + Silver-Mueller x boundary: radiating_boundary
+// (boundary.cpp:42-99), Gpulse (29-40), generate_laser_pulse (8-27).
+// is_left / is_right switch the x-min / x-max sources; both Silver-Mueller
+// writes happen whenever x is open (the reference's rank_x* == -2).
 // ---------------------------------------------------------------------
 struct LaserArgs {
     double pi, l0, waist, xfocus, yfocus, zfocus, omega, phase, amp, dur, t_start, lx;
     double dt, dx, dy, dz, time;
-    int pol, right;
+    int pol, left, right;
 };
 
 __device__ double d_envelope(const LaserArgs& L, double time, double position) {
@@ -132,8 +153,10 @@ __global__ __launch_bounds__(256) void k_laser(T* __restrict__ ey, T* __restrict
     const double yy = ((double)j + 0.0) * L.dy - L.yfocus;
     const double zz = ((double)k + 0.0) * L.dz - L.zfocus;
     double Sl = 0.0, Pl = 0.0, Sr = 0.0, Pr = 0.0;
-    if (L.pol == 1 || L.pol == 3) Sl = d_gpulse(L, L.time, 0.0, yy, zz);
-    if (L.pol == 2 || L.pol == 3) Pl = d_gpulse(L, L.time, 0.0, yy, zz);
+    if (L.left) {
+        if (L.pol == 1 || L.pol == 3) Sl = d_gpulse(L, L.time, 0.0, yy, zz);
+        if (L.pol == 2 || L.pol == 3) Pl = d_gpulse(L, L.time, 0.0, yy, zz);
+    }
     if (L.right) {
         if (L.pol == 1 || L.pol == 3) Sr = d_gpulse(L, L.time, L.lx, yy, zz);
         if (L.pol == 2 || L.pol == 3) Pr = d_gpulse(L, L.time, L.lx, yy, zz);
@@ -244,31 +267,35 @@ int launch_clear(pic_ctx* c) {
     return PIC_OK;
 }
 
-template <typename T> static int e_half_t(pic_ctx* c) {
+template <typename T> static int e_half_t(pic_ctx* c, double coef) {
     const Grid& g = c->g;
     dim3 grid(blocks_for(g.n[0], 256), g.n[1], g.n[2]);
     k_e_half<T><<<grid, 256, 0, c->stream>>>(
         (T*)c->F[0], (T*)c->F[1], (T*)c->F[2], (const T*)c->F[3], (const T*)c->F[4],
         (const T*)c->F[5], (const T*)c->J[0], (const T*)c->J[1], (const T*)c->J[2], g,
-        (T)(c->p.dt / 2.0), (T)c->p.dx, (T)c->p.dy, (T)c->p.dz);
+        (T)coef, (T)c->p.dx, (T)c->p.dy, (T)c->p.dz);
     PIC_LAUNCH_CHECK(c);
     return launch_wrap(c, 0);
 }
 
-template <typename T> static int b_half_t(pic_ctx* c) {
+template <typename T> static int b_half_t(pic_ctx* c, double coef) {
     const Grid& g = c->g;
     dim3 grid(blocks_for(g.n[0], 256), g.n[1], g.n[2]);
     k_b_half<T><<<grid, 256, 0, c->stream>>>(
         (const T*)c->F[0], (const T*)c->F[1], (const T*)c->F[2], (T*)c->F[3], (T*)c->F[4],
-        (T*)c->F[5], g, (T)(c->p.dt / 2.0), (T)c->p.dx, (T)c->p.dy, (T)c->p.dz);
+        (T*)c->F[5], g, (T)coef, (T)c->p.dx, (T)c->p.dy, (T)c->p.dz);
     PIC_LAUNCH_CHECK(c);
     return launch_wrap(c, 3);
 }
 
-int launch_e_half(pic_ctx* c) { return c->fp32 ? e_half_t<float>(c) : e_half_t<double>(c); }
-int launch_b_half(pic_ctx* c) { return c->fp32 ? b_half_t<float>(c) : b_half_t<double>(c); }
+// dt_half = dt / 2 (constants.cpp:42); the full step of
+// update_electromagnetic_fields (field.cpp:24-54) uses dt.
+int launch_e_update(pic_ctx* c, double coef) { return c->fp32 ? e_half_t<float>(c, coef) : e_half_t<double>(c, coef); }
+int launch_b_update(pic_ctx* c, double coef) { return c->fp32 ? b_half_t<float>(c, coef) : b_half_t<double>(c, coef); }
+int launch_e_half(pic_ctx* c) { return launch_e_update(c, c->p.dt / 2.0); }
+int launch_b_half(pic_ctx* c) { return launch_b_update(c, c->p.dt / 2.0); }
 
-template <typename T> static int wrap_t(pic_ctx* c, int first) {
+template <typename T> static int wrap_t(pic_ctx* c, int first, int count) {
     const Grid& g = c->g;
     for (int a = 0; a < 3; a++) {
         if (!c->p.periodic[a]) continue;
@@ -276,22 +303,24 @@ template <typename T> static int wrap_t(pic_ctx* c, int first) {
         int e1 = a == 2 ? g.fe[1] : g.fe[2];
         int64_t total = (int64_t)2 * g.m * e0 * e1;
         k_wrap<T><<<blocks_for(total, 256), 256, 0, c->stream>>>(
-            (T*)c->F[first], (T*)c->F[first + 1], (T*)c->F[first + 2], g, a);
+            (T*)c->F[first], (T*)c->F[first + (count > 1 ? 1 : 0)], (T*)c->F[first + (count > 1 ? 2 : 0)],
+            g, a, count);
         PIC_LAUNCH_CHECK(c);
     }
     return PIC_OK;
 }
 
-int launch_wrap(pic_ctx* c, int first) { return c->fp32 ? wrap_t<float>(c, first) : wrap_t<double>(c, first); }
+int launch_wrap(pic_ctx* c, int first) { return c->fp32 ? wrap_t<float>(c, first, 3) : wrap_t<double>(c, first, 3); }
+int launch_wrap_one(pic_ctx* c, int comp) { return c->fp32 ? wrap_t<float>(c, comp, 1) : wrap_t<double>(c, comp, 1); }
 
-template <typename T> static int laser_t(pic_ctx* c, int step) {
+template <typename T> static int laser_t(pic_ctx* c, int step, int left, int right) {
     const pic_params& P = c->p;
     LaserArgs L;
     L.pi = P.pi; L.l0 = P.l0; L.waist = P.waist; L.xfocus = P.xfocus; L.yfocus = P.yfocus;
     L.zfocus = P.zfocus; L.omega = P.omega; L.phase = P.phase; L.amp = P.laser_amplitude;
     L.dur = P.laser_pulse_duration; L.t_start = P.laser_start_time; L.lx = P.lx;
     L.dt = P.dt; L.dx = P.dx; L.dy = P.dy; L.dz = P.dz; L.time = step * P.dt;
-    L.pol = P.laser_polarization; L.right = P.laser_right;
+    L.pol = P.laser_polarization; L.left = left; L.right = right;
     dim3 grid(blocks_for(c->g.n[1], 256), c->g.n[2]);
     k_laser<T><<<grid, 256, 0, c->stream>>>((T*)c->F[1], (T*)c->F[2], (const T*)c->F[3],
                                             (T*)c->F[4], (T*)c->F[5], (const T*)c->J[1],
@@ -300,17 +329,21 @@ template <typename T> static int laser_t(pic_ctx* c, int step) {
     return PIC_OK;
 }
 
-int launch_laser(pic_ctx* c, int step) {
+int launch_laser(pic_ctx* c, int step, int left, int right) {
     if (c->p.periodic[0]) return PIC_OK;  // x neighbours exist: no source
-    return c->fp32 ? laser_t<float>(c, step) : laser_t<double>(c, step);
+    return c->fp32 ? laser_t<float>(c, step, left, right) : laser_t<double>(c, step, left, right);
 }
 
-template <typename T> static int fold_t(pic_ctx* c) {
+template <typename T> static int fold_t(pic_ctx* c, void* only) {
     FoldPtrs P{};
     P.na = 0;
-    for (int f = 0; f < 3; f++) P.a[P.na++] = c->J[f];
-    for (int s = 0; s < PIC_MAX_SPECIES; s++)
-        if (c->sp[s].declared && c->rho[s]) P.a[P.na++] = c->rho[s];
+    if (only) {
+        P.a[P.na++] = only;
+    } else {
+        for (int f = 0; f < 3; f++) P.a[P.na++] = c->J[f];
+        for (int s = 0; s < PIC_MAX_SPECIES; s++)
+            if (c->sp[s].declared && c->rho[s]) P.a[P.na++] = c->rho[s];
+    }
     const Grid& g = c->g;
     for (int phase = 0; phase < 2; phase++)
         for (int a = 0; a < 3; a++) {
@@ -324,6 +357,20 @@ template <typename T> static int fold_t(pic_ctx* c) {
     return PIC_OK;
 }
 
-int launch_fold(pic_ctx* c) { return c->fp32 ? fold_t<float>(c) : fold_t<double>(c); }
+int launch_fold(pic_ctx* c) { return c->fp32 ? fold_t<float>(c, nullptr) : fold_t<double>(c, nullptr); }
+int launch_fold_one(pic_ctx* c, void* arr) { return c->fp32 ? fold_t<float>(c, arr) : fold_t<double>(c, arr); }
+
+template <typename T> static int filter_t(pic_ctx* c, void* tmp) {
+    const Grid& g = c->g;
+    dim3 grid(blocks_for(g.n[0], 256), g.n[1], g.n[2]);
+    for (int f = 0; f < 3; f++)
+        for (int a = 0; a < 3; a++) {
+            PIC_HIP(c, hipMemcpyAsync(tmp, c->J[f], g.jsize * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+            k_filter<T><<<grid, 256, 0, c->stream>>>((const T*)tmp, (T*)c->J[f], g, a);
+            PIC_LAUNCH_CHECK(c);
+        }
+    return PIC_OK;
+}
+int launch_filter(pic_ctx* c, void* tmp) { return c->fp32 ? filter_t<float>(c, tmp) : filter_t<double>(c, tmp); }
 
 }  // namespace pic

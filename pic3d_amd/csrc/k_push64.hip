@@ -292,6 +292,39 @@ __device__ __forceinline__ int push_one_hbm(const P64& P, int64_t p, double wp, 
 }
 
 // ---------------------------------------------------------------------
+// currents_particle (particle.cpp:372-515) alone: the Esirkepov deposit of
+// particles whose old and new positions are given, through HBM atomics.
+// A particle is skipped (and counted) when its old position lies outside
+// [0, N] on an axis or it moved a cell or more: its boxes would leave the
+// ghosted arrays.
+// ---------------------------------------------------------------------
+template <int M>
+__global__ __launch_bounds__(256) void k_deposit64(const double* __restrict__ xo,
+                                                   const double* __restrict__ yo,
+                                                   const double* __restrict__ zo,
+                                                   const double* __restrict__ xn,
+                                                   const double* __restrict__ yn,
+                                                   const double* __restrict__ zn,
+                                                   const double* __restrict__ w, int64_t n,
+                                                   Fields64 F, Grid g, PushConsts K,
+                                                   int64_t* __restrict__ skipped) {
+    int64_t bad = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const double x = xo[p], y = yo[p], z = zo[p];
+        const bool inside = x >= 0.0 && x <= (double)K.N[0] && y >= 0.0 && y <= (double)K.N[1] &&
+                            z >= 0.0 && z <= (double)K.N[2];
+        if (!inside || !(fabs(xn[p] - x) < 1.0 && fabs(yn[p] - y) < 1.0 && fabs(zn[p] - z) < 1.0)) {
+            bad++;
+            continue;
+        }
+        HbmSink d{{F.jx, F.jy, F.jz, F.rho}, g, 0, 0, 0};
+        esirkepov<M>(K.q * w[p], x, y, z, xn[p], yn[p], zn[p], K, g.m, d);
+    }
+    if (bad) atomicAdd((unsigned long long*)skipped, (unsigned long long)bad);
+}
+
+// ---------------------------------------------------------------------
 // direct kernel: any particle order, global atomics
 // ---------------------------------------------------------------------
 template <int M>
@@ -445,6 +478,24 @@ static int launch_tiled64(pic_ctx* c, Species& S, const P64& P, const Fields64& 
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
     k_push64_tiled<M, TS::X, TS::Y, TS::Z><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
         P, Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+int launch_deposit(pic_ctx* c, int s, int64_t n, const double* const* d6, const double* w,
+                   int64_t* d_skipped) {
+    Species& S = c->sp[s];
+    Fields64 Fd{(const double*)c->F[0], (const double*)c->F[1], (const double*)c->F[2],
+                (const double*)c->F[3], (const double*)c->F[4], (const double*)c->F[5],
+                (double*)c->J[0], (double*)c->J[1], (double*)c->J[2], (double*)c->rho[s]};
+    const PushConsts K = make_consts(c, S);
+    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(n, 1), 256), 4096);
+    if (c->g.m == 2)
+        k_deposit64<2><<<blocks, 256, 0, c->stream>>>(d6[0], d6[1], d6[2], d6[3], d6[4], d6[5], w, n,
+                                                       Fd, c->g, K, d_skipped);
+    else
+        k_deposit64<4><<<blocks, 256, 0, c->stream>>>(d6[0], d6[1], d6[2], d6[3], d6[4], d6[5], w, n,
+                                                       Fd, c->g, K, d_skipped);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }

@@ -318,6 +318,7 @@ int pic_ctx_destroy(pic_ctx* c) {
     for (int f = 0; f < 6; f++) if (c->F[f]) hipFree(c->F[f]);
     for (int f = 0; f < 3; f++) if (c->J[f]) hipFree(c->J[f]);
     dfree(c->tile_cnt); dfree(c->tile_cur); dfree(c->work); dfree(c->nwork); dfree(c->diag_buf);
+    dfree(c->jtmp);
     for (auto& e : c->tev) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -590,7 +591,81 @@ int pic_num_particles(pic_ctx* c, int s, int64_t* n) {
 int pic_laser_boundary(pic_ctx* c, int step) {
     if (!c) return PIC_ERR_ARG;
     PhaseTimer t(c, PIC_T_FIELDS);
-    return launch_laser(c, step);
+    return launch_laser(c, step, 1, c->p.laser_right);
+}
+
+int pic_radiating_boundary(pic_ctx* c, int step, int is_left, int is_right) {
+    if (!c) return PIC_ERR_ARG;
+    PhaseTimer t(c, PIC_T_FIELDS);
+    return launch_laser(c, step, is_left ? 1 : 0, is_right ? 1 : 0);
+}
+
+int pic_fields_full(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    PhaseTimer t(c, PIC_T_FIELDS);
+    int r = launch_e_update(c, c->p.dt);
+    return r ? r : launch_b_update(c, c->p.dt);
+}
+
+int pic_field_exchange(pic_ctx* c, int comp) {
+    if (!c) return PIC_ERR_ARG;
+    if (comp < PIC_EX || comp > PIC_BZ) return set_err(c, PIC_ERR_ARG, "field exchange: %d is not an E/B component", comp);
+    PhaseTimer t(c, PIC_T_FIELDS);
+    return launch_wrap_one(c, comp);
+}
+
+int pic_current_exchange(pic_ctx* c, int comp) {
+    if (!c) return PIC_ERR_ARG;
+    void* arr = nullptr;
+    if (comp >= PIC_JX && comp <= PIC_JZ) arr = c->J[comp - PIC_JX];
+    else if (comp >= PIC_RHO0 && comp <= PIC_RHO3) arr = c->rho[comp - PIC_RHO0];
+    else return set_err(c, PIC_ERR_ARG, "current exchange: %d is not a J/rho component", comp);
+    if (!arr) return PIC_OK;  // species without a charge array
+    PhaseTimer t(c, PIC_T_FOLD);
+    return launch_fold_one(c, arr);
+}
+
+int pic_filter_currents(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    if (!c->jtmp) {
+        int r = dmalloc(c, (char**)&c->jtmp, c->g.jsize * (c->fp32 ? 4 : 8));
+        if (r) return r;
+    }
+    return launch_filter(c, c->jtmp);
+}
+
+int pic_deposit(pic_ctx* c, int s, int64_t n, const double* x_old, const double* y_old,
+                const double* z_old, const double* x, const double* y, const double* z,
+                const double* w, int64_t* skipped) {
+    if (!valid_species(c, s) || n < 0) return PIC_ERR_ARG;
+    if (n > 0 && (!x_old || !y_old || !z_old || !x || !y || !z || !w)) return PIC_ERR_ARG;
+    if (c->fp32) return set_err(c, PIC_ERR_STATE, "pic_deposit is an FP64 entry point");
+    Species& S = c->sp[s];
+    if (S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is immobile (no charge array)", s);
+    if (skipped) *skipped = 0;
+    if (n == 0) return PIC_OK;
+    PIC_HIP(c, hipSetDevice(c->device));
+    char* buf = nullptr;
+    const int64_t bytes = n * 8;
+    int r = dmalloc(c, &buf, 7 * bytes + 256);
+    if (r) return r;
+    const double* src[7] = {x_old, y_old, z_old, x, y, z, w};
+    const double* d[7];
+    for (int i = 0; i < 7 && !r; i++) {
+        d[i] = (const double*)(buf + i * bytes);
+        if (hipMemcpyAsync((void*)d[i], src[i], bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            r = set_err(c, PIC_ERR_HIP, "pic_deposit: upload failed");
+    }
+    int64_t* d_skip = (int64_t*)(buf + 7 * bytes);
+    if (!r) r = launch_fill(c, d_skip, 16);
+    if (!r) r = launch_deposit(c, s, n, d, d[6], d_skip);
+    int64_t h = 0;
+    if (!r && hipMemcpyAsync(&h, d_skip, 8, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+        r = set_err(c, PIC_ERR_HIP, "pic_deposit: readback failed");
+    if (!r && hipStreamSynchronize(c->stream) != hipSuccess) r = set_err(c, PIC_ERR_HIP, "pic_deposit failed");
+    dfree(buf);
+    if (!r && skipped) *skipped = h;
+    return r;
 }
 
 int pic_fields_first(pic_ctx* c) {

@@ -98,6 +98,7 @@ struct pic_ctx {
     int push_kernel = 0;          // 0 auto, 1 naive (global atomics), 2 tiled
     int keep_old = 0;
     int fix32 = 0;                // FP32: int32 fixed-point LDS accumulators
+    void* jtmp = nullptr;         // J-sized scratch of the current filter (lazy)
     // per-phase timing (PIC_OPT_TIMING): event pairs around each phase
     int timing = 0;
     std::vector<hipEvent_t> tev;      // event pool
@@ -125,9 +126,16 @@ int set_err(pic_ctx* c, int code, const char* fmt, ...);
 int launch_clear(pic_ctx* c);
 int launch_e_half(pic_ctx* c);
 int launch_b_half(pic_ctx* c);
+int launch_e_update(pic_ctx* c, double coef);  // E += coef (curl B - J), then wrap
+int launch_b_update(pic_ctx* c, double coef);  // B -= coef curl E, then wrap
 int launch_wrap(pic_ctx* c, int first_comp);  // wraps comps first..first+2
-int launch_laser(pic_ctx* c, int step);
+int launch_wrap_one(pic_ctx* c, int comp);
+int launch_laser(pic_ctx* c, int step, int left, int right);
 int launch_fold(pic_ctx* c);
+int launch_fold_one(pic_ctx* c, void* arr);
+int launch_filter(pic_ctx* c, void* tmp);  // tmp: one J-sized scratch array
+int launch_deposit(pic_ctx* c, int s, int64_t n, const double* const* d6, const double* w,
+                   int64_t* d_skipped);
 int launch_push(pic_ctx* c, int s);
 int launch_boundary(pic_ctx* c, int s);
 int launch_sort(pic_ctx* c, int s);

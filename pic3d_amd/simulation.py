@@ -139,6 +139,32 @@ class Simulation:
     def clear_currents(self):
         self._chk(self._L.pic_clear_currents(self._h), "pic_clear_currents")
 
+    def radiating_boundary(self, step: int, is_left: int = 1, is_right: int = 0):
+        self._chk(self._L.pic_radiating_boundary(self._h, step, is_left, is_right),
+                  "pic_radiating_boundary")
+
+    def fields_full(self):
+        self._chk(self._L.pic_fields_full(self._h), "pic_fields_full")
+
+    def field_exchange(self, comp: int):
+        self._chk(self._L.pic_field_exchange(self._h, comp), "pic_field_exchange")
+
+    def current_exchange(self, comp: int):
+        self._chk(self._L.pic_current_exchange(self._h, comp), "pic_current_exchange")
+
+    def filter_currents(self):
+        self._chk(self._L.pic_filter_currents(self._h), "pic_filter_currents")
+
+    def deposit(self, s: int, parts: dict) -> int:
+        """currents_particle for every particle of parts (x_old.., x.., w)."""
+        a = {k: np.ascontiguousarray(parts[k], dtype=np.float64)
+             for k in ("x_old", "y_old", "z_old", "x", "y", "z", "w")}
+        skipped = C.c_int64()
+        self._chk(self._L.pic_deposit(self._h, s, len(a["x"]), _d(a["x_old"]), _d(a["y_old"]),
+                                      _d(a["z_old"]), _d(a["x"]), _d(a["y"]), _d(a["z"]),
+                                      _d(a["w"]), C.byref(skipped)), "pic_deposit")
+        return skipped.value
+
     def push(self, s: int):
         self._chk(self._L.pic_push(self._h, s), "pic_push")
 

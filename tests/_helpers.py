@@ -50,6 +50,20 @@ def replay(sim, cmds):
             sim.particle_boundary(int(c[4:]))
         elif c == "fold":
             sim.fold_currents()
+        elif c.startswith("rad:"):
+            _, st, l, r = c.split(":")
+            sim.radiating_boundary(int(st), int(l), int(r))
+        elif c == "full":
+            sim.fields_full()
+        elif c == "filter":
+            sim.filter_currents()
+        elif c.startswith("xfield:"):
+            sim.field_exchange(int(c[7:]))
+        elif c.startswith("xcur:"):
+            sim.current_exchange(6 + int(c[5:]))  # jx jy jz rho0 rho1
+        elif c.startswith("dep:"):
+            s = int(c[4:])
+            sim.deposit(s, sim.particles_get(s))
         else:
             raise ValueError(c)
 

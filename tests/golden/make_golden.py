@@ -54,6 +54,25 @@ def cases():
     out.append(("phases_m2_12x10x10", wl,
                 ["laser:1", "first", "clear", "push:0", "bnd:0", "fold", "second",
                  "laser:2", "first", "clear", "push:0"]))
+    # the rest of the per-call surface: both laser sources, the full-dt
+    # field update, single-array exchanges, the binomial filter and the
+    # stand-alone deposit (quadratic and quartic shapes)
+    for mem in (2, 4):
+        wl = W.laser_slab(nx=12, ny=10, nz=10, ppc=2, slab=(2.0, 10.0), seed=16 + mem, mem=mem)
+        wl.species[0].loader.u_th = 0.4
+        wl.params.laser_amplitude = 3.0
+        out.append((f"surface_m{mem}_12x10x10", wl,
+                    ["rad:3:1:1", "first", "clear", "push:0", "bnd:0", "fold", "second",
+                     "rad:4:0:1", "full", "xfield:1", "xfield:5", "filter", "xcur:0",
+                     "xcur:3", "clear", "dep:0", "xcur:2", "rad:5:1:0", "full"]))
+    # the pre-loop push + exchange alone, hot enough that many particles
+    # cross faces: the vector order after the exchange is the reference's
+    wl = W.thermal(n=8, ppc=2, seed=21)
+    wl.species[0].loader.u_th = 0.9
+    out.append(("preloop_m2_8", wl, ["pre"]))
+    wl = W.laser_slab(nx=12, ny=10, nz=10, ppc=2, slab=(0.0, 12.0), seed=22, mem=4)
+    wl.species[0].loader.u_th = 0.9
+    out.append(("preloop_m4_open_12x10x10", wl, ["pre"]))
     return out
 
 

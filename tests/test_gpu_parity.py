@@ -52,7 +52,8 @@ def test_golden_vs_reference(name, kernel):
     """GPU vs the reference's own output (multi-step): 1e-10 norm-wise."""
     print(f"golden {name} kernel {kernel}", flush=True)
     g = load_golden(name)
-    sim = make_sim(g["params"], g["charge"], g["mass"], g["parts"], kernel=kernel)
+    dep = any(c.startswith("dep:") for c in g["cmds"])  # deposit needs x_old
+    sim = make_sim(g["params"], g["charge"], g["mass"], g["parts"], kernel=kernel, keep_old=dep)
     sim.set_background(g["rho_background"])
     replay(sim, g["cmds"])
     for k, ref in g["fields"].items():

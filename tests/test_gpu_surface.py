@@ -1,0 +1,86 @@
+"""GPU vs oracle for the per-call surface beyond the time loop: the full-dt
+field update (field.cpp:24-54), single-array field and current exchanges
+(mpi_utils.cpp:9-60, 232-417), the binomial current filter
+(particle.cpp:520-574), radiating_boundary with either source switched
+(boundary.cpp:42-99) and the stand-alone deposit (currents_particle,
+particle.cpp:372-515).
+
+Bars: bit-exact for the field update, exchanges and filter (same FP64
+operations in the same order); 1e-12 norm-wise for the laser (device libm)
+and the deposit (atomic summation order).
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, load_particles
+from pic3d_amd import Simulation, workloads as W
+from pic3d_amd._abi import FIELD_NAMES
+from pic3d_amd.simulation import field_shape
+
+from _helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(mem, periodic=(0, 1, 1), seed=3):
+    wl = W.laser_slab(nx=14, ny=12, nz=10, ppc=2, slab=(1.0, 13.0), seed=seed, mem=mem)
+    p = wl.params
+    for a in range(3):
+        p.periodic[a] = periodic[a]
+    rng = np.random.default_rng(seed)
+    sim, orc = Simulation(p, keep_old=True), Oracle(p, 1)
+    for c in range(9):
+        a = rng.normal(0, 0.5, field_shape(p, c))
+        sim.field_set(c, a)
+        orc.field_set(c, a)
+    return wl, sim, orc
+
+
+@pytest.mark.parametrize("mem", [2, 4])
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 1), (0, 0, 1)])
+def test_fields_full_and_exchanges(mem, periodic):
+    print(f"surface fields mem {mem} periodic {periodic}", flush=True)
+    _, sim, orc = pair(mem, periodic)
+    sim.fields_full(); orc.fields_full()
+    for comp in (0, 4):
+        sim.field_exchange(comp); orc.field_exchange(comp)
+    for comp in (6, 8):
+        sim.current_exchange(comp); orc.current_exchange(comp)
+    sim.filter_currents(); orc.filter_currents()
+    for c in range(9):
+        assert np.array_equal(sim.field_get(c), orc.field_get(c)), FIELD_NAMES[c]
+
+
+@pytest.mark.parametrize("left,right", [(1, 0), (0, 1), (1, 1), (0, 0)])
+def test_radiating_boundary_sources(left, right):
+    print(f"surface laser {left} {right}", flush=True)
+    _, sim, orc = pair(4)
+    for step in (3, 30):
+        sim.radiating_boundary(step, left, right)
+        orc.radiating_boundary(step, left, right)
+    for c in range(6):
+        a, b = sim.field_get(c), orc.field_get(c)
+        assert rel_err(a, b) <= 1e-12, (FIELD_NAMES[c], rel_err(a, b))
+
+
+@pytest.mark.parametrize("mem", [2, 4])
+def test_standalone_deposit(mem):
+    print(f"surface deposit mem {mem}", flush=True)
+    wl, sim, orc = pair(mem, (1, 1, 1))
+    parts = load_particles(wl.params, wl.species[0].loader)
+    n = len(parts["x"])
+    rng = np.random.default_rng(7)
+    d = {"x_old": parts["x"], "y_old": parts["y"], "z_old": parts["z"], "w": parts["w"]}
+    for k, a in (("x", "x_old"), ("y", "y_old"), ("z", "z_old")):
+        d[k] = d[a] + rng.uniform(-0.9, 0.9, n)
+    d["x"][:5] = d["x_old"][:5] + 1.5   # moved a cell or more: skipped
+    sim.species_set(0, -1.0, 1.0, False, n)
+    orc.species_set(0, -1.0, 1.0)
+    for c in (6, 7, 8):
+        z = np.zeros(field_shape(wl.params, c))
+        sim.field_set(c, z); orc.field_set(c, z)
+    sk_g, sk_o = sim.deposit(0, d), orc.deposit(0, d)
+    assert sk_g == sk_o == 5
+    for c in (6, 7, 8, 9):
+        a, b = sim.field_get(c), orc.field_get(c)
+        assert rel_err(a, b) <= 1e-12, (FIELD_NAMES[c], rel_err(a, b))
