@@ -246,6 +246,47 @@ int pic_deposit(pic_ctx* ctx, int s, int64_t n, const double* x_old, const doubl
                 const double* z_old, const double* x, const double* y, const double* z,
                 const double* w, int64_t* skipped);
 
+/* ---- differentiation form (no counterpart in the reference) ----
+ *
+ * Gradients of the final kinetic energy of one species,
+ *   K = sum_p m w_p (gamma_p - 1) dx dy dz   (pic_diag_out.kinetic[s]),
+ * after nsteps steps of the time loop, with respect to
+ *   - the x-min laser envelope value of every step, e_n (the value
+ *     generate_laser_pulse(t_n, 0) returns, boundary.cpp:8-27; the source
+ *     is Gpulse with e_n in its place), and
+ *   - the initial weight of every particle slot (the density profile).
+ * FP64 only, one mobile species (others immobile).  The run uses the
+ * unsorted push so particle slots stay fixed; particles leaving through
+ * open faces are marked dead in place.  The forward keeps a full-state
+ * checkpoint every K steps; the backward recomputes each K-step segment
+ * from its checkpoint, saving the per-step push inputs, and runs the
+ * adjoint kernels of that segment in reverse.  Memory: one checkpoint per
+ * K steps (fields, currents, particles) plus K push-input states. */
+
+/* Snapshot the current state of the context as the initial state of a
+ * differentiated run of species s. */
+int pic_diff_begin(pic_ctx* ctx, int s);
+/* Number of particle slots (entries of the weight arrays below). */
+int pic_diff_slots(pic_ctx* ctx, int64_t* slots);
+/* Initial positions and weights of the slots (NaN weight: a dead slot);
+ * any pointer may be NULL.  Synchronises. */
+int pic_diff_initial(pic_ctx* ctx, double* x, double* y, double* z, double* w);
+/* From the initial state (with 'weights' in place of the initial weights
+ * when not NULL; dead slots stay dead) run steps first_step ..
+ * first_step+nsteps-1 with the x-min envelope 'envelope' (nsteps values;
+ * NULL = the reference's sin envelope), checkpointing every K steps, and
+ * return K in *objective.  Synchronises. */
+int pic_diff_forward(pic_ctx* ctx, int first_step, int nsteps, int K, const double* envelope,
+                     const double* weights, double* objective);
+/* Reverse of the last pic_diff_forward: grad_objective * dK/de_n into
+ * grad_envelope (nsteps, may be NULL) and grad_objective * dK/dw into
+ * grad_weights (slots, may be NULL).  Leaves the context at the run's
+ * initial state.  Synchronises. */
+int pic_diff_backward(pic_ctx* ctx, double grad_objective, double* grad_envelope,
+                      double* grad_weights);
+/* Release the differentiated run's buffers. */
+int pic_diff_end(pic_ctx* ctx);
+
 /* ---- readback / diagnostics ---- */
 
 /* Full ghosted array of component comp, as doubles in the reference layout. */

@@ -120,6 +120,12 @@ SIGNATURES = {
     "pic_field_exchange": (C.c_int, [C.c_void_p, C.c_int]),
     "pic_current_exchange": (C.c_int, [C.c_void_p, C.c_int]),
     "pic_filter_currents": (C.c_int, [C.c_void_p]),
+    "pic_diff_begin": (C.c_int, [C.c_void_p, C.c_int]),
+    "pic_diff_slots": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "pic_diff_initial": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    "pic_diff_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "pic_diff_backward": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
+    "pic_diff_end": (C.c_int, [C.c_void_p]),
     "pic_deposit": (C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7 + [C.POINTER(C.c_int64)]),
     "pic_field_get": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "pic_field_set": (C.c_int, [C.c_void_p, C.c_int, _dp]),

@@ -1,0 +1,944 @@
+// k_adj.hip -- differentiation form of the FP64 hot loop.
+//
+// Reverse-mode (adjoint) kernels for every operation of one time step --
+// Yee half steps, halo wraps, current fold, laser/Silver-Mueller boundary,
+// and the particle push (gather, Boris, position update, Esirkepov
+// deposit) -- and the checkpoint/recompute driver behind pic_diff_*.  The
+// objective is the final kinetic energy of one species (the FP64
+// diagnostic sum m w (gamma - 1) dV); the parameters are the x-min laser
+// envelope value of every step (what generate_laser_pulse(t, 0) returns,
+// boundary.cpp:8-27) and the initial particle weights (the density
+// profile the particles sample).
+//
+// The field operators are linear; their adjoints are the transposed
+// stencils, written as gathers over the adjoint arrays (no atomics).  The
+// particle adjoint recomputes the step of each particle from its saved
+// state and differentiates it: dual numbers carry d/d(x0,y0,z0) through the
+// gather, d/d(u0,E,B) through Boris and d/d(x0..z1) through the deposit;
+// the field adjoint of the gather is scattered with HBM atomics.
+//
+// Particles keep their slots during a differentiated run (the unsorted
+// push; the boundary marks leavers dead without compacting), so slot p at
+// step n and slot p at step n+1 are the same particle.
+#include <algorithm>
+#include <cmath>
+
+#include "atomics.h"
+#include "laser.h"
+#include "pic_internal.h"
+#include "push64.h"
+#include "shapes.h"
+
+namespace pic {
+
+// ---------------------------------------------------------------------
+// forward-mode dual numbers
+// ---------------------------------------------------------------------
+template <int N> struct Dual {
+    double v;
+    double d[N];
+    __device__ Dual() {}
+    __device__ Dual(double x) : v(x) {
+#pragma unroll
+        for (int i = 0; i < N; i++) d[i] = 0.0;
+    }
+};
+template <int N> __device__ __forceinline__ Dual<N> dvar(double x, int k) {
+    Dual<N> r(x);
+    r.d[k] = 1.0;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r(a.v + b.v);
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = a.d[i] + b.d[i];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r(a.v - b.v);
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = a.d[i] - b.d[i];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a) {
+    Dual<N> r(-a.v);
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = -a.d[i];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) {
+    Dual<N> r(a.v * b.v);
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
+    const double inv = 1.0 / b.v;
+    Dual<N> r(a.v * inv);
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * inv;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, double b) { return a + Dual<N>(b); }
+template <int N> __device__ __forceinline__ Dual<N> operator+(double a, const Dual<N>& b) { return Dual<N>(a) + b; }
+template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, double b) { return a - Dual<N>(b); }
+template <int N> __device__ __forceinline__ Dual<N> operator-(double a, const Dual<N>& b) { return Dual<N>(a) - b; }
+template <int N> __device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, double b) {
+    Dual<N> r(a.v * b);
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = a.d[i] * b;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dual<N> operator*(double a, const Dual<N>& b) { return b * a; }
+template <int N> __device__ __forceinline__ Dual<N> operator/(const Dual<N>& a, double b) { return a * (1.0 / b); }
+template <int N> __device__ __forceinline__ Dual<N> operator/(double a, const Dual<N>& b) { return Dual<N>(a) / b; }
+template <int N> __device__ __forceinline__ Dual<N>& operator+=(Dual<N>& a, const Dual<N>& b) { return a = a + b; }
+template <int N> __device__ __forceinline__ Dual<N> sqrt(const Dual<N>& a) {
+    const double s = ::sqrt(a.v);
+    Dual<N> r(s);
+    const double k = 0.5 / s;
+#pragma unroll
+    for (int i = 0; i < N; i++) r.d[i] = a.d[i] * k;
+    return r;
+}
+__device__ __forceinline__ double val(double x) { return x; }
+template <int N> __device__ __forceinline__ double val(const Dual<N>& x) { return x.v; }
+
+template <int M, typename T> __device__ __forceinline__ int shape_at_t(T x, T* W) {
+    const int ic = (int)(val(x) + 0.5);
+    if (M == 2) shape2_delta<T>(x - T((double)ic), W);
+    else shape4_d<T>(x - T((double)(ic - 2)), W);
+    return ic;
+}
+
+// Boris rotation of particle.cpp:271-311 (E and B as gathered, before the
+// q dt / 2m scaling)
+template <typename T>
+__device__ __forceinline__ void boris_t(const T* u, const T* E, const T* B, double qm, T* out) {
+    const T e0 = E[0] * qm, e1 = E[1] * qm, e2 = E[2] * qm;
+    const T b0 = B[0] * qm, b1 = B[1] * qm, b2 = B[2] * qm;
+    const T px1 = u[0] + e0, py1 = u[1] + e1, pz1 = u[2] + e2;
+    const T g1 = sqrt(1.0 + (px1 * px1 + py1 * py1 + pz1 * pz1));
+    const T tx = b0 / g1, ty = b1 / g1, tz = b2 / g1;
+    const T tt = 2.0 / (1.0 + tx * tx + ty * ty + tz * tz);
+    const T sx = tx * tt, sy = ty * tt, sz = tz * tt;
+    const T px2 = px1 + (py1 * tz - pz1 * ty);
+    const T py2 = py1 + (pz1 * tx - px1 * tz);
+    const T pz2 = pz1 + (px1 * ty - py1 * tx);
+    out[0] = px1 + (py2 * sz - pz2 * sy) + e0;
+    out[1] = py1 + (pz2 * sx - px2 * sz) + e1;
+    out[2] = pz1 + (px2 * sy - py2 * sx) + e2;
+}
+
+// Esirkepov deposit of particle.cpp:372-500 (J only: rho feeds nothing)
+// as a function of the six positions; 'visit(c, k, j, i, value)' gets the
+// contribution to component c at buffer node (k, j, i).
+template <int M, typename T, typename V>
+__device__ __forceinline__ void esirkepov_t(double q, T xo, T yo, T zo, T xn, T yn, T zn,
+                                            const PushConsts& K, int m, V&& visit) {
+    constexpr int H = M / 2, L = M + 2;
+    const double org = (double)(1 - m);
+    T Sx0[M + 1], Sx1[M + 1], Sy0[M + 1], Sy1[M + 1], Sz0[M + 1], Sz1[M + 1];
+    const int i1 = shape_at_t<M>(xo - org, Sx0), i2 = shape_at_t<M>(xn - org, Sx1);
+    const int j1 = shape_at_t<M>(yo - org, Sy0), j2 = shape_at_t<M>(yn - org, Sy1);
+    const int k1 = shape_at_t<M>(zo - org, Sz0), k2 = shape_at_t<M>(zn - org, Sz1);
+    const int bx = min(i1, i2) - H, by = min(j1, j2) - H, bz = min(k1, k2) - H;
+    T X0[L], X1[L], Y0[L], Y1[L], Z0[L], Z1[L];
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        const int a0 = t - (i1 - H - bx), a1 = t - (i2 - H - bx);
+        X0[t] = (a0 >= 0 && a0 <= M) ? Sx0[a0] : T(0.0);
+        X1[t] = (a1 >= 0 && a1 <= M) ? Sx1[a1] : T(0.0);
+        const int b0 = t - (j1 - H - by), b1 = t - (j2 - H - by);
+        Y0[t] = (b0 >= 0 && b0 <= M) ? Sy0[b0] : T(0.0);
+        Y1[t] = (b1 >= 0 && b1 <= M) ? Sy1[b1] : T(0.0);
+        const int c0 = t - (k1 - H - bz), c1 = t - (k2 - H - bz);
+        Z0[t] = (c0 >= 0 && c0 <= M) ? Sz0[c0] : T(0.0);
+        Z1[t] = (c1 >= 0 && c1 <= M) ? Sz1[c1] : T(0.0);
+    }
+    const double cx = q * K.dx / K.dt, cy = q * K.dy / K.dt, cz = q * K.dz / K.dt;
+    const double third = 1.0 / 3.0;
+    for (int k = 0; k < L; k++)
+        for (int j = 0; j < L; j++) {
+            const T sy0 = Y0[j], dsy = Y1[j] - sy0, sz0 = Z0[k], dsz = Z1[k] - sz0;
+            const T wyz = sy0 * sz0 + 0.5 * dsy * sz0 + 0.5 * sy0 * dsz + third * dsy * dsz;
+            T acc(0.0);
+            for (int i = 0; i < L; i++) {
+                acc += X1[i] - X0[i];
+                visit(0, bz + k, by + j, bx + i, -(cx * acc * wyz));
+            }
+        }
+    for (int k = 0; k < L; k++)
+        for (int i = 0; i < L; i++) {
+            const T sx0 = X0[i], dsx = X1[i] - sx0, sz0 = Z0[k], dsz = Z1[k] - sz0;
+            const T wxz = sx0 * sz0 + 0.5 * dsx * sz0 + 0.5 * sx0 * dsz + third * dsx * dsz;
+            T acc(0.0);
+            for (int j = 0; j < L; j++) {
+                acc += Y1[j] - Y0[j];
+                visit(1, bz + k, by + j, bx + i, -(cy * acc * wxz));
+            }
+        }
+    for (int j = 0; j < L; j++)
+        for (int i = 0; i < L; i++) {
+            const T sx0 = X0[i], dsx = X1[i] - sx0, sy0 = Y0[j], dsy = Y1[j] - sy0;
+            const T wxy = sx0 * sy0 + 0.5 * dsx * sy0 + 0.5 * sx0 * dsy + third * dsx * dsy;
+            T acc(0.0);
+            for (int k = 0; k < L; k++) {
+                acc += Z1[k] - Z0[k];
+                visit(2, bz + k, by + j, bx + i, -(cz * acc * wxy));
+            }
+        }
+}
+
+// ---------------------------------------------------------------------
+// particle adjoint
+// ---------------------------------------------------------------------
+struct AdjPush {
+    // state at the start of the push (slot order)
+    const double *x, *y, *z, *ux, *uy, *uz, *w;
+    // fields at the push
+    const double* F[6];
+    // adjoint of J after the fold transpose (read) and of E/B (scatter)
+    const double* lJ[3];
+    double* lF[6];
+    // per-particle adjoint: in = after the push, out = before it
+    double *lx, *ly, *lz, *lux, *luy, *luz;
+    double* lw;  // accumulated d objective / d w
+    int64_t n;
+};
+
+template <int M>
+__global__ __launch_bounds__(256) void k_adj_push64(AdjPush A, Grid g, PushConsts K) {
+    constexpr int H = M / 2;
+    using D3 = Dual<3>;
+    using D9 = Dual<9>;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const double w = A.w[p];
+        if (!(w == w)) continue;  // dead slot
+        double x = A.x[p], y = A.y[p], z = A.z[p];
+        if (!wrap_check(x, y, z, K)) continue;  // not pushed: adjoint passes through
+        // ---- gather at (x, y, z), with d/d(x, y, z)
+        const double org = (double)(1 - g.m);
+        const D3 xc = dvar<3>(x - org, 0), yc = dvar<3>(y - org, 1), zc = dvar<3>(z - org, 2);
+        D3 Wxp[M + 1], Wyp[M + 1], Wzp[M + 1], Wxd[M + 1], Wyd[M + 1], Wzd[M + 1];
+        const int ip = shape_at_t<M>(xc, Wxp), jp = shape_at_t<M>(yc, Wyp), kp = shape_at_t<M>(zc, Wzp);
+        shape_at_t<M>(xc - 0.5, Wxd);
+        shape_at_t<M>(yc - 0.5, Wyd);
+        shape_at_t<M>(zc - 0.5, Wzd);
+        const int id = (int)xc.v, jd = (int)yc.v, kd = (int)zc.v;
+        const D3* WX[6] = {Wxd, Wxp, Wxp, Wxp, Wxd, Wxd};
+        const D3* WY[6] = {Wyp, Wyd, Wyp, Wyd, Wyp, Wyd};
+        const D3* WZ[6] = {Wzp, Wzp, Wzd, Wzd, Wzd, Wzp};
+        const int I0[6] = {id, ip, ip, ip, id, id}, J0[6] = {jp, jd, jp, jd, jp, jd},
+                  K0[6] = {kp, kp, kd, kd, kd, kp};
+        D3 Fp[6];
+        for (int c = 0; c < 6; c++) {
+            D3 e(0.0);
+            for (int k = 0; k <= M; k++)
+                for (int j = 0; j <= M; j++)
+                    for (int i = 0; i <= M; i++)
+                        e += WX[c][i] * WY[c][j] * WZ[c][k] *
+                             A.F[c][g.fi(K0[c] + k - H, J0[c] + j - H, I0[c] + i - H)];
+            Fp[c] = e;
+        }
+        // ---- Boris with d/d(u0, E, B)
+        D9 u0[3] = {dvar<9>(A.ux[p], 0), dvar<9>(A.uy[p], 1), dvar<9>(A.uz[p], 2)};
+        D9 E9[3] = {dvar<9>(Fp[0].v, 3), dvar<9>(Fp[1].v, 4), dvar<9>(Fp[2].v, 5)};
+        D9 B9[3] = {dvar<9>(Fp[3].v, 6), dvar<9>(Fp[4].v, 7), dvar<9>(Fp[5].v, 8)};
+        D9 u1[3];
+        boris_t<D9>(u0, E9, B9, K.qm, u1);
+        const double v1[3] = {u1[0].v, u1[1].v, u1[2].v};
+        const double gam = ::sqrt(1.0 + (v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]));
+        const double dtd[3] = {K.dt / K.dx, K.dt / K.dy, K.dt / K.dz};
+        const double xn = x + v1[0] / gam * K.dt / K.dx;
+        const double yn = y + v1[1] / gam * K.dt / K.dy;
+        const double zn = z + v1[2] / gam * K.dt / K.dz;
+        // ---- adjoint of the state after the push
+        double lx1[3] = {A.lx[p], A.ly[p], A.lz[p]};
+        double lu1[3] = {A.lux[p], A.luy[p], A.luz[p]};
+        double lx0[3] = {0.0, 0.0, 0.0};
+        double gw = 0.0;
+        // ---- deposit (skipped by the forward for moves of a cell or more)
+        if (fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0) {
+            using D6 = Dual<6>;
+            D6 G(0.0);
+            const double* lJ0 = A.lJ[0];
+            const double* lJ1 = A.lJ[1];
+            const double* lJ2 = A.lJ[2];
+            esirkepov_t<M, D6>(K.q, dvar<6>(x, 0), dvar<6>(y, 1), dvar<6>(z, 2), dvar<6>(xn, 3),
+                               dvar<6>(yn, 4), dvar<6>(zn, 5), K, g.m,
+                               [&](int c, int k, int j, int i, const D6& v) {
+                                   const double* l = c == 0 ? lJ0 : (c == 1 ? lJ1 : lJ2);
+                                   G += v * l[g.ji(k, j, i)];
+                               });
+            for (int a = 0; a < 3; a++) {
+                lx0[a] += w * G.d[a];
+                lx1[a] += w * G.d[3 + a];
+            }
+            gw = G.v;
+        }
+        // ---- x1 = x0 + u1 / gamma(u1) * dt / d
+        const double ig = 1.0 / gam, ig3 = ig * ig * ig;
+        for (int b = 0; b < 3; b++) {
+            double s = 0.0;
+            for (int a = 0; a < 3; a++)
+                s += lx1[a] * dtd[a] * ((a == b ? ig : 0.0) - v1[a] * v1[b] * ig3);
+            lu1[b] += s;
+        }
+        for (int a = 0; a < 3; a++) lx0[a] += lx1[a];
+        // ---- Boris transpose
+        double lu0[3], lF[6];
+        for (int b = 0; b < 9; b++) {
+            const double s = lu1[0] * u1[0].d[b] + lu1[1] * u1[1].d[b] + lu1[2] * u1[2].d[b];
+            if (b < 3) lu0[b] = s;
+            else lF[b - 3] = s;
+        }
+        // ---- gather transpose: d/dx0 and the scatter onto the field grid
+        for (int c = 0; c < 6; c++) {
+            for (int a = 0; a < 3; a++) lx0[a] += lF[c] * Fp[c].d[a];
+            if (lF[c] == 0.0) continue;
+            for (int k = 0; k <= M; k++)
+                for (int j = 0; j <= M; j++)
+                    for (int i = 0; i <= M; i++)
+                        hbm_add(&A.lF[c][g.fi(K0[c] + k - H, J0[c] + j - H, I0[c] + i - H)],
+                                WX[c][i].v * WY[c][j].v * WZ[c][k].v * lF[c]);
+        }
+        A.lx[p] = lx0[0]; A.ly[p] = lx0[1]; A.lz[p] = lx0[2];
+        A.lux[p] = lu0[0]; A.luy[p] = lu0[1]; A.luz[p] = lu0[2];
+        A.lw[p] += gw;
+    }
+}
+
+// adjoint of the final kinetic energy sum_p m w (gamma - 1) dV
+__global__ __launch_bounds__(256) void k_adj_objective(AdjPush A, double mdv, double gout) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const double w = A.w[p];
+        double lu[3] = {0.0, 0.0, 0.0}, lw = 0.0;
+        if (w == w) {
+            const double ux = A.ux[p], uy = A.uy[p], uz = A.uz[p];
+            const double gam = ::sqrt(1.0 + (ux * ux + uy * uy + uz * uz));
+            lu[0] = gout * mdv * w * ux / gam;
+            lu[1] = gout * mdv * w * uy / gam;
+            lu[2] = gout * mdv * w * uz / gam;
+            lw = gout * mdv * (gam - 1.0);
+        }
+        A.lx[p] = 0.0; A.ly[p] = 0.0; A.lz[p] = 0.0;
+        A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
+        A.lw[p] = lw;
+    }
+}
+
+// ---------------------------------------------------------------------
+// field adjoints (gathers over the adjoint arrays)
+// ---------------------------------------------------------------------
+__device__ __forceinline__ bool inner(const Grid& g, int k, int j, int i) {
+    return i >= g.m && i < g.m + g.n[0] && j >= g.m && j < g.m + g.n[1] && k >= g.m && k < g.m + g.n[2];
+}
+
+// transpose of E += c (curl B - J) (k_e_half): lB += .., lJ -= c lE
+__global__ __launch_bounds__(256) void k_adj_e(const double* __restrict__ lEx, const double* __restrict__ lEy,
+                                               const double* __restrict__ lEz, double* __restrict__ lBx,
+                                               double* __restrict__ lBy, double* __restrict__ lBz,
+                                               double* __restrict__ lJx, double* __restrict__ lJy,
+                                               double* __restrict__ lJz, Grid g, double c, double dx,
+                                               double dy, double dz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.fsize) return;
+    const int i = (int)(t % g.fe[0]), j = (int)((t / g.fe[0]) % g.fe[1]), k = (int)(t / ((int64_t)g.fe[0] * g.fe[1]));
+    auto L = [&](const double* a, int kk, int jj, int ii) -> double {
+        return inner(g, kk, jj, ii) ? a[g.fi(kk, jj, ii)] : 0.0;
+    };
+    lBz[t] += c * (L(lEx, k, j, i) / dy - L(lEx, k, j + 1, i) / dy - L(lEy, k, j, i) / dx + L(lEy, k, j, i + 1) / dx);
+    lBy[t] += c * (-L(lEx, k, j, i) / dz + L(lEx, k + 1, j, i) / dz + L(lEz, k, j, i) / dx - L(lEz, k, j, i + 1) / dx);
+    lBx[t] += c * (L(lEy, k, j, i) / dz - L(lEy, k + 1, j, i) / dz - L(lEz, k, j, i) / dy + L(lEz, k, j + 1, i) / dy);
+    if (inner(g, k, j, i)) {
+        const int64_t q = g.ji(k, j, i);
+        lJx[q] -= c * lEx[t];
+        lJy[q] -= c * lEy[t];
+        lJz[q] -= c * lEz[t];
+    }
+}
+
+// transpose of B -= c curl E (k_b_half): lE += ..
+__global__ __launch_bounds__(256) void k_adj_b(double* __restrict__ lEx, double* __restrict__ lEy,
+                                               double* __restrict__ lEz, const double* __restrict__ lBx,
+                                               const double* __restrict__ lBy, const double* __restrict__ lBz,
+                                               Grid g, double c, double dx, double dy, double dz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= g.fsize) return;
+    const int i = (int)(t % g.fe[0]), j = (int)((t / g.fe[0]) % g.fe[1]), k = (int)(t / ((int64_t)g.fe[0] * g.fe[1]));
+    auto L = [&](const double* a, int kk, int jj, int ii) -> double {
+        return inner(g, kk, jj, ii) ? a[g.fi(kk, jj, ii)] : 0.0;
+    };
+    lEx[t] += c * (-L(lBy, k - 1, j, i) / dz + L(lBy, k, j, i) / dz + L(lBz, k, j - 1, i) / dy - L(lBz, k, j, i) / dy);
+    lEy[t] += c * (L(lBx, k - 1, j, i) / dz - L(lBx, k, j, i) / dz - L(lBz, k, j, i - 1) / dx + L(lBz, k, j, i) / dx);
+    lEz[t] += c * (-L(lBx, k, j - 1, i) / dy + L(lBx, k, j, i) / dy + L(lBy, k, j, i - 1) / dx - L(lBy, k, j, i) / dx);
+}
+
+// transpose of the halo wrap of three components along one axis
+// (k_wrap): the ghost's adjoint moves onto its source
+__global__ __launch_bounds__(256) void k_adj_wrap(double* __restrict__ a0, double* __restrict__ a1,
+                                                  double* __restrict__ a2, Grid g, int axis) {
+    const int m = g.m, N = g.n[axis];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int e0, e1;
+    if (axis == 0) { e0 = g.fe[1]; e1 = g.fe[2]; }
+    else if (axis == 1) { e0 = g.fe[0]; e1 = g.fe[2]; }
+    else { e0 = g.fe[0]; e1 = g.fe[1]; }
+    if (t >= (int64_t)2 * m * e0 * e1) return;
+    const int gi = (int)(t % (2 * m));
+    const int64_t r = t / (2 * m);
+    const int u = (int)(r % e0), v = (int)(r / e0);
+    const int dst = gi < m ? gi : m + N + (gi - m);
+    const int src = gi < m ? N + gi : m + (gi - m);
+    int64_t od, os;
+    if (axis == 0) { od = g.fi(v, u, dst); os = g.fi(v, u, src); }
+    else if (axis == 1) { od = g.fi(v, dst, u); os = g.fi(v, src, u); }
+    else { od = g.fi(dst, v, u); os = g.fi(src, v, u); }
+    double* A[3] = {a0, a1, a2};
+    for (int q = 0; q < 3; q++) {
+        A[q][os] += A[q][od];
+        A[q][od] = 0.0;
+    }
+}
+
+// transpose of the current fold (k_fold): phase 1 = sync, phase 0 = accumulate
+__global__ __launch_bounds__(256) void k_adj_fold(double* __restrict__ a0, double* __restrict__ a1,
+                                                  double* __restrict__ a2, Grid g, int axis, int phase) {
+    const int m = g.m, N = g.n[axis];
+    const int st = m, end = m + N - 1;
+    int e0, e1;
+    if (axis == 0) { e0 = g.je[1]; e1 = g.je[2]; }
+    else if (axis == 1) { e0 = g.je[0]; e1 = g.je[2]; }
+    else { e0 = g.je[0]; e1 = g.je[1]; }
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)(m + 1) * e0 * e1) return;
+    const int l = (int)(t % (m + 1));
+    const int64_t r = t / (m + 1);
+    const int u = (int)(r % e0) - 1, v = (int)(r / e0) - 1;
+    auto idx = [&](int s) -> int64_t {
+        if (axis == 0) return g.ji(v, u, s);
+        if (axis == 1) return g.ji(v, s, u);
+        return g.ji(s, v, u);
+    };
+    const int64_t i_lo_int = idx(st + l), i_hi_gh = idx(end + 1 + l);
+    const int64_t i_hi_int = idx(end - m + l), i_lo_gh = idx(-1 + l);
+    double* A[3] = {a0, a1, a2};
+    for (int q = 0; q < 3; q++) {
+        double* D = A[q];
+        if (phase == 1) {
+            D[i_hi_int] += D[i_lo_gh];
+            D[i_lo_gh] = 0.0;
+            D[i_lo_int] += D[i_hi_gh];
+            D[i_hi_gh] = 0.0;
+        } else {
+            D[i_hi_gh] += D[i_lo_int];
+            D[i_lo_gh] += D[i_hi_int];
+        }
+    }
+}
+
+// transpose of the laser / Silver-Mueller writes (k_laser) and the
+// derivative of the objective with respect to this step's x-min envelope
+// value; g_row(y, z) = Gpulse(t, 0, y, z) / envelope.
+__global__ __launch_bounds__(256) void k_adj_laser(double* __restrict__ lEy, double* __restrict__ lEz,
+                                                   double* __restrict__ lBx, double* __restrict__ lBy,
+                                                   double* __restrict__ lBz, double* __restrict__ lJy,
+                                                   double* __restrict__ lJz, Grid g, LaserArgs L,
+                                                   double* __restrict__ genv) {
+    const int j = g.m + blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = g.m + blockIdx.y;
+    double ge = 0.0;
+    if (j < g.m + g.n[1]) {
+        const int m = g.m, N = g.n[0];
+        const double dt = L.dt;
+        const double alpha = 1 / (1 + dt / L.dx);
+        const double beta = 1 - dt / L.dx;
+        const double dtdz = dt / L.dz, dtdy = dt / L.dy;
+        const double yy = ((double)j + 0.0) * L.dy - L.yfocus;
+        const double zz = ((double)k + 0.0) * L.dz - L.zfocus;
+        // left: column m-1 from column m
+        {
+            const int64_t o = g.fi(k, j, m), gl = g.fi(k, j, m - 1);
+            const double a = lBz[gl], b = lBy[gl];
+            lBz[gl] = 0.0;
+            lBy[gl] = 0.0;
+            hbm_add(&lEy[o], -2.0 * alpha * a);
+            hbm_add(&lBx[o], -alpha * dtdz * a + alpha * dtdy * b);
+            hbm_add(&lBx[g.fi(k - 1, j, m)], alpha * dtdz * a);
+            hbm_add(&lBx[g.fi(k, j - 1, m)], -alpha * dtdy * b);
+            hbm_add(&lBz[o], -alpha * beta * a);
+            hbm_add(&lBy[o], -alpha * beta * b);
+            hbm_add(&lEz[o], 2.0 * alpha * b);
+            hbm_add(&lJy[g.ji(k, j, m)], -alpha * dt * a);
+            hbm_add(&lJz[g.ji(k, j, m)], alpha * dt * b);
+            if (L.left) {
+                // S_l = P_l = envelope * g_row for the enabled polarisations
+                LaserArgs U = L;
+                U.env_on = 1;
+                U.env_val = 1.0;
+                const double gr = d_gpulse(U, L.time, 0.0, yy, zz);
+                if (L.pol == 1 || L.pol == 3) ge += 4.0 * alpha * a * gr;
+                if (L.pol == 2 || L.pol == 3) ge += -4.0 * alpha * b * gr;
+            }
+        }
+        // right: column N+1 from column N (the sources there do not depend
+        // on the x-min envelope)
+        {
+            const int64_t o = g.fi(k, j, N), gr = g.fi(k, j, N + 1);
+            const double a = lBz[gr], b = lBy[gr];
+            lBz[gr] = 0.0;
+            lBy[gr] = 0.0;
+            hbm_add(&lEy[o], 2.0 * alpha * a);
+            hbm_add(&lBx[o], -alpha * dtdz * a + alpha * dtdy * b);
+            hbm_add(&lBx[g.fi(k - 1, j, N)], alpha * dtdz * a);
+            hbm_add(&lBx[g.fi(k, j - 1, N)], -alpha * dtdy * b);
+            hbm_add(&lBz[o], -alpha * beta * a);
+            hbm_add(&lBy[o], -alpha * beta * b);
+            hbm_add(&lEz[o], -2.0 * alpha * b);
+            hbm_add(&lJy[g.ji(k, j, N)], alpha * dt * a);
+            hbm_add(&lJz[g.ji(k, j, N)], -alpha * dt * b);
+        }
+    }
+    // block sum of ge
+    __shared__ double red[256];
+    red[threadIdx.x] = ge;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] != 0.0) hbm_add(genv, red[0]);
+}
+
+}  // namespace pic
+
+// ---------------------------------------------------------------------
+// objective: final kinetic energy of the species (FP64 sum)
+// ---------------------------------------------------------------------
+namespace pic {
+
+__global__ __launch_bounds__(256) void k_adj_kinetic(const double* __restrict__ w, const double* __restrict__ ux,
+                                                     const double* __restrict__ uy, const double* __restrict__ uz,
+                                                     int64_t n, double* __restrict__ out) {
+    double s = 0.0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const double wp = w[p];
+        if (!(wp == wp)) continue;
+        s += wp * (::sqrt(1.0 + (ux[p] * ux[p] + uy[p] * uy[p] + uz[p] * uz[p])) - 1.0);
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) hbm_add(out, red[0]);
+}
+
+// set the weights of live slots (dead slots stay dead)
+__global__ __launch_bounds__(256) void k_adj_set_w(double* __restrict__ w, const double* __restrict__ win, int64_t n) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        if (w[p] == w[p]) w[p] = win[p];
+}
+
+// ---------------------------------------------------------------------
+// checkpoint / recompute driver
+// ---------------------------------------------------------------------
+struct DiffState {
+    int s = -1;
+    int64_t slots = 0;               // particle slots of species s (live and dead)
+    int first = 1, nsteps = 0, K = 1;
+    int forward_done = 0;
+    int saved_push_kernel = 0;
+    std::vector<double> env;         // envelope per step (empty: the analytic one)
+    double objective = 0.0;
+    // snapshots: [0] the run's initial state, [i] the state before step first + i*K
+    std::vector<char*> ck;
+    size_t ck_bytes = 0;
+    // per-step buffers of the segment being reversed
+    char* seg = nullptr;
+    int seg_steps = 0;
+    size_t step_bytes = 0;
+    // adjoint state
+    double* lF[6] = {};
+    double* lJ[3] = {};
+    double* lp[7] = {};              // x y z ux uy uz w
+    double* genv = nullptr;          // per step (device)
+    double* red = nullptr;           // reduction scratch
+};
+
+namespace {
+
+int dalloc(pic_ctx* c, void** p, size_t bytes) {
+    if (hipMalloc(p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+        *p = nullptr;
+        return set_err(c, PIC_ERR_NOMEM, "differentiated run: device allocation of %zu bytes failed", bytes);
+    }
+    return PIC_OK;
+}
+
+int fill0(pic_ctx* c, void* p, size_t bytes) { return launch_fill(c, p, (int64_t)bytes); }
+
+double* pcol(pic_ctx* c, int s, int f) {  // particle array f of species s (current buffer)
+    Species& S = c->sp[s];
+    double* a[7] = {S.x[S.cur], S.y[S.cur], S.z[S.cur], S.ux[S.cur], S.uy[S.cur], S.uz[S.cur], S.w[S.cur]};
+    return a[f];
+}
+
+// state <-> snapshot: 6 fields, 3 currents, 7 particle arrays, counters
+int copy_state(pic_ctx* c, DiffState& D, char* snap, bool save) {
+    const Grid& g = c->g;
+    const size_t fb = g.fsize * 8, jb = g.jsize * 8, pb = D.slots * 8;
+    size_t off = 0;
+    auto mv = [&](void* live, size_t bytes) -> int {
+        void* dst = save ? (void*)(snap + off) : live;
+        const void* src = save ? (const void*)live : (const void*)(snap + off);
+        if (bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+            return set_err(c, PIC_ERR_HIP, "differentiated run: snapshot copy failed");
+        off += (bytes + 255) / 256 * 256;
+        return PIC_OK;
+    };
+    int r = PIC_OK;
+    for (int f = 0; f < 6 && !r; f++) r = mv(c->F[f], fb);
+    for (int f = 0; f < 3 && !r; f++) r = mv(c->J[f], jb);
+    for (int f = 0; f < 7 && !r; f++) r = mv(pcol(c, D.s, f), pb);
+    if (!r) r = mv(c->sp[D.s].d_cnt, CNT_N * 8);
+    return r;
+}
+
+size_t snapshot_bytes(const pic_ctx* c, const DiffState& D) {
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    return 6 * up(c->g.fsize * 8) + 3 * up(c->g.jsize * 8) + 7 * up(D.slots * 8) + up(CNT_N * 8);
+}
+
+// one step of run_main_time_loop (simulation.cpp:159-183) with the
+// envelope value e; 'save' (may be null) receives the particle state and
+// the fields at the push
+int diff_step(pic_ctx* c, DiffState& D, int st, char* save) {
+    int r;
+    c->env_on = D.env.empty() ? 0 : 1;
+    c->env_val = D.env.empty() ? 0.0 : D.env[st - D.first];
+    r = pic_laser_boundary(c, st);
+    c->env_on = 0;
+    if (r) return r;
+    if ((r = pic_fields_first(c))) return r;
+    if ((r = pic_clear_currents(c))) return r;
+    if (save) {
+        const size_t pb = (D.slots * 8 + 255) / 256 * 256, fb = (c->g.fsize * 8 + 255) / 256 * 256;
+        for (int f = 0; f < 7; f++)
+            if (D.slots)
+                PIC_HIP(c, hipMemcpyAsync(save + f * pb, pcol(c, D.s, f), D.slots * 8, hipMemcpyDeviceToDevice, c->stream));
+        for (int f = 0; f < 6; f++)
+            PIC_HIP(c, hipMemcpyAsync(save + 7 * pb + f * fb, c->F[f], c->g.fsize * 8, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if ((r = launch_push(c, D.s))) return r;
+    if ((r = pic_particle_boundary(c, D.s, nullptr))) return r;
+    if ((r = pic_fold_currents(c))) return r;
+    return pic_fields_second(c);
+}
+
+int adj_wrap3(pic_ctx* c, double* const* a) {
+    const Grid& g = c->g;
+    for (int ax = 2; ax >= 0; ax--) {
+        if (!c->p.periodic[ax]) continue;
+        const int e0 = ax == 0 ? g.fe[1] : g.fe[0], e1 = ax == 2 ? g.fe[1] : g.fe[2];
+        const int64_t total = (int64_t)2 * g.m * e0 * e1;
+        k_adj_wrap<<<blocks_for(total, 256), 256, 0, c->stream>>>(a[0], a[1], a[2], g, ax);
+        PIC_LAUNCH_CHECK(c);
+    }
+    return PIC_OK;
+}
+
+int adj_e(pic_ctx* c, DiffState& D, double coef) {
+    const Grid& g = c->g;
+    k_adj_e<<<blocks_for(g.fsize, 256), 256, 0, c->stream>>>(D.lF[0], D.lF[1], D.lF[2], D.lF[3], D.lF[4],
+                                                              D.lF[5], D.lJ[0], D.lJ[1], D.lJ[2], g, coef,
+                                                              c->p.dx, c->p.dy, c->p.dz);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+int adj_b(pic_ctx* c, DiffState& D, double coef) {
+    const Grid& g = c->g;
+    k_adj_b<<<blocks_for(g.fsize, 256), 256, 0, c->stream>>>(D.lF[0], D.lF[1], D.lF[2], D.lF[3], D.lF[4],
+                                                              D.lF[5], g, coef, c->p.dx, c->p.dy, c->p.dz);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+int adj_fold(pic_ctx* c, DiffState& D) {
+    const Grid& g = c->g;
+    for (int phase = 1; phase >= 0; phase--)
+        for (int ax = 2; ax >= 0; ax--) {
+            if (!c->p.periodic[ax]) continue;
+            const int e0 = ax == 0 ? g.je[1] : g.je[0], e1 = ax == 2 ? g.je[1] : g.je[2];
+            const int64_t total = (int64_t)(g.m + 1) * e0 * e1;
+            k_adj_fold<<<blocks_for(total, 256), 256, 0, c->stream>>>(D.lJ[0], D.lJ[1], D.lJ[2], g, ax, phase);
+            PIC_LAUNCH_CHECK(c);
+        }
+    return PIC_OK;
+}
+
+AdjPush adj_view(pic_ctx* c, DiffState& D, const char* save) {
+    AdjPush A;
+    const size_t pb = (D.slots * 8 + 255) / 256 * 256, fb = (c->g.fsize * 8 + 255) / 256 * 256;
+    const double* P[7];
+    for (int f = 0; f < 7; f++) P[f] = save ? (const double*)(save + f * pb) : pcol(c, D.s, f);
+    A.x = P[0]; A.y = P[1]; A.z = P[2]; A.ux = P[3]; A.uy = P[4]; A.uz = P[5]; A.w = P[6];
+    for (int f = 0; f < 6; f++) A.F[f] = save ? (const double*)(save + 7 * pb + f * fb) : (const double*)c->F[f];
+    for (int f = 0; f < 3; f++) A.lJ[f] = D.lJ[f];
+    for (int f = 0; f < 6; f++) A.lF[f] = D.lF[f];
+    A.lx = D.lp[0]; A.ly = D.lp[1]; A.lz = D.lp[2]; A.lux = D.lp[3]; A.luy = D.lp[4]; A.luz = D.lp[5];
+    A.lw = D.lp[6];
+    A.n = D.slots;
+    return A;
+}
+
+// reverse of diff_step for step st, from the saved push-time state
+int adj_step(pic_ctx* c, DiffState& D, int st, const char* save) {
+    const double h = c->p.dt / 2.0;
+    double* lE[3] = {D.lF[0], D.lF[1], D.lF[2]};
+    double* lB[3] = {D.lF[3], D.lF[4], D.lF[5]};
+    int r;
+    // fields_second: B then E
+    if ((r = adj_wrap3(c, lE)) || (r = adj_e(c, D, h))) return r;
+    if ((r = adj_wrap3(c, lB)) || (r = adj_b(c, D, h))) return r;
+    // fold, boundary (dead slots carry no adjoint), push
+    if ((r = adj_fold(c, D))) return r;
+    const PushConsts K = push_consts64(c, c->sp[D.s]);
+    const AdjPush A = adj_view(c, D, save);
+    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(D.slots, 1), 256), 8192);
+    if (c->g.m == 2) k_adj_push64<2><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
+    else k_adj_push64<4><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
+    PIC_LAUNCH_CHECK(c);
+    // clear: the currents before it get their adjoint from this step's
+    // E half step and laser boundary only
+    for (int f = 0; f < 3; f++)
+        if ((r = fill0(c, D.lJ[f], c->g.jsize * 8))) return r;
+    // fields_first: E then B
+    if ((r = adj_wrap3(c, lB)) || (r = adj_b(c, D, h))) return r;
+    if ((r = adj_wrap3(c, lE)) || (r = adj_e(c, D, h))) return r;
+    // laser boundary (x open only)
+    if (!c->p.periodic[0]) {
+        LaserArgs L = laser_args(c, st, 1, c->p.laser_right);
+        L.env_on = D.env.empty() ? 0 : 1;
+        L.env_val = D.env.empty() ? 0.0 : D.env[st - D.first];
+        dim3 grid(blocks_for(c->g.n[1], 256), c->g.n[2]);
+        k_adj_laser<<<grid, 256, 0, c->stream>>>(D.lF[1], D.lF[2], D.lF[3], D.lF[4], D.lF[5], D.lJ[1], D.lJ[2],
+                                                 c->g, L, D.genv + (st - D.first));
+        PIC_LAUNCH_CHECK(c);
+    }
+    return PIC_OK;
+}
+
+void free_adjoint(DiffState& D) {
+    for (auto& p : D.lF) { if (p) hipFree(p); p = nullptr; }
+    for (auto& p : D.lJ) { if (p) hipFree(p); p = nullptr; }
+    for (auto& p : D.lp) { if (p) hipFree(p); p = nullptr; }
+    if (D.genv) hipFree(D.genv);
+    D.genv = nullptr;
+    if (D.seg) hipFree(D.seg);
+    D.seg = nullptr;
+    D.seg_steps = 0;
+}
+
+void free_checkpoints(DiffState& D, bool keep_initial) {
+    for (size_t i = keep_initial ? 1 : 0; i < D.ck.size(); i++)
+        if (D.ck[i]) hipFree(D.ck[i]);
+    D.ck.resize(keep_initial && !D.ck.empty() ? 1 : 0);
+}
+
+}  // namespace
+
+void diff_free(pic_ctx* c) {
+    if (!c || !c->diff) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    DiffState& D = *c->diff;
+    free_adjoint(D);
+    free_checkpoints(D, false);
+    if (D.red) hipFree(D.red);
+    c->push_kernel = D.saved_push_kernel;
+    delete c->diff;
+    c->diff = nullptr;
+}
+
+}  // namespace pic
+
+using namespace pic;
+
+extern "C" {
+
+int pic_diff_begin(pic_ctx* c, int s) {
+    if (!c || s < 0 || s >= PIC_MAX_SPECIES) return PIC_ERR_ARG;
+    if (c->fp32) return set_err(c, PIC_ERR_STATE, "the differentiated run is FP64 only");
+    Species& S = c->sp[s];
+    if (!S.declared || S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is not a mobile species", s);
+    for (int o = 0; o < PIC_MAX_SPECIES; o++)
+        if (o != s && c->sp[o].declared && !c->sp[o].immobile && c->sp[o].cap > 0)
+            return set_err(c, PIC_ERR_STATE, "differentiated run: species %d is mobile too (one mobile species)", o);
+    diff_free(c);
+    PIC_HIP(c, hipSetDevice(c->device));
+    int64_t cnt[CNT_N];
+    PIC_HIP(c, hipMemcpyAsync(cnt, S.d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    c->diff = new DiffState();
+    DiffState& D = *c->diff;
+    D.s = s;
+    D.slots = cnt[CNT_SLOTS];
+    D.saved_push_kernel = c->push_kernel;
+    D.ck_bytes = snapshot_bytes(c, D);
+    void* p = nullptr;
+    int r = dalloc(c, &p, D.ck_bytes);
+    if (!r) {
+        D.ck.push_back((char*)p);
+        r = copy_state(c, D, D.ck[0], true);
+    }
+    if (!r) r = dalloc(c, (void**)&D.red, 256);
+    if (!r && hipStreamSynchronize(c->stream) != hipSuccess) r = set_err(c, PIC_ERR_HIP, "pic_diff_begin failed");
+    if (r) diff_free(c);
+    return r;
+}
+
+int pic_diff_slots(pic_ctx* c, int64_t* slots) {
+    if (!c || !slots) return PIC_ERR_ARG;
+    if (!c->diff) return set_err(c, PIC_ERR_STATE, "no differentiated run (pic_diff_begin)");
+    *slots = c->diff->slots;
+    return PIC_OK;
+}
+
+int pic_diff_initial(pic_ctx* c, double* x, double* y, double* z, double* w) {
+    if (!c) return PIC_ERR_ARG;
+    if (!c->diff) return set_err(c, PIC_ERR_STATE, "no differentiated run (pic_diff_begin)");
+    DiffState& D = *c->diff;
+    PIC_HIP(c, hipSetDevice(c->device));
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t base = 6 * up(c->g.fsize * 8) + 3 * up(c->g.jsize * 8), pb = up(D.slots * 8);
+    double* dst[4] = {x, y, z, w};
+    const int col[4] = {0, 1, 2, 6};
+    for (int i = 0; i < 4; i++)
+        if (dst[i] && D.slots)
+            PIC_HIP(c, hipMemcpyAsync(dst[i], D.ck[0] + base + col[i] * pb, D.slots * 8, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double* envelope,
+                     const double* weights, double* objective) {
+    if (!c || nsteps < 0 || K < 1 || !objective) return PIC_ERR_ARG;
+    if (!c->diff) return set_err(c, PIC_ERR_STATE, "no differentiated run (pic_diff_begin)");
+    DiffState& D = *c->diff;
+    PIC_HIP(c, hipSetDevice(c->device));
+    free_adjoint(D);
+    free_checkpoints(D, true);
+    D.first = first_step;
+    D.nsteps = nsteps;
+    D.K = K;
+    D.forward_done = 0;
+    if (envelope) D.env.assign(envelope, envelope + nsteps);
+    else D.env.clear();
+    int r = copy_state(c, D, D.ck[0], false);
+    if (!r && weights && D.slots) {  // new initial weights: into the state and the snapshot
+        double* tmp = nullptr;
+        r = dalloc(c, (void**)&tmp, D.slots * 8);
+        if (!r && hipMemcpyAsync(tmp, weights, D.slots * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            r = set_err(c, PIC_ERR_HIP, "pic_diff_forward: weight upload failed");
+        if (!r) {
+            k_adj_set_w<<<blocks_for(D.slots, 256), 256, 0, c->stream>>>(pcol(c, D.s, 6), tmp, D.slots);
+            if (hipGetLastError() != hipSuccess) r = set_err(c, PIC_ERR_HIP, "pic_diff_forward: weight kernel failed");
+        }
+        if (!r) r = copy_state(c, D, D.ck[0], true);
+        if (!r && hipStreamSynchronize(c->stream) != hipSuccess) r = set_err(c, PIC_ERR_HIP, "pic_diff_forward failed");
+        if (tmp) hipFree(tmp);
+    }
+    c->push_kernel = 1;  // slot-stable push
+    for (int st = first_step; st < first_step + nsteps && !r; st++) {
+        const int i = st - first_step;
+        if (i > 0 && i % K == 0) {
+            void* p = nullptr;
+            r = dalloc(c, &p, D.ck_bytes);
+            if (!r) {
+                D.ck.push_back((char*)p);
+                r = copy_state(c, D, D.ck.back(), true);
+            }
+        }
+        if (!r) r = diff_step(c, D, st, nullptr);
+    }
+    c->push_kernel = D.saved_push_kernel;
+    if (r) return r;
+    // objective: sum m w (gamma - 1) dV
+    Species& S = c->sp[D.s];
+    if ((r = fill0(c, D.red, 8))) return r;
+    if (D.slots) {
+        k_adj_kinetic<<<(int)std::min<int64_t>(blocks_for(D.slots, 256), 2048), 256, 0, c->stream>>>(
+            pcol(c, D.s, 6), pcol(c, D.s, 3), pcol(c, D.s, 4), pcol(c, D.s, 5), D.slots, D.red);
+        PIC_LAUNCH_CHECK(c);
+    }
+    double sum = 0.0;
+    PIC_HIP(c, hipMemcpyAsync(&sum, D.red, 8, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    D.objective = sum * S.m * c->p.dx * c->p.dy * c->p.dz;
+    *objective = D.objective;
+    D.forward_done = 1;
+    return PIC_OK;
+}
+
+int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, double* grad_weights) {
+    if (!c) return PIC_ERR_ARG;
+    if (!c->diff || !c->diff->forward_done)
+        return set_err(c, PIC_ERR_STATE, "pic_diff_backward needs a pic_diff_forward first");
+    DiffState& D = *c->diff;
+    PIC_HIP(c, hipSetDevice(c->device));
+    const Grid& g = c->g;
+    int r = PIC_OK;
+    free_adjoint(D);
+    for (int f = 0; f < 6 && !r; f++) r = dalloc(c, (void**)&D.lF[f], g.fsize * 8);
+    for (int f = 0; f < 3 && !r; f++) r = dalloc(c, (void**)&D.lJ[f], g.jsize * 8);
+    for (int f = 0; f < 7 && !r; f++) r = dalloc(c, (void**)&D.lp[f], std::max<int64_t>(D.slots, 1) * 8);
+    if (!r) r = dalloc(c, (void**)&D.genv, std::max(D.nsteps, 1) * 8);
+    D.step_bytes = 7 * ((D.slots * 8 + 255) / 256 * 256) + 6 * ((g.fsize * 8 + 255) / 256 * 256);
+    D.seg_steps = std::min(D.K, std::max(D.nsteps, 1));
+    if (!r) r = dalloc(c, (void**)&D.seg, D.step_bytes * D.seg_steps);
+    for (int f = 0; f < 6 && !r; f++) r = fill0(c, D.lF[f], g.fsize * 8);
+    for (int f = 0; f < 3 && !r; f++) r = fill0(c, D.lJ[f], g.jsize * 8);
+    if (!r) r = fill0(c, D.genv, std::max(D.nsteps, 1) * 8);
+    if (r) return r;
+    // adjoint of the objective at the final state (the live state now)
+    Species& S = c->sp[D.s];
+    {
+        const AdjPush A = adj_view(c, D, nullptr);
+        k_adj_objective<<<blocks_for(std::max<int64_t>(D.slots, 1), 256), 256, 0, c->stream>>>(
+            A, S.m * c->p.dx * c->p.dy * c->p.dz, grad_objective);
+        PIC_LAUNCH_CHECK(c);
+    }
+    c->push_kernel = 1;
+    const int nseg = (D.nsteps + D.K - 1) / D.K;
+    for (int sg = nseg - 1; sg >= 0 && !r; sg--) {
+        const int a = D.first + sg * D.K, b = std::min(a + D.K, D.first + D.nsteps);
+        r = copy_state(c, D, D.ck[sg], false);
+        for (int st = a; st < b && !r; st++) r = diff_step(c, D, st, D.seg + (size_t)(st - a) * D.step_bytes);
+        for (int st = b - 1; st >= a && !r; st--) r = adj_step(c, D, st, D.seg + (size_t)(st - a) * D.step_bytes);
+    }
+    c->push_kernel = D.saved_push_kernel;
+    // leave the context at the run's initial state
+    if (!r) r = copy_state(c, D, D.ck[0], false);
+    if (r) return r;
+    if (grad_envelope && D.nsteps)
+        PIC_HIP(c, hipMemcpyAsync(grad_envelope, D.genv, D.nsteps * 8, hipMemcpyDeviceToHost, c->stream));
+    if (grad_weights && D.slots)
+        PIC_HIP(c, hipMemcpyAsync(grad_weights, D.lp[6], D.slots * 8, hipMemcpyDeviceToHost, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_diff_end(pic_ctx* c) {
+    if (!c) return PIC_ERR_ARG;
+    diff_free(c);
+    return PIC_OK;
+}
+
+}  // extern "C"

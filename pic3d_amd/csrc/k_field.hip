@@ -5,6 +5,7 @@
 // ghost element), x fastest so a wavefront reads 64 consecutive doubles.
 // The FP64 instantiations keep the reference's operation order (this file
 // is compiled with -ffp-contract=off) so single-step fields are bit-exact.
+#include "laser.h"
 #include "pic_internal.h"
 
 namespace pic {
@@ -114,30 +115,6 @@ __global__ __launch_bounds__(256) void k_filter(const T* __restrict__ src, T* __
 // is_left / is_right switch the x-min / x-max sources; both Silver-Mueller
 // writes happen whenever x is open (the reference's rank_x* == -2).
 // ---------------------------------------------------------------------
-struct LaserArgs {
-    double pi, l0, waist, xfocus, yfocus, zfocus, omega, phase, amp, dur, t_start, lx;
-    double dt, dx, dy, dz, time;
-    int pol, left, right;
-};
-
-__device__ double d_envelope(const LaserArgs& L, double time, double position) {
-    double t = time - position - L.t_start;
-    double env = 0.0;
-    if (t >= 0.0 && t <= 1.0 * L.dur) env = sin(L.pi * t / L.dur);
-    return L.amp * env;
-}
-
-__device__ double d_gpulse(const LaserArgs& L, double time, double x, double y, double z) {
-    double k = 2.0 * L.pi / L.l0;
-    double zr = 0.5 * k * L.waist * L.waist;
-    double xe = x - L.xfocus;
-    double gouy = atan(xe / zr);
-    double w = L.waist * sqrt(1 + pow(xe / zr, 2.0));
-    double ph = (xe / zr) * (y * y + z * z) / w / w - gouy + k * xe - L.omega * time;
-    return -1.0 * d_envelope(L, time, x) * L.waist / w * exp(-(y * y + z * z) / (w * w)) *
-           cos(ph + L.phase);
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_laser(T* __restrict__ ey, T* __restrict__ ez,
                                                const T* __restrict__ bx, T* __restrict__ by,
@@ -313,7 +290,7 @@ template <typename T> static int wrap_t(pic_ctx* c, int first, int count) {
 int launch_wrap(pic_ctx* c, int first) { return c->fp32 ? wrap_t<float>(c, first, 3) : wrap_t<double>(c, first, 3); }
 int launch_wrap_one(pic_ctx* c, int comp) { return c->fp32 ? wrap_t<float>(c, comp, 1) : wrap_t<double>(c, comp, 1); }
 
-template <typename T> static int laser_t(pic_ctx* c, int step, int left, int right) {
+LaserArgs laser_args(const pic_ctx* c, int step, int left, int right) {
     const pic_params& P = c->p;
     LaserArgs L;
     L.pi = P.pi; L.l0 = P.l0; L.waist = P.waist; L.xfocus = P.xfocus; L.yfocus = P.yfocus;
@@ -321,6 +298,12 @@ template <typename T> static int laser_t(pic_ctx* c, int step, int left, int rig
     L.dur = P.laser_pulse_duration; L.t_start = P.laser_start_time; L.lx = P.lx;
     L.dt = P.dt; L.dx = P.dx; L.dy = P.dy; L.dz = P.dz; L.time = step * P.dt;
     L.pol = P.laser_polarization; L.left = left; L.right = right;
+    L.env_on = c->env_on; L.env_val = c->env_val;
+    return L;
+}
+
+template <typename T> static int laser_t(pic_ctx* c, int step, int left, int right) {
+    const LaserArgs L = laser_args(c, step, left, right);
     dim3 grid(blocks_for(c->g.n[1], 256), c->g.n[2]);
     k_laser<T><<<grid, 256, 0, c->stream>>>((T*)c->F[1], (T*)c->F[2], (const T*)c->F[3],
                                             (T*)c->F[4], (T*)c->F[5], (const T*)c->J[1],

@@ -319,6 +319,7 @@ int pic_ctx_destroy(pic_ctx* c) {
     for (int f = 0; f < 3; f++) if (c->J[f]) hipFree(c->J[f]);
     dfree(c->tile_cnt); dfree(c->tile_cur); dfree(c->work); dfree(c->nwork); dfree(c->diag_buf);
     dfree(c->jtmp);
+    diff_free(c);
     for (auto& e : c->tev) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);

@@ -71,6 +71,8 @@ struct Species {
 enum { CNT_SLOTS = 0, CNT_LIVE = 1, CNT_LOST = 2, CNT_WRAP = 8, CNT_LOST_STEP = 14,
        CNT_WRAP_STEP = 20, CNT_OOB = 26, CNT_PEND = 27, CNT_N = 40 };
 
+struct DiffState;  // differentiated run (k_adj.hip)
+
 }  // namespace pic
 
 struct pic_ctx {
@@ -99,6 +101,10 @@ struct pic_ctx {
     int keep_old = 0;
     int fix32 = 0;                // FP32: int32 fixed-point LDS accumulators
     void* jtmp = nullptr;         // J-sized scratch of the current filter (lazy)
+    // x-min envelope override of the laser source (differentiated runs)
+    int env_on = 0;
+    double env_val = 0.0;
+    pic::DiffState* diff = nullptr;
     // per-phase timing (PIC_OPT_TIMING): event pairs around each phase
     int timing = 0;
     std::vector<hipEvent_t> tev;      // event pool
@@ -136,6 +142,7 @@ int launch_fold_one(pic_ctx* c, void* arr);
 int launch_filter(pic_ctx* c, void* tmp);  // tmp: one J-sized scratch array
 int launch_deposit(pic_ctx* c, int s, int64_t n, const double* const* d6, const double* w,
                    int64_t* d_skipped);
+void diff_free(pic_ctx* c);
 int launch_push(pic_ctx* c, int s);
 int launch_boundary(pic_ctx* c, int s);
 int launch_sort(pic_ctx* c, int s);
