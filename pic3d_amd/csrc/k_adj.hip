@@ -559,6 +559,7 @@ struct DiffState {
     // snapshots: [0] the run's initial state, [i] the state before step first + i*K
     std::vector<char*> ck;
     size_t ck_bytes = 0;
+    char* fin = nullptr;             // the state at the end of the forward
     // per-step buffers of the segment being reversed
     char* seg = nullptr;
     int seg_steps = 0;
@@ -750,6 +751,8 @@ void free_checkpoints(DiffState& D, bool keep_initial) {
     for (size_t i = keep_initial ? 1 : 0; i < D.ck.size(); i++)
         if (D.ck[i]) hipFree(D.ck[i]);
     D.ck.resize(keep_initial && !D.ck.empty() ? 1 : 0);
+    if (D.fin) hipFree(D.fin);
+    D.fin = nullptr;
 }
 
 }  // namespace
@@ -833,6 +836,7 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
     if (!c->diff) return set_err(c, PIC_ERR_STATE, "no differentiated run (pic_diff_begin)");
     DiffState& D = *c->diff;
     PIC_HIP(c, hipSetDevice(c->device));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
     free_adjoint(D);
     free_checkpoints(D, true);
     D.first = first_step;
@@ -869,6 +873,8 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
         if (!r) r = diff_step(c, D, st, nullptr);
     }
     c->push_kernel = D.saved_push_kernel;
+    if (!r) r = dalloc(c, (void**)&D.fin, D.ck_bytes);
+    if (!r) r = copy_state(c, D, D.fin, true);
     if (r) return r;
     // objective: sum m w (gamma - 1) dV
     Species& S = c->sp[D.s];
@@ -893,8 +899,9 @@ int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, 
         return set_err(c, PIC_ERR_STATE, "pic_diff_backward needs a pic_diff_forward first");
     DiffState& D = *c->diff;
     PIC_HIP(c, hipSetDevice(c->device));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
     const Grid& g = c->g;
-    int r = PIC_OK;
+    int r = copy_state(c, D, D.fin, false);  // the final state of the forward
     free_adjoint(D);
     for (int f = 0; f < 6 && !r; f++) r = dalloc(c, (void**)&D.lF[f], g.fsize * 8);
     for (int f = 0; f < 3 && !r; f++) r = dalloc(c, (void**)&D.lJ[f], g.jsize * 8);

@@ -26,7 +26,8 @@ TOL = 1e-4
 
 
 def laser_run(mem, nsteps=40, K=7):
-    wl = W.laser_slab(nx=24, ny=8, nz=8, ppc=2, slab=(3.0, 14.0), seed=31, mem=mem)
+    n_t = 8 if mem == 2 else 10  # at least 2 mem + 2 cells per axis
+    wl = W.laser_slab(nx=24, ny=n_t, nz=n_t, ppc=2, slab=(3.0, 14.0), seed=31, mem=mem)
     p = wl.params
     p.laser_amplitude = 2.0
     p.laser_pulse_duration = 6.0  # a short pulse that is on during the run
@@ -38,7 +39,7 @@ def laser_run(mem, nsteps=40, K=7):
 
 
 def langmuir_run(mem, nsteps=30, K=8):
-    wl = W.thermal(n=8, ppc=2, seed=32, mem=mem)
+    wl = W.thermal(n=8 if mem == 2 else 10, ppc=2, seed=32, mem=mem)
     p = wl.params
     parts = load_particles(p, wl.species[0].loader)
     parts["ux"] = 0.02 * np.sin(2 * np.pi * parts["x"] / p.nx) + parts["ux"] * 0.1
@@ -131,7 +132,8 @@ def test_torch_autograd_function():
     k = KineticEnergy.apply(env, w, run)
     k.backward()
     ge, gw = run.backward()
-    assert np.allclose(env.grad.numpy(), ge, rtol=0, atol=0)
-    assert np.allclose(w.grad.numpy(), gw, rtol=0, atol=0)
+    # a second backward differs only by the order of atomic additions
+    assert np.allclose(env.grad.numpy(), ge, rtol=1e-10, atol=1e-14 * np.max(np.abs(ge)))
+    assert np.allclose(w.grad.numpy(), gw, rtol=1e-10, atol=1e-14 * np.max(np.abs(gw)))
     run.close()
     sim.close()
