@@ -255,9 +255,10 @@ int pic_deposit(pic_ctx* ctx, int s, int64_t n, const double* x_old, const doubl
  *     generate_laser_pulse(t_n, 0) returns, boundary.cpp:8-27; the source
  *     is Gpulse with e_n in its place), and
  *   - the initial weight of every particle slot (the density profile).
- * FP64 only, one mobile species (others immobile).  The run uses the
- * unsorted push so particle slots stay fixed; particles leaving through
- * open faces are marked dead in place.  The forward keeps a full-state
+ * FP64 only, one mobile species (others immobile).  pic_diff_begin sorts
+ * the species by tile once; the run never re-sorts, so particle slots stay
+ * fixed (particles that leave their tile take the tiled push's HBM path;
+ * particles leaving through open faces are marked dead in place).  The forward keeps a full-state
  * checkpoint every K steps; the backward recomputes each K-step segment
  * from its checkpoint, saving the per-step push inputs, and runs the
  * adjoint kernels of that segment in reverse.  Memory: one checkpoint per

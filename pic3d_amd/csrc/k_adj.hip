@@ -17,9 +17,12 @@
 // gather, d/d(u0,E,B) through Boris and d/d(x0..z1) through the deposit;
 // the field adjoint of the gather is scattered with HBM atomics.
 //
-// Particles keep their slots during a differentiated run (the unsorted
-// push; the boundary marks leavers dead without compacting), so slot p at
-// step n and slot p at step n+1 are the same particle.
+// Particles keep their slots during a differentiated run: the species is
+// sorted by tile once when the run begins and never again (the tiled push
+// sends particles that left their tile through its HBM path; the direct
+// push ignores order), and the boundary marks leavers dead without
+// compacting.  So slot p at step n and slot p at step n+1 are the same
+// particle.
 #include <algorithm>
 #include <cmath>
 
@@ -554,6 +557,8 @@ struct DiffState {
     int first = 1, nsteps = 0, K = 1;
     int forward_done = 0;
     int saved_push_kernel = 0;
+    int saved_sort_interval = 1;
+    int kernel = 2;                  // push kernel of the run (1 direct, 2 tiled, never re-sorted)
     std::vector<double> env;         // envelope per step (empty: the analytic one)
     double objective = 0.0;
     // snapshots: [0] the run's initial state, [i] the state before step first + i*K
@@ -766,6 +771,7 @@ void diff_free(pic_ctx* c) {
     free_checkpoints(D, false);
     if (D.red) hipFree(D.red);
     c->push_kernel = D.saved_push_kernel;
+    c->sort_interval = D.saved_sort_interval;
     delete c->diff;
     c->diff = nullptr;
 }
@@ -786,6 +792,14 @@ int pic_diff_begin(pic_ctx* c, int s) {
             return set_err(c, PIC_ERR_STATE, "differentiated run: species %d is mobile too (one mobile species)", o);
     diff_free(c);
     PIC_HIP(c, hipSetDevice(c->device));
+    // The tiled push keeps slots fixed as long as the species is not
+    // re-sorted: sort once here (grouping by tile, dropping dead slots);
+    // particles that later leave their tile take the push's HBM path.
+    const int kernel = c->push_kernel == 1 ? 1 : 2;
+    if (kernel == 2 && S.cap > 0) {
+        int r0 = launch_sort(c, s);
+        if (r0) return r0;
+    }
     int64_t cnt[CNT_N];
     PIC_HIP(c, hipMemcpyAsync(cnt, S.d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(c, hipStreamSynchronize(c->stream));
@@ -794,6 +808,8 @@ int pic_diff_begin(pic_ctx* c, int s) {
     D.s = s;
     D.slots = cnt[CNT_SLOTS];
     D.saved_push_kernel = c->push_kernel;
+    D.saved_sort_interval = c->sort_interval;
+    D.kernel = kernel;
     D.ck_bytes = snapshot_bytes(c, D);
     void* p = nullptr;
     int r = dalloc(c, &p, D.ck_bytes);
@@ -859,7 +875,8 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
         if (!r && hipStreamSynchronize(c->stream) != hipSuccess) r = set_err(c, PIC_ERR_HIP, "pic_diff_forward failed");
         if (tmp) hipFree(tmp);
     }
-    c->push_kernel = 1;  // slot-stable push
+    c->push_kernel = D.kernel;  // slot-stable push: no re-sort during the run
+    c->sort_interval = 1 << 30;
     for (int st = first_step; st < first_step + nsteps && !r; st++) {
         const int i = st - first_step;
         if (i > 0 && i % K == 0) {
@@ -873,6 +890,7 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
         if (!r) r = diff_step(c, D, st, nullptr);
     }
     c->push_kernel = D.saved_push_kernel;
+    c->sort_interval = D.saved_sort_interval;
     if (!r) r = dalloc(c, (void**)&D.fin, D.ck_bytes);
     if (!r) r = copy_state(c, D, D.fin, true);
     if (r) return r;
@@ -922,7 +940,8 @@ int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, 
             A, S.m * c->p.dx * c->p.dy * c->p.dz, grad_objective);
         PIC_LAUNCH_CHECK(c);
     }
-    c->push_kernel = 1;
+    c->push_kernel = D.kernel;
+    c->sort_interval = 1 << 30;
     const int nseg = (D.nsteps + D.K - 1) / D.K;
     for (int sg = nseg - 1; sg >= 0 && !r; sg--) {
         const int a = D.first + sg * D.K, b = std::min(a + D.K, D.first + D.nsteps);
@@ -931,6 +950,7 @@ int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, 
         for (int st = b - 1; st >= a && !r; st--) r = adj_step(c, D, st, D.seg + (size_t)(st - a) * D.step_bytes);
     }
     c->push_kernel = D.saved_push_kernel;
+    c->sort_interval = D.saved_sort_interval;
     // leave the context at the run's initial state
     if (!r) r = copy_state(c, D, D.ck[0], false);
     if (r) return r;

@@ -1,6 +1,9 @@
 """Differentiation form on the GPU: adjoint gradients of the final electron
 kinetic energy against central finite differences of the FP64 forward.
 
+Both push kernels run the forward (the direct one, and the tiled one with
+the grouping frozen at the start of the run).
+
 Cases:
   * a laser pulse (x-min source, open x, periodic y/z) driving a plasma
     slab, quadratic and quartic shapes: d K / d envelope (every step) and
@@ -25,14 +28,14 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-4
 
 
-def laser_run(mem, nsteps=40, K=7):
+def laser_run(mem, nsteps=40, K=7, kernel=1):
     n_t = 8 if mem == 2 else 10  # at least 2 mem + 2 cells per axis
     wl = W.laser_slab(nx=24, ny=n_t, nz=n_t, ppc=2, slab=(3.0, 14.0), seed=31, mem=mem)
     p = wl.params
     p.laser_amplitude = 2.0
     p.laser_pulse_duration = 6.0  # a short pulse that is on during the run
     parts = load_particles(p, wl.species[0].loader)
-    sim = Simulation(p, push_kernel=1)
+    sim = Simulation(p, push_kernel=kernel)
     sim.species_set(0, -1.0, 1.0, False, len(parts["x"]))
     sim.particles_set(0, parts)
     return sim, DiffRun(sim, 0, first_step=1, nsteps=nsteps, checkpoint_every=K)
@@ -64,10 +67,11 @@ def close(fd, ad, scale):
     return abs(fd - ad) <= TOL * abs(ad) + 1e-12 * scale
 
 
+@pytest.mark.parametrize("kernel", [1, 2])
 @pytest.mark.parametrize("mem", [2, 4])
-def test_laser_envelope_and_weights(mem):
-    print(f"diff laser mem {mem}", flush=True)
-    sim, run = laser_run(mem)
+def test_laser_envelope_and_weights(mem, kernel):
+    print(f"diff laser mem {mem} kernel {kernel}", flush=True)
+    sim, run = laser_run(mem, kernel=kernel)
     env0, w0 = run.reference_envelope(), run.initial_weights()
     k0 = run.forward(env0, w0)
     ge, gw = run.backward()
