@@ -151,10 +151,13 @@ enum pic_option {
     PIC_OPT_KEEP_OLD = 2,      /* 1: keep x_old/y_old/z_old per particle (FP64)    */
     PIC_OPT_SORT_INTERVAL = 3, /* re-sort particles by tile every N pushes (>= 1)  */
     PIC_OPT_TIMING = 4,        /* 1: time every phase with HIP events              */
-    PIC_OPT_FP32_ACCUM = 5     /* FP32 only: 1 (default) int32 fixed-point deposit
+    PIC_OPT_FP32_ACCUM = 5,    /* FP32 only: 1 (default) int32 fixed-point deposit
                                   accumulators in LDS (each add rounded to 2^-19 of
                                   one particle's largest contribution); 0 FP64
                                   accumulators in LDS (smaller tiles, slower) */
+    PIC_OPT_ADJ_PUSH = 6       /* differentiated runs: 0 (default) the LDS-tiled
+                                  adjoint push (HBM kernel after a direct forward);
+                                  1 dual-number kernel; 2 separable HBM kernel */
 };
 int pic_set_option(pic_ctx* ctx, int option, int64_t value);
 

@@ -20,6 +20,7 @@ FIELD_NAMES = ["ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz",
 PIC_PROFILE_UNIFORM, PIC_PROFILE_SLAB, PIC_PROFILE_RAMP_TARGET = 0, 1, 2
 PIC_OPT_PUSH_KERNEL, PIC_OPT_KEEP_OLD, PIC_OPT_SORT_INTERVAL, PIC_OPT_TIMING = 1, 2, 3, 4
 PIC_OPT_FP32_ACCUM = 5
+PIC_OPT_ADJ_PUSH = 6
 TIMING_CATS = ["fields", "clear", "push", "sort", "boundary", "fold"]
 
 

@@ -313,6 +313,438 @@ __global__ __launch_bounds__(256) void k_adj_push64(AdjPush A, Grid g, PushConst
     }
 }
 
+// ---------------------------------------------------------------------
+// particle adjoint, separable form
+//
+// The same derivatives as k_adj_push64, evaluated by separable
+// contractions instead of dual numbers: B-spline weights come with their
+// derivatives (one direction per axis), the gather and its position
+// gradient are contracted axis by axis, and the deposit's adjoint is
+//   G = sum lambda_J * J(x0, x1) over the Esirkepov box,
+// with, for Jx = -cx A_i W_jk (A the cumulative x sum, W the y/z weight):
+//   dG/dx0,1 = -cx sum_i dA_i/dx0,1 P_i,        P_i = sum_jk lJ_ijk W_jk
+//   dG/dy0,1 = -cx sum_j dY0,1_j sum_k q_jk dW/dY0,1,   q_jk = sum_i lJ_ijk A_i
+//   dG/dz0,1 likewise; Jy and Jz by permuting the axes.
+// ---------------------------------------------------------------------
+template <int M> __device__ __forceinline__ int shape_dv(double x, double* W, double* dW) {
+    Dual<1> Wd[M + 1];
+    const int ic = shape_at_t<M>(dvar<1>(x, 0), Wd);
+#pragma unroll
+    for (int t = 0; t <= M; t++) {
+        W[t] = Wd[t].v;
+        dW[t] = Wd[t].d[0];
+    }
+    return ic;
+}
+
+// register rotations for rolled loops (compile-time indices only):
+// rot_down reads element 0 per iteration; rot_push appends
+template <int L> __device__ __forceinline__ void rot_down(double* A) {
+#pragma unroll
+    for (int t = 0; t < L - 1; t++) A[t] = A[t + 1];
+}
+template <int L> __device__ __forceinline__ void rot_push(double* A, double v) {
+#pragma unroll
+    for (int t = 0; t < L - 1; t++) A[t] = A[t + 1];
+    A[L - 1] = v;
+}
+
+// weights S placed at offset d in a box of M+2
+template <int M> __device__ __forceinline__ void place_t(const double* S, int d, double* A) {
+#pragma unroll
+    for (int t = 0; t < M + 2; t++) {
+        const double a0 = (t <= M) ? S[t] : 0.0;
+        const double a1 = (t >= 1) ? S[t - 1] : 0.0;
+        A[t] = d ? a1 : a0;
+    }
+}
+
+// Adjoint of one Esirkepov current component along axis 'a' (cumulative
+// sum along a; weights over the other two axes b (middle loop) and c
+// (outer loop)).  lj(k, j, i) reads lambda of this component at box node
+// (k, j, i).  Adds to g[0..5] (d/dx0,y0,z0,x1,y1,z1) and returns G.
+template <int L, typename LJ>
+__device__ __forceinline__ double esirkepov_adj_axis(int a, const double* A0, const double* A1,
+                                                     const double* dA0, const double* dA1,
+                                                     const double* B0, const double* B1,
+                                                     const double* dB0, const double* dB1,
+                                                     const double* C0, const double* C1,
+                                                     const double* dC0, const double* dC1,
+                                                     double coef, LJ lj, double* g, int ab, int ac) {
+    // cumulative sums along a and their derivatives
+    double S[L], dS0[L], dS1[L];
+    double s = 0.0, s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        s += A1[t] - A0[t];
+        s0 -= dA0[t];
+        s1 += dA1[t];
+        S[t] = s; dS0[t] = s0; dS1[t] = s1;
+    }
+    double P[L], R0[L], R1[L], T0[L], T1[L], Cr0[L], Cr1[L];
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        P[t] = 0.0; R0[t] = 0.0; R1[t] = 0.0; T0[t] = 0.0; T1[t] = 0.0;
+        Cr0[t] = C0[t]; Cr1[t] = C1[t];
+    }
+    const double sixth = 1.0 / 6.0, third = 1.0 / 3.0;
+#pragma unroll 1
+    for (int oc = 0; oc < L; oc++) {
+        const double c0 = Cr0[0], dc = Cr1[0] - c0;
+        rot_down<L>(Cr0);
+        rot_down<L>(Cr1);
+        double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+        for (int ob = 0; ob < L; ob++) {
+            const double b0 = B0[ob], db = B1[ob] - b0;
+            const double W = b0 * c0 + 0.5 * db * c0 + 0.5 * b0 * dc + third * db * dc;
+            double q = 0.0;
+#pragma unroll
+            for (int oa = 0; oa < L; oa++) {
+                const double lam = lj(oc, ob, oa);
+                P[oa] += lam * W;
+                q += lam * S[oa];
+            }
+            R0[ob] += q * (0.5 * c0 + sixth * dc);
+            R1[ob] += q * (0.5 * c0 + third * dc);
+            t0 += q * (0.5 * b0 + sixth * db);
+            t1 += q * (0.5 * b0 + third * db);
+        }
+        rot_push<L>(T0, t0);
+        rot_push<L>(T1, t1);
+    }
+    double G = 0.0, ga0 = 0.0, ga1 = 0.0, gb0 = 0.0, gb1 = 0.0, gc0 = 0.0, gc1 = 0.0;
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        G += S[t] * P[t];
+        ga0 += dS0[t] * P[t];
+        ga1 += dS1[t] * P[t];
+        gb0 += R0[t] * dB0[t];
+        gb1 += R1[t] * dB1[t];
+        gc0 += T0[t] * dC0[t];
+        gc1 += T1[t] * dC1[t];
+    }
+    g[a] += -coef * ga0;
+    g[3 + a] += -coef * ga1;
+    g[ab] += -coef * gb0;
+    g[3 + ab] += -coef * gb1;
+    g[ac] += -coef * gc0;
+    g[3 + ac] += -coef * gc1;
+    return -coef * G;
+}
+
+// g[0..5] += d/d(x0..z1) of sum lambda_J * J(q = 1 per unit weight);
+// returns the sum itself (d/dw).  lj(c, k, j, i) reads lambda_J of
+// component c at buffer node (k, j, i).
+template <int M, typename LJ>
+__device__ __forceinline__ double deposit_adj(double q, double xo, double yo, double zo, double xn,
+                                              double yn, double zn, const PushConsts& K, int m,
+                                              LJ lj, double* g) {
+    constexpr int H = M / 2, L = M + 2;
+    const double org = (double)(1 - m);
+    double S[6][M + 1], dS[6][M + 1];
+    const int i1 = shape_dv<M>(xo - org, S[0], dS[0]), i2 = shape_dv<M>(xn - org, S[3], dS[3]);
+    const int j1 = shape_dv<M>(yo - org, S[1], dS[1]), j2 = shape_dv<M>(yn - org, S[4], dS[4]);
+    const int k1 = shape_dv<M>(zo - org, S[2], dS[2]), k2 = shape_dv<M>(zn - org, S[5], dS[5]);
+    const int bx = min(i1, i2) - H, by = min(j1, j2) - H, bz = min(k1, k2) - H;
+    double X0[L], X1[L], Y0[L], Y1[L], Z0[L], Z1[L];
+    double dX0[L], dX1[L], dY0[L], dY1[L], dZ0[L], dZ1[L];
+    place_t<M>(S[0], i1 - H - bx, X0); place_t<M>(dS[0], i1 - H - bx, dX0);
+    place_t<M>(S[3], i2 - H - bx, X1); place_t<M>(dS[3], i2 - H - bx, dX1);
+    place_t<M>(S[1], j1 - H - by, Y0); place_t<M>(dS[1], j1 - H - by, dY0);
+    place_t<M>(S[4], j2 - H - by, Y1); place_t<M>(dS[4], j2 - H - by, dY1);
+    place_t<M>(S[2], k1 - H - bz, Z0); place_t<M>(dS[2], k1 - H - bz, dZ0);
+    place_t<M>(S[5], k2 - H - bz, Z1); place_t<M>(dS[5], k2 - H - bz, dZ1);
+    double G = 0.0;
+    // Jx: cumulative in x, weights over y (middle) and z (outer)
+    G += esirkepov_adj_axis<L>(0, X0, X1, dX0, dX1, Y0, Y1, dY0, dY1, Z0, Z1, dZ0, dZ1,
+                               q * K.dx / K.dt,
+                               [&](int oz, int oy, int ox) { return lj(0, bz + oz, by + oy, bx + ox); },
+                               g, 1, 2);
+    // Jy: cumulative in y, weights over x (middle) and z (outer)
+    G += esirkepov_adj_axis<L>(1, Y0, Y1, dY0, dY1, X0, X1, dX0, dX1, Z0, Z1, dZ0, dZ1,
+                               q * K.dy / K.dt,
+                               [&](int oz, int ox, int oy) { return lj(1, bz + oz, by + oy, bx + ox); },
+                               g, 0, 2);
+    // Jz: cumulative in z, weights over x (middle) and y (outer)
+    G += esirkepov_adj_axis<L>(2, Z0, Z1, dZ0, dZ1, X0, X1, dX0, dX1, Y0, Y1, dY0, dY1,
+                               q * K.dz / K.dt,
+                               [&](int oy, int ox, int oz) { return lj(2, bz + oz, by + oy, bx + ox); },
+                               g, 0, 1);
+    return G;
+}
+
+// Adjoint of one particle's push.  fr(c, k, j, i): field c at a buffer
+// node; lj(c, k, j, i): lambda_J; lf(c, k, j, i, v): adds v to lambda_F.
+// lx, lu: in = adjoint after the push, out = before it.  Returns dK/dw
+// of this step's deposit.
+template <int M, typename FR, typename LJ, typename LF>
+__device__ __forceinline__ double adj_particle(double x, double y, double z, const double* u0v,
+                                               double w, const PushConsts& K, int m, FR fr, LJ lj,
+                                               LF lf, double* lx, double* lu) {
+    constexpr int H = M / 2;
+    const double org = (double)(1 - m);
+    const double xc = x - org, yc = y - org, zc = z - org;
+    // gather with position gradient, separably
+    double Wp[3][M + 1], dWp[3][M + 1], Wd[3][M + 1], dWd[3][M + 1];
+    const int ip = shape_dv<M>(xc, Wp[0], dWp[0]), jp = shape_dv<M>(yc, Wp[1], dWp[1]),
+              kp = shape_dv<M>(zc, Wp[2], dWp[2]);
+    shape_dv<M>(xc - 0.5, Wd[0], dWd[0]);
+    shape_dv<M>(yc - 0.5, Wd[1], dWd[1]);
+    shape_dv<M>(zc - 0.5, Wd[2], dWd[2]);
+    const int id = (int)xc, jd = (int)yc, kd = (int)zc;
+    // staggering per component: 1 = dual along that axis
+    constexpr int SX[6] = {1, 0, 0, 0, 1, 1}, SY[6] = {0, 1, 0, 1, 0, 1}, SZ[6] = {0, 0, 1, 1, 1, 0};
+    double Fv[6], Fg[6][3];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const double* WX = SX[c] ? Wd[0] : Wp[0];
+        const double* dWX = SX[c] ? dWd[0] : dWp[0];
+        const double* WY = SY[c] ? Wd[1] : Wp[1];
+        const double* dWY = SY[c] ? dWd[1] : dWp[1];
+        const double* WZ = SZ[c] ? Wd[2] : Wp[2];
+        const double* dWZ = SZ[c] ? dWd[2] : dWp[2];
+        const int i0 = (SX[c] ? id : ip) - H, j0 = (SY[c] ? jd : jp) - H, k0 = (SZ[c] ? kd : kp) - H;
+        double v = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+        double wzr[M + 1], dwzr[M + 1];
+#pragma unroll
+        for (int t = 0; t <= M; t++) { wzr[t] = WZ[t]; dwzr[t] = dWZ[t]; }
+#pragma unroll 1
+        for (int k = 0; k <= M; k++) {
+            const double wz = wzr[0], dwz = dwzr[0];
+            rot_down<M + 1>(wzr);
+            rot_down<M + 1>(dwzr);
+#pragma unroll
+            for (int j = 0; j <= M; j++) {
+                double s = 0.0, sd = 0.0;
+#pragma unroll
+                for (int i = 0; i <= M; i++) {
+                    const double f = fr(c, k0 + k, j0 + j, i0 + i);
+                    s += WX[i] * f;
+                    sd += dWX[i] * f;
+                }
+                const double yz = WY[j] * wz;
+                v += yz * s;
+                gx += yz * sd;
+                gy += dWY[j] * wz * s;
+                gz += WY[j] * dwz * s;
+            }
+        }
+        Fv[c] = v;
+        Fg[c][0] = gx; Fg[c][1] = gy; Fg[c][2] = gz;
+    }
+    // Boris with d/d(u0, E, B)
+    using D9 = Dual<9>;
+    D9 u9[3] = {dvar<9>(u0v[0], 0), dvar<9>(u0v[1], 1), dvar<9>(u0v[2], 2)};
+    D9 E9[3] = {dvar<9>(Fv[0], 3), dvar<9>(Fv[1], 4), dvar<9>(Fv[2], 5)};
+    D9 B9[3] = {dvar<9>(Fv[3], 6), dvar<9>(Fv[4], 7), dvar<9>(Fv[5], 8)};
+    D9 u1[3];
+    boris_t<D9>(u9, E9, B9, K.qm, u1);
+    const double v1[3] = {u1[0].v, u1[1].v, u1[2].v};
+    const double gam = ::sqrt(1.0 + (v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]));
+    const double xn = x + v1[0] / gam * K.dt / K.dx;
+    const double yn = y + v1[1] / gam * K.dt / K.dy;
+    const double zn = z + v1[2] / gam * K.dt / K.dz;
+    const double dtd[3] = {K.dt / K.dx, K.dt / K.dy, K.dt / K.dz};
+    double lx0[3] = {0.0, 0.0, 0.0};
+    double gw = 0.0;
+    if (fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0) {
+        double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        gw = deposit_adj<M>(K.q, x, y, z, xn, yn, zn, K, m, lj, g);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lx0[a] += w * g[a];
+            lx[a] += w * g[3 + a];
+        }
+    }
+    // x1 = x0 + u1 / gamma(u1) * dt / d
+    const double ig = 1.0 / gam, ig3 = ig * ig * ig;
+    double lu1[3] = {lu[0], lu[1], lu[2]};
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        double s = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) s += lx[a] * dtd[a] * ((a == b ? ig : 0.0) - v1[a] * v1[b] * ig3);
+        lu1[b] += s;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) lx0[a] += lx[a];
+    // Boris transpose
+    double lF[6];
+#pragma unroll
+    for (int b = 0; b < 9; b++) {
+        const double s = lu1[0] * u1[0].d[b] + lu1[1] * u1[1].d[b] + lu1[2] * u1[2].d[b];
+        if (b < 3) lu[b] = s;
+        else lF[b - 3] = s;
+    }
+    // gather transpose
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+#pragma unroll
+        for (int a = 0; a < 3; a++) lx0[a] += lF[c] * Fg[c][a];
+        const double* WX = SX[c] ? Wd[0] : Wp[0];
+        const double* WY = SY[c] ? Wd[1] : Wp[1];
+        const double* WZ = SZ[c] ? Wd[2] : Wp[2];
+        const int i0 = (SX[c] ? id : ip) - H, j0 = (SY[c] ? jd : jp) - H, k0 = (SZ[c] ? kd : kp) - H;
+        double wzr[M + 1];
+#pragma unroll
+        for (int t = 0; t <= M; t++) wzr[t] = WZ[t];
+#pragma unroll 1
+        for (int k = 0; k <= M; k++) {
+            const double wz = wzr[0] * lF[c];
+            rot_down<M + 1>(wzr);
+#pragma unroll
+            for (int j = 0; j <= M; j++) {
+                const double yz = WY[j] * wz;
+#pragma unroll
+                for (int i = 0; i <= M; i++) lf(c, k0 + k, j0 + j, i0 + i, WX[i] * yz);
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++) lx[a] = lx0[a];
+    return gw;
+}
+
+// separable adjoint push, any particle order (HBM atomics)
+template <int M>
+__global__ __launch_bounds__(256) void k_adj_push64_sep(AdjPush A, Grid g, PushConsts K) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const double w = A.w[p];
+        if (!(w == w)) continue;
+        double x = A.x[p], y = A.y[p], z = A.z[p];
+        if (!wrap_check(x, y, z, K)) continue;
+        const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
+        double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
+        double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
+        const double gw = adj_particle<M>(
+            x, y, z, u0, w, K, g.m,
+            [&](int c, int k, int j, int i) { return A.F[c][g.fi(k, j, i)]; },
+            [&](int c, int k, int j, int i) { return A.lJ[c][g.ji(k, j, i)]; },
+            [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); }, lx, lu);
+        A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
+        A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
+        A.lw[p] += gw;
+    }
+}
+
+// LDS-tiled adjoint push: one workgroup per (tile, chunk) work item of the
+// frozen tile grouping.  lambda_J of the tile's deposit box is staged in
+// LDS; lambda_F of the tile's gather box accumulates in LDS (ds_add_f64)
+// and is flushed with one global atomic per non-zero point.  Particles
+// that left their tile take the HBM path.
+template <int M, int TX, int TY, int TZ>
+struct AdjBox {
+    static constexpr int H = M / 2;
+    static constexpr int AX = TX + M + 3, AY = TY + M + 3, AZ = TZ + M + 3, AVOL = AX * AY * AZ;
+    static constexpr int FX = TX + M + 1, FY = TY + M + 1, FZ = TZ + M + 1, FVOL = FX * FY * FZ;
+    static constexpr size_t LDS = (size_t)(3 * AVOL + 12 * FVOL) * sizeof(double);
+};
+
+template <int M, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, PushConsts K, Tiling tl,
+                                                          const int64_t* __restrict__ tile_off,
+                                                          const int64_t* __restrict__ work,
+                                                          const int64_t* __restrict__ nwork, int chunk) {
+    using B = AdjBox<M, TX, TY, TZ>;
+    extern __shared__ double lds_adj[];
+    double* lj = lds_adj;                 // 3 * AVOL: lambda_J of the deposit box
+    double* lfb = lds_adj + 3 * B::AVOL;  // 6 * FVOL: lambda_F accumulator
+    double* fld = lfb + 6 * B::FVOL;      // 6 * FVOL: the fields of the gather box
+    const int64_t wi = blockIdx.x;
+    if (wi >= *nwork) return;
+    const int64_t item = work[wi];
+    const int tile = (int)(item >> 20);
+    const int64_t cidx = item & ((1 << 20) - 1);
+    const int64_t beg = tile_off[tile] + cidx * chunk;
+    const int64_t end = min(tile_off[tile + 1], beg + chunk);
+    const int txi = tile % tl.nt[0], tyi = (tile / tl.nt[0]) % tl.nt[1], tzi = tile / (tl.nt[0] * tl.nt[1]);
+    const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
+    const int m = g.m;
+    // deposit box origin (J buffer indices) and gather box origin (E/B)
+    const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
+    const int fox = T0x + m - 1 - B::H, foy = T0y + m - 1 - B::H, foz = T0z + m - 1 - B::H;
+    for (int e = threadIdx.x; e < 3 * B::AVOL; e += blockDim.x) {
+        const int c = e / B::AVOL, r = e % B::AVOL;
+        const int lx = r % B::AX, ly = (r / B::AX) % B::AY, lz = r / (B::AX * B::AY);
+        // the box of a partial last tile reaches past the array: clamp (unused)
+        const int gx = min(aox + lx, g.je[0] - 2), gy = min(aoy + ly, g.je[1] - 2), gz = min(aoz + lz, g.je[2] - 2);
+        lj[e] = A.lJ[c][g.ji(gz, gy, gx)];
+    }
+    for (int e = threadIdx.x; e < 6 * B::FVOL; e += blockDim.x) {
+        lfb[e] = 0.0;
+        const int c = e / B::FVOL, r = e % B::FVOL;
+        const int lx = r % B::FX, ly = (r / B::FX) % B::FY, lz = r / (B::FX * B::FY);
+        const int gx = min(fox + lx, g.fe[0] - 1), gy = min(foy + ly, g.fe[1] - 1), gz = min(foz + lz, g.fe[2] - 1);
+        fld[e] = A.F[c][g.fi(gz, gy, gx)];
+    }
+    __syncthreads();
+    uint32_t strays = 0;
+    int it = 0;
+    const int len = (int)(end - beg);
+    for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
+        const int64_t p = beg + visit_slot(t, len);
+        const double w = A.w[p];
+        if (!(w == w)) continue;
+        double x = A.x[p], y = A.y[p], z = A.z[p];
+        if (!wrap_check(x, y, z, K)) continue;
+        const int cx = min((int)x, g.n[0] - 1), cy = min((int)y, g.n[1] - 1), cz = min((int)z, g.n[2] - 1);
+        if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
+            (unsigned)(cz - T0z) >= (unsigned)TZ) {
+            strays |= 1u << it;
+            continue;
+        }
+        const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
+        double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
+        double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
+        const double gw = adj_particle<M>(
+            x, y, z, u0, w, K, m,
+            [&](int c, int k, int j, int i) {
+                return fld[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)];
+            },
+            [&](int c, int k, int j, int i) {
+                return lj[c * B::AVOL + ((k - aoz) * B::AY + (j - aoy)) * B::AX + (i - aox)];
+            },
+            [&](int c, int k, int j, int i, double v) {
+                lds_add(&lfb[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)], v);
+            },
+            lx, lu);
+        A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
+        A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
+        A.lw[p] += gw;
+    }
+    if (strays) {
+        it = 0;
+        for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
+            if (!(strays & (1u << it))) continue;
+            const int64_t p = beg + visit_slot(t, len);
+            double x = A.x[p], y = A.y[p], z = A.z[p];
+            wrap_check(x, y, z, K);
+            const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
+            double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
+            double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
+            const double gw = adj_particle<M>(
+                x, y, z, u0, A.w[p], K, m,
+                [&](int c, int k, int j, int i) { return A.F[c][g.fi(k, j, i)]; },
+                [&](int c, int k, int j, int i) { return A.lJ[c][g.ji(k, j, i)]; },
+                [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); }, lx, lu);
+            A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
+            A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
+            A.lw[p] += gw;
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 6 * B::FVOL; e += blockDim.x) {
+        const double v = lfb[e];
+        if (v == 0.0) continue;
+        const int c = e / B::FVOL, r = e % B::FVOL;
+        const int gx = fox + r % B::FX, gy = foy + (r / B::FX) % B::FY, gz = foz + r / (B::FX * B::FY);
+        if (gx >= g.fe[0] || gy >= g.fe[1] || gz >= g.fe[2]) continue;
+        hbm_add(&A.lF[c][g.fi(gz, gy, gx)], v);
+    }
+}
+
 // adjoint of the final kinetic energy sum_p m w (gamma - 1) dV
 __global__ __launch_bounds__(256) void k_adj_objective(AdjPush A, double mdv, double gout) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.n;
@@ -704,6 +1136,41 @@ AdjPush adj_view(pic_ctx* c, DiffState& D, const char* save) {
     return A;
 }
 
+template <int M>
+int launch_adj_tiled(pic_ctx* c, DiffState& D, const AdjPush& A, const PushConsts& K) {
+    using TS = TileShape<0, M>;
+    using B = AdjBox<M, TS::X, TS::Y, TS::Z>;
+    Species& S = c->sp[D.s];
+    int r = build_work(c, D.s, PIC_PUSH_CHUNK);
+    if (r) return r;
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_adj_push64_tiled<M, TS::X, TS::Y, TS::Z>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
+    const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
+    k_adj_push64_tiled<M, TS::X, TS::Y, TS::Z><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
+        A, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+// adjoint push: the LDS-tiled kernel when the run's forward is tiled, the
+// separable HBM kernel for the direct forward, or (PIC_OPT_ADJ_PUSH = 1)
+// the dual-number kernel, an independent evaluation of the same derivatives
+int launch_adj_push(pic_ctx* c, DiffState& D, const AdjPush& A, const PushConsts& K) {
+    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(D.slots, 1), 256), 8192);
+    const bool q = c->g.m == 2;
+    if (c->adj_push == 1) {
+        if (q) k_adj_push64<2><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
+        else k_adj_push64<4><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
+    } else if (D.kernel == 2 && c->adj_push == 0) {
+        return q ? launch_adj_tiled<2>(c, D, A, K) : launch_adj_tiled<4>(c, D, A, K);
+    } else {
+        if (q) k_adj_push64_sep<2><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
+        else k_adj_push64_sep<4><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
+    }
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
 // reverse of diff_step for step st, from the saved push-time state
 int adj_step(pic_ctx* c, DiffState& D, int st, const char* save) {
     const double h = c->p.dt / 2.0;
@@ -717,10 +1184,7 @@ int adj_step(pic_ctx* c, DiffState& D, int st, const char* save) {
     if ((r = adj_fold(c, D))) return r;
     const PushConsts K = push_consts64(c, c->sp[D.s]);
     const AdjPush A = adj_view(c, D, save);
-    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(D.slots, 1), 256), 8192);
-    if (c->g.m == 2) k_adj_push64<2><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
-    else k_adj_push64<4><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
-    PIC_LAUNCH_CHECK(c);
+    if ((r = launch_adj_push(c, D, A, K))) return r;
     // clear: the currents before it get their adjoint from this step's
     // E half step and laser boundary only
     for (int f = 0; f < 3; f++)

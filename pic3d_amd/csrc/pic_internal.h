@@ -105,6 +105,7 @@ struct pic_ctx {
     int env_on = 0;
     double env_val = 0.0;
     pic::DiffState* diff = nullptr;
+    int adj_push = 0;             // adjoint push kernel: 0 auto, 1 dual numbers, 2 separable HBM
     // per-phase timing (PIC_OPT_TIMING): event pairs around each phase
     int timing = 0;
     std::vector<hipEvent_t> tev;      // event pool
