@@ -646,7 +646,9 @@ template <int M, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, PushConsts K, Tiling tl,
                                                           const int64_t* __restrict__ tile_off,
                                                           const int64_t* __restrict__ work,
-                                                          const int64_t* __restrict__ nwork, int chunk) {
+                                                          const int64_t* __restrict__ nwork, int chunk,
+                                                          int64_t* __restrict__ stray_list,
+                                                          int64_t* __restrict__ stray_n) {
     using B = AdjBox<M, TX, TY, TZ>;
     extern __shared__ double lds_adj[];
     double* lj = lds_adj;                 // 3 * AVOL: lambda_J of the deposit box
@@ -714,25 +716,15 @@ __global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, Pus
         A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
         A.lw[p] += gw;
     }
+    // particles that left their tile: appended to the list the HBM kernel
+    // (k_adj_push64_list) processes after this one
     if (strays) {
         it = 0;
-        for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
-            if (!(strays & (1u << it))) continue;
-            const int64_t p = beg + visit_slot(t, len);
-            double x = A.x[p], y = A.y[p], z = A.z[p];
-            wrap_check(x, y, z, K);
-            const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
-            double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
-            double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
-            const double gw = adj_particle<M>(
-                x, y, z, u0, A.w[p], K, m,
-                [&](int c, int k, int j, int i) { return A.F[c][g.fi(k, j, i)]; },
-                [&](int c, int k, int j, int i) { return A.lJ[c][g.ji(k, j, i)]; },
-                [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); }, lx, lu);
-            A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
-            A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
-            A.lw[p] += gw;
-        }
+        for (int t = threadIdx.x; t < len; t += blockDim.x, it++)
+            if (strays & (1u << it)) {
+                const unsigned long long at = atomicAdd((unsigned long long*)stray_n, 1ull);
+                stray_list[at] = beg + visit_slot(t, len);
+            }
     }
     __syncthreads();
     for (int e = threadIdx.x; e < 6 * B::FVOL; e += blockDim.x) {
@@ -742,6 +734,31 @@ __global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, Pus
         const int gx = fox + r % B::FX, gy = foy + (r / B::FX) % B::FY, gz = foz + r / (B::FX * B::FY);
         if (gx >= g.fe[0] || gy >= g.fe[1] || gz >= g.fe[2]) continue;
         hbm_add(&A.lF[c][g.fi(gz, gy, gx)], v);
+    }
+}
+
+// the tiled kernel's strays, through HBM
+template <int M>
+__global__ __launch_bounds__(256) void k_adj_push64_list(AdjPush A, Grid g, PushConsts K,
+                                                         const int64_t* __restrict__ list,
+                                                         const int64_t* __restrict__ n) {
+    const int64_t cnt = *n;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cnt;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = list[q];
+        double x = A.x[p], y = A.y[p], z = A.z[p];
+        wrap_check(x, y, z, K);
+        const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
+        double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
+        double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
+        const double gw = adj_particle<M>(
+            x, y, z, u0, A.w[p], K, g.m,
+            [&](int c, int k, int j, int i) { return A.F[c][g.fi(k, j, i)]; },
+            [&](int c, int k, int j, int i) { return A.lJ[c][g.ji(k, j, i)]; },
+            [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); }, lx, lu);
+        A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
+        A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
+        A.lw[p] += gw;
     }
 }
 
@@ -1006,6 +1023,7 @@ struct DiffState {
     double* lJ[3] = {};
     double* lp[7] = {};              // x y z ux uy uz w
     double* genv = nullptr;          // per step (device)
+    int64_t* stray = nullptr;        // tiled adjoint: slots outside their tile (+ count)
     double* red = nullptr;           // reduction scratch
 };
 
@@ -1146,8 +1164,12 @@ int launch_adj_tiled(pic_ctx* c, DiffState& D, const AdjPush& A, const PushConst
     PIC_HIP(c, hipFuncSetAttribute((const void*)k_adj_push64_tiled<M, TS::X, TS::Y, TS::Z>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
     const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
+    int64_t* cnt = D.stray + std::max<int64_t>(D.slots, 1);
+    if ((r = launch_fill(c, cnt, 16))) return r;
     k_adj_push64_tiled<M, TS::X, TS::Y, TS::Z><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
-        A, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK);
+        A, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, D.stray, cnt);
+    PIC_LAUNCH_CHECK(c);
+    k_adj_push64_list<M><<<1024, 256, 0, c->stream>>>(A, c->g, K, D.stray, cnt);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
@@ -1211,6 +1233,8 @@ void free_adjoint(DiffState& D) {
     for (auto& p : D.lp) { if (p) hipFree(p); p = nullptr; }
     if (D.genv) hipFree(D.genv);
     D.genv = nullptr;
+    if (D.stray) hipFree(D.stray);
+    D.stray = nullptr;
     if (D.seg) hipFree(D.seg);
     D.seg = nullptr;
     D.seg_steps = 0;
@@ -1389,6 +1413,7 @@ int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, 
     for (int f = 0; f < 3 && !r; f++) r = dalloc(c, (void**)&D.lJ[f], g.jsize * 8);
     for (int f = 0; f < 7 && !r; f++) r = dalloc(c, (void**)&D.lp[f], std::max<int64_t>(D.slots, 1) * 8);
     if (!r) r = dalloc(c, (void**)&D.genv, std::max(D.nsteps, 1) * 8);
+    if (!r) r = dalloc(c, (void**)&D.stray, (std::max<int64_t>(D.slots, 1) + 1) * 8);
     D.step_bytes = 7 * ((D.slots * 8 + 255) / 256 * 256) + 6 * ((g.fsize * 8 + 255) / 256 * 256);
     D.seg_steps = std::min(D.K, std::max(D.nsteps, 1));
     if (!r) r = dalloc(c, (void**)&D.seg, D.step_bytes * D.seg_steps);
