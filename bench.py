@@ -164,6 +164,44 @@ def run_gpu(args, precision: str) -> dict:
     return out
 
 
+def run_adjoint(args) -> dict:
+    """Differentiation form: gradient of the final electron kinetic energy
+    with respect to every step's x-min laser envelope value and every
+    initial weight (FP64), on the laser-slab case at 1/8 of its cells
+    (128x64x64, same shapes and ppc, the slab moved to x in [8, 88) so the pulse reaches it within the run: 5.2M electrons).
+    Times one plain forward step (tiled push, re-sorted every step), the
+    differentiated forward (checkpoint every K steps) and the backward
+    (recompute + adjoint kernels), per step."""
+    from pic3d_amd import Simulation, workloads as W
+    from pic3d_amd.diff import DiffRun
+    from pic3d_amd.simulation import setup
+    wl = W.laser_slab(nx=128, ny=64, nz=64, slab=(8.0, 88.0))
+    sim = Simulation(wl.params)
+    setup(sim, wl, device_loader=True)
+    sim.preloop()
+    sim.step(1, 2)
+    sim.sync()
+    t = time.perf_counter()
+    sim.step(3, 5)
+    sim.sync()
+    fwd = (time.perf_counter() - t) / 5
+    nsteps, K = args.adjoint_steps, 10
+    run = DiffRun(sim, 0, first_step=8, nsteps=nsteps, checkpoint_every=K)
+    t = time.perf_counter()
+    obj = run.forward()
+    tf = (time.perf_counter() - t) / nsteps
+    t = time.perf_counter()
+    ge, gw = run.backward()
+    tb = (time.perf_counter() - t) / nsteps
+    run.close()
+    sim.close()
+    return {"workload": "laser_slab 128x64x64, mem 4, 16 ppc in x[8,88) (5.2M e-)",
+            "steps": nsteps, "checkpoint_every": K, "objective": obj,
+            "forward_ms_per_step": fwd * 1e3, "diff_forward_ms_per_step": tf * 1e3,
+            "backward_ms_per_step": tb * 1e3, "backward_over_forward": tb / fwd,
+            "grad_envelope_norm": float((ge ** 2).sum() ** 0.5)}
+
+
 def copy_probe(nbytes: int):
     import torch
     n = nbytes // 4
@@ -189,6 +227,8 @@ def main():
                     help="FP32 deposit accumulators: 1 int32 fixed point (default), 0 FP64 in LDS")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--adjoint-steps", type=int, default=20,
+                    help="steps of the differentiated run (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (read BYTES, write BYTES) "
                          "three times: the known-size kernel that scales rocprofv3 "
@@ -227,6 +267,8 @@ def main():
     }
     if "fp64" in res and "fp32" in res:
         line["fp32_over_fp64"] = res["fp32"]["pushes_per_s"] / res["fp64"]["pushes_per_s"]
+    if args.adjoint_steps > 0:
+        line["adjoint"] = run_adjoint(args)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.workload, hp, args.cpu_threads)
     print(json.dumps(line), flush=True)

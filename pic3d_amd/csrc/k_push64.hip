@@ -326,7 +326,7 @@ struct AccBox {
 };
 
 template <int M, int TX, int TY, int TZ>
-__global__ __launch_bounds__(256) void k_push64_tiled(P64 P, Fields64 F, Grid g, PushConsts K,
+__global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid g, PushConsts K,
                                                       Tiling tl, const int64_t* __restrict__ tile_off,
                                                       const int64_t* __restrict__ work,
                                                       const int64_t* __restrict__ nwork, int chunk) {
