@@ -179,20 +179,22 @@ __host__ __device__ inline int visit_slot(int t, int len) {
     return (t & 63) * (l64 >> 6) + (t >> 6);
 }
 
-// Tile edges of the sorted push, chosen so two workgroups' LDS (field tile
-// + accumulator box) fit in one CU's 160 KB.  KIND: 0 = FP64 fields and
-// FP64 accumulators; 1 = FP32 fields, FP64 accumulators; 2 = FP32 fields,
-// int32 fixed-point accumulators.
-//   0: quadratic 8x4x4 (60 KB), quartic 4x4x4 (78 KB)
+// Tile edges of the sorted push (LDS per workgroup: field tile +
+// accumulator box; a CU has 160 KB).  KIND: 0 = FP64 fields and FP64
+// accumulators; 1 = FP32 fields, FP64 accumulators; 2 = FP32 fields, int32
+// fixed-point accumulators.
+//   0: quadratic 8x4x4 (60 KB), quartic 4x4x4 (78 KB): two workgroups per CU
 //   1: quadratic 8x8x4 (69 KB), quartic 4x4x4 (60 KB)
-//   2: quadratic 8x8x8 (67 KB), quartic 8x8x4 (76 KB)
+//   2: quadratic 8x8x8 (67 KB), quartic 8x4x4 (54 KB).  Measured at the
+//      laser-slab case: quartic 8x4x4 (three workgroups per CU) 12.8 ms vs
+//      8x8x4 (two) 14.5 ms; quadratic 8x8x8 6.2 ms vs 8x8x4 6.8 ms.
 template <int KIND, int M> struct TileShape;
 template <> struct TileShape<0, 2> { static constexpr int X = 8, Y = 4, Z = 4; };
 template <> struct TileShape<0, 4> { static constexpr int X = 4, Y = 4, Z = 4; };
 template <> struct TileShape<1, 2> { static constexpr int X = 8, Y = 8, Z = 4; };
 template <> struct TileShape<1, 4> { static constexpr int X = 4, Y = 4, Z = 4; };
 template <> struct TileShape<2, 2> { static constexpr int X = 8, Y = 8, Z = 8; };
-template <> struct TileShape<2, 4> { static constexpr int X = 8, Y = 8, Z = 4; };
+template <> struct TileShape<2, 4> { static constexpr int X = 8, Y = 4, Z = 4; };
 
 // Set c->tl for the context's precision, order and accumulator kind.
 void set_tiling(pic_ctx* c);
