@@ -98,6 +98,44 @@ class DiffRun:
             self._L.pic_diff_end(self.sim._h)
 
 
+def knot_matrix(knot_times, step_times) -> np.ndarray:
+    """Piecewise-linear interpolation (hat functions) from envelope knot
+    amplitudes at knot_times to the envelope value of every step:
+    envelope = Phi @ knots, Phi of shape (nsteps, nknots).  Outside the
+    knot range the end knots hold."""
+    kt = np.asarray(knot_times, dtype=np.float64)
+    ts = np.asarray(step_times, dtype=np.float64)
+    phi = np.zeros((len(ts), len(kt)))
+    for n, t in enumerate(ts):
+        if t <= kt[0]:
+            phi[n, 0] = 1.0
+        elif t >= kt[-1]:
+            phi[n, -1] = 1.0
+        else:
+            k = int(np.searchsorted(kt, t, side="right") - 1)
+            a = (t - kt[k]) / (kt[k + 1] - kt[k])
+            phi[n, k] = 1.0 - a
+            phi[n, k + 1] = a
+    return phi
+
+
+def step_times(run) -> np.ndarray:
+    """t_n = n dt of the run's steps (the time radiating_boundary uses)."""
+    return (run.first + np.arange(run.nsteps)) * run.sim.p.dt
+
+
+def reference_knots(run, nknots: int):
+    """Knot times spread over the run and the reference envelope there
+    (a0 sin(pi t / T) on [0, T])."""
+    p = run.sim.p
+    ts = step_times(run)
+    kt = np.linspace(ts[0], ts[-1], nknots)
+    t = kt - p.laser_start_time
+    amp = np.where((t >= 0) & (t <= p.laser_pulse_duration),
+                   p.laser_amplitude * np.sin(p.pi * np.clip(t, 0, None) / p.laser_pulse_duration), 0.0)
+    return kt, amp
+
+
 def per_cell(grad_w: np.ndarray, x, y, z, params, ppc: float) -> np.ndarray:
     """d objective / d density per cell for a loader that gives every
     particle of a cell the weight density / ppc: the slot gradients summed
@@ -132,6 +170,18 @@ try:
             ge, gw = ctx.run.backward(float(grad_out))
             dev = ctx.dev
             return (torch.from_numpy(ge).to(dev), torch.from_numpy(np.nan_to_num(gw)).to(dev), None)
+
+    def envelope_from_knots(knots, phi: np.ndarray):
+        """Step envelope from knot amplitudes (a torch tensor), differentiable."""
+        return torch.from_numpy(phi).to(knots.dtype) @ knots
+
+    def weights_from_profile(profile, cell_x: np.ndarray, ppc: float, live: np.ndarray):
+        """Slot weights from an x density profile (a torch tensor over the x
+        cells): w = n(x cell of the slot's initial position) / ppc; dead
+        slots keep NaN."""
+        w = profile[torch.from_numpy(cell_x)] / ppc
+        nan = torch.full_like(w, float("nan"))
+        return torch.where(torch.from_numpy(live), w, nan)
 
 except ImportError:  # torch is optional for the numpy interface above
     KineticEnergy = None
