@@ -67,13 +67,13 @@ def load_traffic(name: str):
 
 def cpu_baseline(wl_name: str, precision: str, threads: int) -> dict:
     """The CPU oracle (FP64 restatement of the reference loop, OpenMP) on a
-    bounded sample of the same workload: the full grid with a thinner
-    electron slab, pre-loop push + 2 timed steps."""
+    bounded sample of the same workload, about 10-30 s of CPU work: the full
+    grid with a thinner electron slab, pre-loop push + 4 timed steps."""
     from oracle.oracle import Oracle, load_particles
     from pic3d_amd import workloads as W
     if wl_name == "laser_slab":
-        wl = W.laser_slab(slab=(64.0, 72.0))
-        sample = "laser_slab grid 256x128x128, mem 4, slab x in [64,72) 16 ppc (2.1M e-), 2 steps"
+        wl = W.laser_slab(slab=(64.0, 160.0))
+        sample = "laser_slab grid 256x128x128, mem 4, slab x in [64,160) 16 ppc (25.2M e-), 4 steps"
     elif wl_name == "thermal":
         wl = W.thermal()
         sample = "thermal 32^3, 8 ppc, mem 2, 2 steps"
@@ -91,11 +91,12 @@ def cpu_baseline(wl_name: str, precision: str, threads: int) -> dict:
     o.preloop()
     n0 = o.num_particles(0)
     t = time.perf_counter()
-    o.step(1, 2)
+    nst = 4
+    o.step(1, nst)
     dt = time.perf_counter() - t
     n1 = o.num_particles(0)
     o.close()
-    return {"value": 2 * 0.5 * (n0 + n1) / dt, "unit": "particle pushes/s", "cores": threads,
+    return {"value": nst * 0.5 * (n0 + n1) / dt, "unit": "particle pushes/s", "cores": threads,
             "kind": "port", "sample": sample, "seconds": round(dt, 3)}
 
 
@@ -227,6 +228,9 @@ def main():
                     help="FP32 deposit accumulators: 1 int32 fixed point (default), 0 FP64 in LDS")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--ramp-steps", type=int, default=5,
+                    help="timed steps of the FP32 ramp-target throughput case reported beside "
+                         "the headline (0: skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=20,
                     help="steps of the differentiated run (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -267,6 +271,13 @@ def main():
     }
     if "fp64" in res and "fp32" in res:
         line["fp32_over_fp64"] = res["fp32"]["pushes_per_s"] / res["fp64"]["pushes_per_s"]
+    if args.ramp_steps > 0 and args.workload == "laser_slab":
+        ra = argparse.Namespace(**vars(args))
+        ra.workload, ra.steps, ra.warmup = "ramp_target", args.ramp_steps, 2
+        r = run_gpu(ra, "fp32")
+        line["ramp_target"] = {k: r[k] for k in ("pushes_per_s", "ms_per_step", "push_kernel_pushes_per_s",
+                                                 "yee_cell_updates_per_s", "phase_ms_per_step",
+                                                 "particles", "cells", "roofline", "workload")}
     if args.adjoint_steps > 0:
         line["adjoint"] = run_adjoint(args)
     if not args.no_cpu_baseline:
