@@ -35,77 +35,85 @@
 namespace pic {
 
 // ---------------------------------------------------------------------
-// forward-mode dual numbers
+// forward-mode dual numbers (value + N partial derivatives, scalar S)
 // ---------------------------------------------------------------------
-template <int N> struct Dual {
-    double v;
-    double d[N];
+template <int N, typename S = double> struct Dual {
+    S v;
+    S d[N];
     __device__ Dual() {}
-    __device__ Dual(double x) : v(x) {
+    __device__ Dual(double x) : v((S)x) {
 #pragma unroll
-        for (int i = 0; i < N; i++) d[i] = 0.0;
+        for (int i = 0; i < N; i++) d[i] = S(0);
+    }
+    __device__ Dual(float x) : v((S)x) {
+#pragma unroll
+        for (int i = 0; i < N; i++) d[i] = S(0);
     }
 };
-template <int N> __device__ __forceinline__ Dual<N> dvar(double x, int k) {
-    Dual<N> r(x);
-    r.d[k] = 1.0;
+template <int N, typename S = double> __device__ __forceinline__ Dual<N, S> dvar(S x, int k) {
+    Dual<N, S> r(x);
+    r.d[k] = S(1);
     return r;
 }
-template <int N> __device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) {
-    Dual<N> r(a.v + b.v);
+#define DUAL_T template <int N, typename S>
+DUAL_T __device__ __forceinline__ Dual<N, S> operator+(const Dual<N, S>& a, const Dual<N, S>& b) {
+    Dual<N, S> r(a.v + b.v);
 #pragma unroll
     for (int i = 0; i < N; i++) r.d[i] = a.d[i] + b.d[i];
     return r;
 }
-template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) {
-    Dual<N> r(a.v - b.v);
+DUAL_T __device__ __forceinline__ Dual<N, S> operator-(const Dual<N, S>& a, const Dual<N, S>& b) {
+    Dual<N, S> r(a.v - b.v);
 #pragma unroll
     for (int i = 0; i < N; i++) r.d[i] = a.d[i] - b.d[i];
     return r;
 }
-template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a) {
-    Dual<N> r(-a.v);
+DUAL_T __device__ __forceinline__ Dual<N, S> operator-(const Dual<N, S>& a) {
+    Dual<N, S> r(-a.v);
 #pragma unroll
     for (int i = 0; i < N; i++) r.d[i] = -a.d[i];
     return r;
 }
-template <int N> __device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) {
-    Dual<N> r(a.v * b.v);
+DUAL_T __device__ __forceinline__ Dual<N, S> operator*(const Dual<N, S>& a, const Dual<N, S>& b) {
+    Dual<N, S> r(a.v * b.v);
 #pragma unroll
     for (int i = 0; i < N; i++) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
     return r;
 }
-template <int N> __device__ __forceinline__ Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
-    const double inv = 1.0 / b.v;
-    Dual<N> r(a.v * inv);
+DUAL_T __device__ __forceinline__ Dual<N, S> operator/(const Dual<N, S>& a, const Dual<N, S>& b) {
+    const S inv = S(1) / b.v;
+    Dual<N, S> r(a.v * inv);
 #pragma unroll
     for (int i = 0; i < N; i++) r.d[i] = (a.d[i] - r.v * b.d[i]) * inv;
     return r;
 }
-template <int N> __device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, double b) { return a + Dual<N>(b); }
-template <int N> __device__ __forceinline__ Dual<N> operator+(double a, const Dual<N>& b) { return Dual<N>(a) + b; }
-template <int N> __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, double b) { return a - Dual<N>(b); }
-template <int N> __device__ __forceinline__ Dual<N> operator-(double a, const Dual<N>& b) { return Dual<N>(a) - b; }
-template <int N> __device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, double b) {
-    Dual<N> r(a.v * b);
+DUAL_T __device__ __forceinline__ Dual<N, S> operator+(const Dual<N, S>& a, double b) { return a + Dual<N, S>(b); }
+DUAL_T __device__ __forceinline__ Dual<N, S> operator+(double a, const Dual<N, S>& b) { return Dual<N, S>(a) + b; }
+DUAL_T __device__ __forceinline__ Dual<N, S> operator-(const Dual<N, S>& a, double b) { return a - Dual<N, S>(b); }
+DUAL_T __device__ __forceinline__ Dual<N, S> operator-(double a, const Dual<N, S>& b) { return Dual<N, S>(a) - b; }
+DUAL_T __device__ __forceinline__ Dual<N, S> operator*(const Dual<N, S>& a, double b) {
+    const S bs = (S)b;
+    Dual<N, S> r(a.v * bs);
 #pragma unroll
-    for (int i = 0; i < N; i++) r.d[i] = a.d[i] * b;
+    for (int i = 0; i < N; i++) r.d[i] = a.d[i] * bs;
     return r;
 }
-template <int N> __device__ __forceinline__ Dual<N> operator*(double a, const Dual<N>& b) { return b * a; }
-template <int N> __device__ __forceinline__ Dual<N> operator/(const Dual<N>& a, double b) { return a * (1.0 / b); }
-template <int N> __device__ __forceinline__ Dual<N> operator/(double a, const Dual<N>& b) { return Dual<N>(a) / b; }
-template <int N> __device__ __forceinline__ Dual<N>& operator+=(Dual<N>& a, const Dual<N>& b) { return a = a + b; }
-template <int N> __device__ __forceinline__ Dual<N> sqrt(const Dual<N>& a) {
-    const double s = ::sqrt(a.v);
-    Dual<N> r(s);
-    const double k = 0.5 / s;
+DUAL_T __device__ __forceinline__ Dual<N, S> operator*(double a, const Dual<N, S>& b) { return b * a; }
+DUAL_T __device__ __forceinline__ Dual<N, S> operator/(const Dual<N, S>& a, double b) { return a * (1.0 / b); }
+DUAL_T __device__ __forceinline__ Dual<N, S> operator/(double a, const Dual<N, S>& b) { return Dual<N, S>(a) / b; }
+DUAL_T __device__ __forceinline__ Dual<N, S>& operator+=(Dual<N, S>& a, const Dual<N, S>& b) { return a = a + b; }
+DUAL_T __device__ __forceinline__ Dual<N, S> sqrt(const Dual<N, S>& a) {
+    const S s = ::sqrt(a.v);
+    Dual<N, S> r(s);
+    const S k = S(0.5) / s;
 #pragma unroll
     for (int i = 0; i < N; i++) r.d[i] = a.d[i] * k;
     return r;
 }
+#undef DUAL_T
 __device__ __forceinline__ double val(double x) { return x; }
-template <int N> __device__ __forceinline__ double val(const Dual<N>& x) { return x.v; }
+__device__ __forceinline__ float val(float x) { return x; }
+template <int N, typename S> __device__ __forceinline__ S val(const Dual<N, S>& x) { return x.v; }
 
 template <int M, typename T> __device__ __forceinline__ int shape_at_t(T x, T* W) {
     const int ic = (int)(val(x) + 0.5);
@@ -325,55 +333,63 @@ __global__ __launch_bounds__(256) void k_adj_push64(AdjPush A, Grid g, PushConst
 //   dG/dx0,1 = -cx sum_i dA_i/dx0,1 P_i,        P_i = sum_jk lJ_ijk W_jk
 //   dG/dy0,1 = -cx sum_j dY0,1_j sum_k q_jk dW/dY0,1,   q_jk = sum_i lJ_ijk A_i
 //   dG/dz0,1 likewise; Jy and Jz by permuting the axes.
+// Templated on the scalar T (double: the FP64 form; float: the FP32 form).
+// A position along an axis is an integer buffer node 'base' plus a
+// fraction 'frac' (FP64: base 0 and the whole buffer coordinate; FP32: the
+// particle's cell node and its in-cell offset, so no precision is lost far
+// from the origin).
 // ---------------------------------------------------------------------
-template <int M> __device__ __forceinline__ int shape_dv(double x, double* W, double* dW) {
-    Dual<1> Wd[M + 1];
-    const int ic = shape_at_t<M>(dvar<1>(x, 0), Wd);
+
+// weights and derivatives of the shape at buffer coordinate base + frac;
+// returns the centre node (int)(coordinate + 0.5)
+template <int M, typename T>
+__device__ __forceinline__ int shape_dv(int base, T frac, T* W, T* dW) {
+    const int t = (int)floor(val(frac) + T(0.5));
+    Dual<1, T> Wd[M + 1];
+    const Dual<1, T> x = dvar<1, T>(frac, 0);
+    if (M == 2) shape2_delta<Dual<1, T>>(x - Dual<1, T>((double)t), Wd);
+    else shape4_d<Dual<1, T>>(x - Dual<1, T>((double)(t - 2)), Wd);
 #pragma unroll
-    for (int t = 0; t <= M; t++) {
-        W[t] = Wd[t].v;
-        dW[t] = Wd[t].d[0];
+    for (int q = 0; q <= M; q++) {
+        W[q] = Wd[q].v;
+        dW[q] = Wd[q].d[0];
     }
-    return ic;
+    return base + t;
 }
 
 // register rotations for rolled loops (compile-time indices only):
 // rot_down reads element 0 per iteration; rot_push appends
-template <int L> __device__ __forceinline__ void rot_down(double* A) {
+template <int L, typename T> __device__ __forceinline__ void rot_down(T* A) {
 #pragma unroll
     for (int t = 0; t < L - 1; t++) A[t] = A[t + 1];
 }
-template <int L> __device__ __forceinline__ void rot_push(double* A, double v) {
+template <int L, typename T> __device__ __forceinline__ void rot_push(T* A, T v) {
 #pragma unroll
     for (int t = 0; t < L - 1; t++) A[t] = A[t + 1];
     A[L - 1] = v;
 }
 
 // weights S placed at offset d in a box of M+2
-template <int M> __device__ __forceinline__ void place_t(const double* S, int d, double* A) {
+template <int M, typename T> __device__ __forceinline__ void place_t(const T* S, int d, T* A) {
 #pragma unroll
     for (int t = 0; t < M + 2; t++) {
-        const double a0 = (t <= M) ? S[t] : 0.0;
-        const double a1 = (t >= 1) ? S[t - 1] : 0.0;
+        const T a0 = (t <= M) ? S[t] : T(0);
+        const T a1 = (t >= 1) ? S[t - 1] : T(0);
         A[t] = d ? a1 : a0;
     }
 }
 
-// Adjoint of one Esirkepov current component along axis 'a' (cumulative
+// Adjoint of one Esirkepov current component along axis a (cumulative
 // sum along a; weights over the other two axes b (middle loop) and c
-// (outer loop)).  lj(k, j, i) reads lambda of this component at box node
-// (k, j, i).  Adds to g[0..5] (d/dx0,y0,z0,x1,y1,z1) and returns G.
-template <int L, typename LJ>
-__device__ __forceinline__ double esirkepov_adj_axis(int a, const double* A0, const double* A1,
-                                                     const double* dA0, const double* dA1,
-                                                     const double* B0, const double* B1,
-                                                     const double* dB0, const double* dB1,
-                                                     const double* C0, const double* C1,
-                                                     const double* dC0, const double* dC1,
-                                                     double coef, LJ lj, double* g, int ab, int ac) {
-    // cumulative sums along a and their derivatives
-    double S[L], dS0[L], dS1[L];
-    double s = 0.0, s0 = 0.0, s1 = 0.0;
+// (outer loop)).  lj(oc, ob, oa) reads lambda of this component at the box
+// point.  Adds to g[0..5] (d/dx0,y0,z0,x1,y1,z1) and returns G.
+template <int L, typename T, typename LJ>
+__device__ __forceinline__ T esirkepov_adj_axis(int a, const T* A0, const T* A1, const T* dA0,
+                                                const T* dA1, const T* B0, const T* B1, const T* dB0,
+                                                const T* dB1, const T* C0, const T* C1, const T* dC0,
+                                                const T* dC1, T coef, LJ lj, T* g, int ab, int ac) {
+    T S[L], dS0[L], dS1[L];
+    T s = T(0), s0 = T(0), s1 = T(0);
 #pragma unroll
     for (int t = 0; t < L; t++) {
         s += A1[t] - A0[t];
@@ -381,39 +397,39 @@ __device__ __forceinline__ double esirkepov_adj_axis(int a, const double* A0, co
         s1 += dA1[t];
         S[t] = s; dS0[t] = s0; dS1[t] = s1;
     }
-    double P[L], R0[L], R1[L], T0[L], T1[L], Cr0[L], Cr1[L];
+    T P[L], R0[L], R1[L], T0[L], T1[L], Cr0[L], Cr1[L];
 #pragma unroll
     for (int t = 0; t < L; t++) {
-        P[t] = 0.0; R0[t] = 0.0; R1[t] = 0.0; T0[t] = 0.0; T1[t] = 0.0;
+        P[t] = T(0); R0[t] = T(0); R1[t] = T(0); T0[t] = T(0); T1[t] = T(0);
         Cr0[t] = C0[t]; Cr1[t] = C1[t];
     }
-    const double sixth = 1.0 / 6.0, third = 1.0 / 3.0;
+    const T sixth = T(1.0 / 6.0), third = T(1.0 / 3.0), half = T(0.5);
 #pragma unroll 1
     for (int oc = 0; oc < L; oc++) {
-        const double c0 = Cr0[0], dc = Cr1[0] - c0;
+        const T c0 = Cr0[0], dc = Cr1[0] - c0;
         rot_down<L>(Cr0);
         rot_down<L>(Cr1);
-        double t0 = 0.0, t1 = 0.0;
+        T t0 = T(0), t1 = T(0);
 #pragma unroll
         for (int ob = 0; ob < L; ob++) {
-            const double b0 = B0[ob], db = B1[ob] - b0;
-            const double W = b0 * c0 + 0.5 * db * c0 + 0.5 * b0 * dc + third * db * dc;
-            double q = 0.0;
+            const T b0 = B0[ob], db = B1[ob] - b0;
+            const T W = b0 * c0 + half * db * c0 + half * b0 * dc + third * db * dc;
+            T q = T(0);
 #pragma unroll
             for (int oa = 0; oa < L; oa++) {
-                const double lam = lj(oc, ob, oa);
+                const T lam = lj(oc, ob, oa);
                 P[oa] += lam * W;
                 q += lam * S[oa];
             }
-            R0[ob] += q * (0.5 * c0 + sixth * dc);
-            R1[ob] += q * (0.5 * c0 + third * dc);
-            t0 += q * (0.5 * b0 + sixth * db);
-            t1 += q * (0.5 * b0 + third * db);
+            R0[ob] += q * (half * c0 + sixth * dc);
+            R1[ob] += q * (half * c0 + third * dc);
+            t0 += q * (half * b0 + sixth * db);
+            t1 += q * (half * b0 + third * db);
         }
         rot_push<L>(T0, t0);
         rot_push<L>(T1, t1);
     }
-    double G = 0.0, ga0 = 0.0, ga1 = 0.0, gb0 = 0.0, gb1 = 0.0, gc0 = 0.0, gc1 = 0.0;
+    T G = T(0), ga0 = T(0), ga1 = T(0), gb0 = T(0), gb1 = T(0), gc0 = T(0), gc1 = T(0);
 #pragma unroll
     for (int t = 0; t < L; t++) {
         G += S[t] * P[t];
@@ -433,97 +449,99 @@ __device__ __forceinline__ double esirkepov_adj_axis(int a, const double* A0, co
     return -coef * G;
 }
 
-// g[0..5] += d/d(x0..z1) of sum lambda_J * J(q = 1 per unit weight);
-// returns the sum itself (d/dw).  lj(c, k, j, i) reads lambda_J of
-// component c at buffer node (k, j, i).
-template <int M, typename LJ>
-__device__ __forceinline__ double deposit_adj(double q, double xo, double yo, double zo, double xn,
-                                              double yn, double zn, const PushConsts& K, int m,
-                                              LJ lj, double* g) {
+// Constants of the differentiated push in the scalar of the form.
+template <typename T> struct AdjK {
+    T q, qm;        // charge; q dt / (2 m)
+    T dtd[3];       // dt / dx, dt / dy, dt / dz
+    T jc[3];        // q dx / dt, q dy / dt, q dz / dt (Esirkepov's factor per unit weight)
+};
+
+// g[0..5] += d/d(x0..z1) of sum lambda_J * J (per unit weight); returns
+// the sum itself (d/dw).  Old position base + f0, new base + f1 (buffer
+// coordinates).  lj(c, k, j, i) reads lambda_J of component c at buffer
+// node (k, j, i).
+template <int M, typename T, typename LJ>
+__device__ __forceinline__ T deposit_adj(const int* base, const T* f0, const T* f1,
+                                         const AdjK<T>& K, LJ lj, T* g) {
     constexpr int H = M / 2, L = M + 2;
-    const double org = (double)(1 - m);
-    double S[6][M + 1], dS[6][M + 1];
-    const int i1 = shape_dv<M>(xo - org, S[0], dS[0]), i2 = shape_dv<M>(xn - org, S[3], dS[3]);
-    const int j1 = shape_dv<M>(yo - org, S[1], dS[1]), j2 = shape_dv<M>(yn - org, S[4], dS[4]);
-    const int k1 = shape_dv<M>(zo - org, S[2], dS[2]), k2 = shape_dv<M>(zn - org, S[5], dS[5]);
+    T S[6][M + 1], dS[6][M + 1];
+    const int i1 = shape_dv<M, T>(base[0], f0[0], S[0], dS[0]), i2 = shape_dv<M, T>(base[0], f1[0], S[3], dS[3]);
+    const int j1 = shape_dv<M, T>(base[1], f0[1], S[1], dS[1]), j2 = shape_dv<M, T>(base[1], f1[1], S[4], dS[4]);
+    const int k1 = shape_dv<M, T>(base[2], f0[2], S[2], dS[2]), k2 = shape_dv<M, T>(base[2], f1[2], S[5], dS[5]);
     const int bx = min(i1, i2) - H, by = min(j1, j2) - H, bz = min(k1, k2) - H;
-    double X0[L], X1[L], Y0[L], Y1[L], Z0[L], Z1[L];
-    double dX0[L], dX1[L], dY0[L], dY1[L], dZ0[L], dZ1[L];
+    T X0[L], X1[L], Y0[L], Y1[L], Z0[L], Z1[L];
+    T dX0[L], dX1[L], dY0[L], dY1[L], dZ0[L], dZ1[L];
     place_t<M>(S[0], i1 - H - bx, X0); place_t<M>(dS[0], i1 - H - bx, dX0);
     place_t<M>(S[3], i2 - H - bx, X1); place_t<M>(dS[3], i2 - H - bx, dX1);
     place_t<M>(S[1], j1 - H - by, Y0); place_t<M>(dS[1], j1 - H - by, dY0);
     place_t<M>(S[4], j2 - H - by, Y1); place_t<M>(dS[4], j2 - H - by, dY1);
     place_t<M>(S[2], k1 - H - bz, Z0); place_t<M>(dS[2], k1 - H - bz, dZ0);
     place_t<M>(S[5], k2 - H - bz, Z1); place_t<M>(dS[5], k2 - H - bz, dZ1);
-    double G = 0.0;
+    T G = T(0);
     // Jx: cumulative in x, weights over y (middle) and z (outer)
-    G += esirkepov_adj_axis<L>(0, X0, X1, dX0, dX1, Y0, Y1, dY0, dY1, Z0, Z1, dZ0, dZ1,
-                               q * K.dx / K.dt,
-                               [&](int oz, int oy, int ox) { return lj(0, bz + oz, by + oy, bx + ox); },
-                               g, 1, 2);
+    G += esirkepov_adj_axis<L, T>(0, X0, X1, dX0, dX1, Y0, Y1, dY0, dY1, Z0, Z1, dZ0, dZ1, K.jc[0],
+                                  [&](int oz, int oy, int ox) { return lj(0, bz + oz, by + oy, bx + ox); },
+                                  g, 1, 2);
     // Jy: cumulative in y, weights over x (middle) and z (outer)
-    G += esirkepov_adj_axis<L>(1, Y0, Y1, dY0, dY1, X0, X1, dX0, dX1, Z0, Z1, dZ0, dZ1,
-                               q * K.dy / K.dt,
-                               [&](int oz, int ox, int oy) { return lj(1, bz + oz, by + oy, bx + ox); },
-                               g, 0, 2);
+    G += esirkepov_adj_axis<L, T>(1, Y0, Y1, dY0, dY1, X0, X1, dX0, dX1, Z0, Z1, dZ0, dZ1, K.jc[1],
+                                  [&](int oz, int ox, int oy) { return lj(1, bz + oz, by + oy, bx + ox); },
+                                  g, 0, 2);
     // Jz: cumulative in z, weights over x (middle) and y (outer)
-    G += esirkepov_adj_axis<L>(2, Z0, Z1, dZ0, dZ1, X0, X1, dX0, dX1, Y0, Y1, dY0, dY1,
-                               q * K.dz / K.dt,
-                               [&](int oy, int ox, int oz) { return lj(2, bz + oz, by + oy, bx + ox); },
-                               g, 0, 1);
+    G += esirkepov_adj_axis<L, T>(2, Z0, Z1, dZ0, dZ1, X0, X1, dX0, dX1, Y0, Y1, dY0, dY1, K.jc[2],
+                                  [&](int oy, int ox, int oz) { return lj(2, bz + oz, by + oy, bx + ox); },
+                                  g, 0, 1);
     return G;
 }
 
-// Adjoint of one particle's push.  fr(c, k, j, i): field c at a buffer
-// node; lj(c, k, j, i): lambda_J; lf(c, k, j, i, v): adds v to lambda_F.
-// lx, lu: in = adjoint after the push, out = before it.  Returns dK/dw
-// of this step's deposit.
-template <int M, typename FR, typename LJ, typename LF>
-__device__ __forceinline__ double adj_particle(double x, double y, double z, const double* u0v,
-                                               double w, const PushConsts& K, int m, FR fr, LJ lj,
-                                               LF lf, double* lx, double* lu) {
+// Adjoint of one particle's push from its state before the push:
+// position base + frac (buffer coordinates), momentum u0, weight w.
+// fr(c, k, j, i): field c at a buffer node; lj(c, k, j, i): lambda_J;
+// lf(c, k, j, i, v): adds v to lambda_F.  lx, lu: in = adjoint after the
+// push, out = before it.  Returns this step's d objective / d w.
+template <int M, typename T, typename FR, typename LJ, typename LF>
+__device__ __forceinline__ T adj_particle(const int* base, const T* frac, const T* u0v, T w,
+                                          const AdjK<T>& K, FR fr, LJ lj, LF lf, T* lx, T* lu) {
     constexpr int H = M / 2;
-    const double org = (double)(1 - m);
-    const double xc = x - org, yc = y - org, zc = z - org;
     // gather with position gradient, separably
-    double Wp[3][M + 1], dWp[3][M + 1], Wd[3][M + 1], dWd[3][M + 1];
-    const int ip = shape_dv<M>(xc, Wp[0], dWp[0]), jp = shape_dv<M>(yc, Wp[1], dWp[1]),
-              kp = shape_dv<M>(zc, Wp[2], dWp[2]);
-    shape_dv<M>(xc - 0.5, Wd[0], dWd[0]);
-    shape_dv<M>(yc - 0.5, Wd[1], dWd[1]);
-    shape_dv<M>(zc - 0.5, Wd[2], dWd[2]);
-    const int id = (int)xc, jd = (int)yc, kd = (int)zc;
+    T Wp[3][M + 1], dWp[3][M + 1], Wd[3][M + 1], dWd[3][M + 1];
+    int ipn[3], idn[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        ipn[a] = shape_dv<M, T>(base[a], frac[a], Wp[a], dWp[a]);
+        idn[a] = shape_dv<M, T>(base[a], frac[a] - T(0.5), Wd[a], dWd[a]);
+    }
     // staggering per component: 1 = dual along that axis
     constexpr int SX[6] = {1, 0, 0, 0, 1, 1}, SY[6] = {0, 1, 0, 1, 0, 1}, SZ[6] = {0, 0, 1, 1, 1, 0};
-    double Fv[6], Fg[6][3];
+    T Fv[6], Fg[6][3];
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-        const double* WX = SX[c] ? Wd[0] : Wp[0];
-        const double* dWX = SX[c] ? dWd[0] : dWp[0];
-        const double* WY = SY[c] ? Wd[1] : Wp[1];
-        const double* dWY = SY[c] ? dWd[1] : dWp[1];
-        const double* WZ = SZ[c] ? Wd[2] : Wp[2];
-        const double* dWZ = SZ[c] ? dWd[2] : dWp[2];
-        const int i0 = (SX[c] ? id : ip) - H, j0 = (SY[c] ? jd : jp) - H, k0 = (SZ[c] ? kd : kp) - H;
-        double v = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-        double wzr[M + 1], dwzr[M + 1];
+        const T* WX = SX[c] ? Wd[0] : Wp[0];
+        const T* dWX = SX[c] ? dWd[0] : dWp[0];
+        const T* WY = SY[c] ? Wd[1] : Wp[1];
+        const T* dWY = SY[c] ? dWd[1] : dWp[1];
+        const T* WZ = SZ[c] ? Wd[2] : Wp[2];
+        const T* dWZ = SZ[c] ? dWd[2] : dWp[2];
+        const int i0 = (SX[c] ? idn[0] : ipn[0]) - H, j0 = (SY[c] ? idn[1] : ipn[1]) - H,
+                  k0 = (SZ[c] ? idn[2] : ipn[2]) - H;
+        T v = T(0), gx = T(0), gy = T(0), gz = T(0);
+        T wzr[M + 1], dwzr[M + 1];
 #pragma unroll
         for (int t = 0; t <= M; t++) { wzr[t] = WZ[t]; dwzr[t] = dWZ[t]; }
 #pragma unroll 1
         for (int k = 0; k <= M; k++) {
-            const double wz = wzr[0], dwz = dwzr[0];
+            const T wz = wzr[0], dwz = dwzr[0];
             rot_down<M + 1>(wzr);
             rot_down<M + 1>(dwzr);
 #pragma unroll
             for (int j = 0; j <= M; j++) {
-                double s = 0.0, sd = 0.0;
+                T s = T(0), sd = T(0);
 #pragma unroll
                 for (int i = 0; i <= M; i++) {
-                    const double f = fr(c, k0 + k, j0 + j, i0 + i);
+                    const T f = fr(c, k0 + k, j0 + j, i0 + i);
                     s += WX[i] * f;
                     sd += dWX[i] * f;
                 }
-                const double yz = WY[j] * wz;
+                const T yz = WY[j] * wz;
                 v += yz * s;
                 gx += yz * sd;
                 gy += dWY[j] * wz * s;
@@ -534,23 +552,22 @@ __device__ __forceinline__ double adj_particle(double x, double y, double z, con
         Fg[c][0] = gx; Fg[c][1] = gy; Fg[c][2] = gz;
     }
     // Boris with d/d(u0, E, B)
-    using D9 = Dual<9>;
-    D9 u9[3] = {dvar<9>(u0v[0], 0), dvar<9>(u0v[1], 1), dvar<9>(u0v[2], 2)};
-    D9 E9[3] = {dvar<9>(Fv[0], 3), dvar<9>(Fv[1], 4), dvar<9>(Fv[2], 5)};
-    D9 B9[3] = {dvar<9>(Fv[3], 6), dvar<9>(Fv[4], 7), dvar<9>(Fv[5], 8)};
+    using D9 = Dual<9, T>;
+    D9 u9[3] = {dvar<9, T>(u0v[0], 0), dvar<9, T>(u0v[1], 1), dvar<9, T>(u0v[2], 2)};
+    D9 E9[3] = {dvar<9, T>(Fv[0], 3), dvar<9, T>(Fv[1], 4), dvar<9, T>(Fv[2], 5)};
+    D9 B9[3] = {dvar<9, T>(Fv[3], 6), dvar<9, T>(Fv[4], 7), dvar<9, T>(Fv[5], 8)};
     D9 u1[3];
-    boris_t<D9>(u9, E9, B9, K.qm, u1);
-    const double v1[3] = {u1[0].v, u1[1].v, u1[2].v};
-    const double gam = ::sqrt(1.0 + (v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]));
-    const double xn = x + v1[0] / gam * K.dt / K.dx;
-    const double yn = y + v1[1] / gam * K.dt / K.dy;
-    const double zn = z + v1[2] / gam * K.dt / K.dz;
-    const double dtd[3] = {K.dt / K.dx, K.dt / K.dy, K.dt / K.dz};
-    double lx0[3] = {0.0, 0.0, 0.0};
-    double gw = 0.0;
-    if (fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0) {
-        double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        gw = deposit_adj<M>(K.q, x, y, z, xn, yn, zn, K, m, lj, g);
+    boris_t<D9>(u9, E9, B9, (double)K.qm, u1);
+    const T v1[3] = {u1[0].v, u1[1].v, u1[2].v};
+    const T gam = ::sqrt(T(1) + (v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]));
+    T f1[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) f1[a] = frac[a] + v1[a] / gam * K.dtd[a];
+    T lx0[3] = {T(0), T(0), T(0)};
+    T gw = T(0);
+    if (fabs(f1[0] - frac[0]) < T(1) && fabs(f1[1] - frac[1]) < T(1) && fabs(f1[2] - frac[2]) < T(1)) {
+        T g[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        gw = deposit_adj<M, T>(base, frac, f1, K, lj, g);
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             lx0[a] += w * g[a];
@@ -558,22 +575,22 @@ __device__ __forceinline__ double adj_particle(double x, double y, double z, con
         }
     }
     // x1 = x0 + u1 / gamma(u1) * dt / d
-    const double ig = 1.0 / gam, ig3 = ig * ig * ig;
-    double lu1[3] = {lu[0], lu[1], lu[2]};
+    const T ig = T(1) / gam, ig3 = ig * ig * ig;
+    T lu1[3] = {lu[0], lu[1], lu[2]};
 #pragma unroll
     for (int b = 0; b < 3; b++) {
-        double s = 0.0;
+        T s = T(0);
 #pragma unroll
-        for (int a = 0; a < 3; a++) s += lx[a] * dtd[a] * ((a == b ? ig : 0.0) - v1[a] * v1[b] * ig3);
+        for (int a = 0; a < 3; a++) s += lx[a] * K.dtd[a] * ((a == b ? ig : T(0)) - v1[a] * v1[b] * ig3);
         lu1[b] += s;
     }
 #pragma unroll
     for (int a = 0; a < 3; a++) lx0[a] += lx[a];
     // Boris transpose
-    double lF[6];
+    T lF[6];
 #pragma unroll
     for (int b = 0; b < 9; b++) {
-        const double s = lu1[0] * u1[0].d[b] + lu1[1] * u1[1].d[b] + lu1[2] * u1[2].d[b];
+        const T s = lu1[0] * u1[0].d[b] + lu1[1] * u1[1].d[b] + lu1[2] * u1[2].d[b];
         if (b < 3) lu[b] = s;
         else lF[b - 3] = s;
     }
@@ -582,20 +599,21 @@ __device__ __forceinline__ double adj_particle(double x, double y, double z, con
     for (int c = 0; c < 6; c++) {
 #pragma unroll
         for (int a = 0; a < 3; a++) lx0[a] += lF[c] * Fg[c][a];
-        const double* WX = SX[c] ? Wd[0] : Wp[0];
-        const double* WY = SY[c] ? Wd[1] : Wp[1];
-        const double* WZ = SZ[c] ? Wd[2] : Wp[2];
-        const int i0 = (SX[c] ? id : ip) - H, j0 = (SY[c] ? jd : jp) - H, k0 = (SZ[c] ? kd : kp) - H;
-        double wzr[M + 1];
+        const T* WX = SX[c] ? Wd[0] : Wp[0];
+        const T* WY = SY[c] ? Wd[1] : Wp[1];
+        const T* WZ = SZ[c] ? Wd[2] : Wp[2];
+        const int i0 = (SX[c] ? idn[0] : ipn[0]) - H, j0 = (SY[c] ? idn[1] : ipn[1]) - H,
+                  k0 = (SZ[c] ? idn[2] : ipn[2]) - H;
+        T wzr[M + 1];
 #pragma unroll
         for (int t = 0; t <= M; t++) wzr[t] = WZ[t];
 #pragma unroll 1
         for (int k = 0; k <= M; k++) {
-            const double wz = wzr[0] * lF[c];
+            const T wz = wzr[0] * lF[c];
             rot_down<M + 1>(wzr);
 #pragma unroll
             for (int j = 0; j <= M; j++) {
-                const double yz = WY[j] * wz;
+                const T yz = WY[j] * wz;
 #pragma unroll
                 for (int i = 0; i <= M; i++) lf(c, k0 + k, j0 + j, i0 + i, WX[i] * yz);
             }
@@ -606,26 +624,45 @@ __device__ __forceinline__ double adj_particle(double x, double y, double z, con
     return gw;
 }
 
+__host__ __device__ inline AdjK<double> adj_consts64(const PushConsts& P) {
+    AdjK<double> K;
+    K.q = P.q; K.qm = P.qm;
+    K.dtd[0] = P.dt / P.dx; K.dtd[1] = P.dt / P.dy; K.dtd[2] = P.dt / P.dz;
+    K.jc[0] = P.q * P.dx / P.dt; K.jc[1] = P.q * P.dy / P.dt; K.jc[2] = P.q * P.dz / P.dt;
+    return K;
+}
+
+// FP64 particle view -> one particle's adjoint (position in global cells)
+template <int M, typename FR, typename LJ, typename LF>
+__device__ __forceinline__ void adj_slot64(const AdjPush& A, int64_t p, double x, double y, double z,
+                                           const AdjK<double>& K, int m, FR fr, LJ lj, LF lf) {
+    const double org = (double)(1 - m);
+    const int base[3] = {0, 0, 0};
+    const double frac[3] = {x - org, y - org, z - org};
+    const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
+    double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
+    double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
+    const double gw = adj_particle<M, double>(base, frac, u0, A.w[p], K, fr, lj, lf, lx, lu);
+    A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
+    A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
+    A.lw[p] += gw;
+}
+
 // separable adjoint push, any particle order (HBM atomics)
 template <int M>
-__global__ __launch_bounds__(256) void k_adj_push64_sep(AdjPush A, Grid g, PushConsts K) {
+__global__ __launch_bounds__(256) void k_adj_push64_sep(AdjPush A, Grid g, PushConsts P) {
+    const AdjK<double> K = adj_consts64(P);
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.n;
          p += (int64_t)gridDim.x * blockDim.x) {
         const double w = A.w[p];
         if (!(w == w)) continue;
         double x = A.x[p], y = A.y[p], z = A.z[p];
-        if (!wrap_check(x, y, z, K)) continue;
-        const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
-        double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
-        double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
-        const double gw = adj_particle<M>(
-            x, y, z, u0, w, K, g.m,
+        if (!wrap_check(x, y, z, P)) continue;
+        adj_slot64<M>(
+            A, p, x, y, z, K, g.m,
             [&](int c, int k, int j, int i) { return A.F[c][g.fi(k, j, i)]; },
             [&](int c, int k, int j, int i) { return A.lJ[c][g.ji(k, j, i)]; },
-            [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); }, lx, lu);
-        A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
-        A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
-        A.lw[p] += gw;
+            [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); });
     }
 }
 
@@ -650,6 +687,7 @@ __global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, Pus
                                                           int64_t* __restrict__ stray_list,
                                                           int64_t* __restrict__ stray_n) {
     using B = AdjBox<M, TX, TY, TZ>;
+    const AdjK<double> KA = adj_consts64(K);
     extern __shared__ double lds_adj[];
     double* lj = lds_adj;                 // 3 * AVOL: lambda_J of the deposit box
     double* lfb = lds_adj + 3 * B::AVOL;  // 6 * FVOL: lambda_F accumulator
@@ -697,11 +735,8 @@ __global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, Pus
             strays |= 1u << it;
             continue;
         }
-        const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
-        double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
-        double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
-        const double gw = adj_particle<M>(
-            x, y, z, u0, w, K, m,
+        adj_slot64<M>(
+            A, p, x, y, z, KA, m,
             [&](int c, int k, int j, int i) {
                 return fld[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)];
             },
@@ -710,11 +745,7 @@ __global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, Pus
             },
             [&](int c, int k, int j, int i, double v) {
                 lds_add(&lfb[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)], v);
-            },
-            lx, lu);
-        A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
-        A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
-        A.lw[p] += gw;
+            });
     }
     // particles that left their tile: appended to the list the HBM kernel
     // (k_adj_push64_list) processes after this one
@@ -742,23 +773,18 @@ template <int M>
 __global__ __launch_bounds__(256) void k_adj_push64_list(AdjPush A, Grid g, PushConsts K,
                                                          const int64_t* __restrict__ list,
                                                          const int64_t* __restrict__ n) {
+    const AdjK<double> KA = adj_consts64(K);
     const int64_t cnt = *n;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cnt;
          q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = list[q];
         double x = A.x[p], y = A.y[p], z = A.z[p];
         wrap_check(x, y, z, K);
-        const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
-        double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
-        double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
-        const double gw = adj_particle<M>(
-            x, y, z, u0, A.w[p], K, g.m,
+        adj_slot64<M>(
+            A, p, x, y, z, KA, g.m,
             [&](int c, int k, int j, int i) { return A.F[c][g.fi(k, j, i)]; },
             [&](int c, int k, int j, int i) { return A.lJ[c][g.ji(k, j, i)]; },
-            [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); }, lx, lu);
-        A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
-        A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
-        A.lw[p] += gw;
+            [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); });
     }
 }
 
