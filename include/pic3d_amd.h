@@ -24,6 +24,7 @@
 #ifndef PIC3D_AMD_H
 #define PIC3D_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -142,6 +143,8 @@ typedef struct pic_ctx pic_ctx;
  * device 'device' and zeroes them. */
 int pic_ctx_create(const pic_params* p, int device, pic_ctx** out);
 int pic_ctx_destroy(pic_ctx* ctx);
+/* The parameter set the context was created with. */
+int pic_ctx_params(const pic_ctx* ctx, pic_params* out);
 const char* pic_last_error(const pic_ctx* ctx);
 int pic_sync(pic_ctx* ctx);
 
@@ -157,7 +160,10 @@ enum pic_option {
                                   accumulators in LDS (smaller tiles, slower) */
     PIC_OPT_ADJ_PUSH = 6       /* differentiated runs: 0 (default) the LDS-tiled
                                   adjoint push (HBM kernel after a direct forward);
-                                  1 dual-number kernel; 2 separable HBM kernel */
+                                  1 dual-number kernel (FP64; FP32 takes 2);
+                                  2 separable HBM kernel; 3 (FP32) the
+                                  LDS-tiled kernel with fp32 lambda_F
+                                  accumulators (smaller LDS footprint) */
 };
 int pic_set_option(pic_ctx* ctx, int option, int64_t value);
 
@@ -258,10 +264,14 @@ int pic_deposit(pic_ctx* ctx, int s, int64_t n, const double* x_old, const doubl
  *     generate_laser_pulse(t_n, 0) returns, boundary.cpp:8-27; the source
  *     is Gpulse with e_n in its place), and
  *   - the initial weight of every particle slot (the density profile).
- * FP64 only, one mobile species (others immobile).  pic_diff_begin sorts
- * the species by tile once; the run never re-sorts, so particle slots stay
- * fixed (particles that leave their tile take the tiled push's HBM path;
- * particles leaving through open faces are marked dead in place).  The forward keeps a full-state
+ * One mobile species (others immobile), in the context's precision: an
+ * FP32 context runs the FP32 forward and FP32 adjoint kernels (fields,
+ * momenta, offsets and their adjoints in fp32; the objective, the envelope
+ * gradient and the weight gradients accumulated in FP64).  pic_diff_begin
+ * sorts the species by tile once; the run never re-sorts, so particle
+ * slots stay fixed (particles that leave their tile take the tiled push's
+ * HBM path; particles leaving through open faces are marked dead in
+ * place).  The forward keeps a full-state
  * checkpoint every K steps; the backward recomputes each K-step segment
  * from its checkpoint, saving the per-step push inputs, and runs the
  * adjoint kernels of that segment in reverse.  Memory: one checkpoint per
@@ -301,6 +311,31 @@ int64_t pic_field_size(const pic_params* p, int comp);
 int pic_diag(pic_ctx* ctx, pic_diag_out* out);
 /* Cumulative boundary counts of species s since the last pic_particles_set. */
 int pic_particle_counts(pic_ctx* ctx, int s, pic_counts* out);
+
+/* ---- output staging (io_utils.cpp:11-347 write these; libpic3d_io holds
+ * the HDF5 side, include/pic3d_io.h) ---- */
+
+/* Page-locked host memory (hipHostMalloc) for staging buffers that the
+ * device transfers into directly; free with pic_pinned_free. */
+int pic_pinned_alloc(size_t bytes, void** host);
+int pic_pinned_free(void* host);
+/* Physical cells of component comp as doubles, [nz][ny][nx] -- the buffer
+ * io_utils.cpp:84-97 (fields) and 108-119 (charge densities) fill before
+ * each H5Dwrite.  Packed on the device (FP32 widened), then one transfer
+ * into 'host' (direct DMA when host is pinned).  Synchronises. */
+int pic_field_interior(pic_ctx* ctx, int comp, double* host);
+/* Particle slots of species s (live and removed-but-not-compacted). */
+int pic_particle_slots(pic_ctx* ctx, int s, int64_t* slots);
+/* The live particles among slots [first_slot, first_slot + nslots) as the
+ * reference's 144-byte particle records (include/particle.h:9-29, the
+ * records io_utils.cpp:218-236 writes), in slot order, into 'records'
+ * (room for nslots); *nrec = records written.  id is the low 32 bits of
+ * the id; x..z in global cell units; x_old..z_old as kept with
+ * PIC_OPT_KEEP_OLD (FP64), else the position; px..pz = u; sx..sz = 0 (the
+ * spin is not carried); gamma = sqrt(1 + u.u) (particle.cpp:309); charge
+ * and mass of the species; weight; rank 0.  Synchronises. */
+int pic_particle_records(pic_ctx* ctx, int s, int64_t first_slot, int64_t nslots, void* records,
+                         int64_t* nrec);
 
 /* ---- timing helpers for measurement (HIP events on the context stream) ---- */
 int pic_timer_start(pic_ctx* ctx);

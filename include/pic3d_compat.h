@@ -143,8 +143,10 @@ void free_dtensor3(double*** m, long nzl, long nyl, long nxl);
 void free_dtensor4(double**** m, long n4l, long n3l, long n2l, long n1l);
 void free_memory();
 
-/* ---- called by run_main_time_loop; the caller's io_utils.cpp defines them
- * (include/io_utils.h) ---- */
+/* ---- called by run_main_time_loop (include/io_utils.h).  Defined by the
+ * caller's io_utils.cpp, or by libpic3d_compat_io.so: the reference's files
+ * written with serial HDF5 through libpic3d_io (fields staged from the
+ * device, particles from the vectors) ---- */
 void output_data_hdf5(int step, int my_rank, int num_procs, MPI_Comm my_comm, hid_t plist_id);
 void output_particles_hdf5(int step, int my_rank, int num_procs, MPI_Comm my_comm,
                            const std::vector<std::vector<particle>>& particles,

@@ -137,6 +137,12 @@ SIGNATURES = {
     "pic_timing_get": (C.c_int, [C.c_void_p, _dp, _ip, C.c_int]),
     "pic_timer_start": (C.c_int, [C.c_void_p]),
     "pic_timer_stop": (C.c_int, [C.c_void_p, _dp]),
+    "pic_ctx_params": (C.c_int, [C.c_void_p, C.POINTER(PicParams)]),
+    "pic_pinned_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "pic_pinned_free": (C.c_int, [C.c_void_p]),
+    "pic_field_interior": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "pic_particle_slots": (C.c_int, [C.c_void_p, C.c_int, _ip]),
+    "pic_particle_records": (C.c_int, [C.c_void_p, C.c_int, C.c_int64, C.c_int64, C.c_void_p, _ip]),
 }
 
 

@@ -29,7 +29,9 @@ COMMON = [
     f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
 ]
 # per-file floating-point contraction
-CONTRACT = {"k_push32.hip": "-ffp-contract=fast"}
+# (the adjoint kernels are checked against finite differences, not bit for
+# bit, so they contract too)
+CONTRACT = {"k_push32.hip": "-ffp-contract=fast", "k_adj.hip": "-ffp-contract=fast"}
 DEFAULT_CONTRACT = "-ffp-contract=off"
 
 
@@ -73,6 +75,7 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
     build_compat(verbose)
+    build_io(verbose)
     return OUT
 
 
@@ -103,6 +106,55 @@ def build_compat(verbose: bool = False) -> str:
           f"-L{HERE}", "-lpic3d_compat", "-lpic3d_amd",
           "-Wl,-rpath,$ORIGIN/../../../pic3d_amd", "-o", DRIVER_OUT], verbose)
     return COMPAT_OUT
+
+
+IO_SRC = os.path.join(HERE, "io", "pic3d_io.cpp")
+IO_OUT = os.path.join(HERE, "libpic3d_io.so")
+COMPAT_IO_SRC = os.path.join(HERE, "io", "pic3d_compat_io.cpp")
+COMPAT_IO_OUT = os.path.join(HERE, "libpic3d_compat_io.so")
+DRIVER_H5_OUT = os.path.join(ROOT, "tests", "compat", "bin", "compat_main_h5")
+HDF5_PREFIX = "/opt/conda"  # the image's serial HDF5 1.10 (include/ and lib/)
+H5_LINKS = os.path.join(HERE, "_hdf5")
+
+
+def have_hdf5() -> bool:
+    return os.path.exists(os.path.join(HDF5_PREFIX, "include", "hdf5.h")) and \
+        os.path.exists(os.path.join(HDF5_PREFIX, "lib", "libhdf5.so"))
+
+
+def build_io(verbose: bool = False):
+    """libpic3d_io.so: HDF5 output of a context in the reference's format
+    (C ABI, include/pic3d_io.h); libpic3d_compat_io.so: the reference's
+    io_utils.h functions for programs on the C++ drop-in; and the compat
+    test driver that writes through them.  Needs the image's serial HDF5
+    (an optional dependency, as in the reference): skipped without it."""
+    if not have_hdf5():
+        print("pic3d_amd.build: no HDF5 under %s; the output library is not built" % HDF5_PREFIX)
+        return None
+    # The HDF5 library is reached through links in pic3d_amd/_hdf5 (the
+    # libraries' run path), not through HDF5_PREFIX/lib itself, which also
+    # holds an older libstdc++ that must not shadow the system one.
+    os.makedirs(H5_LINKS, exist_ok=True)
+    real = os.path.realpath(os.path.join(HDF5_PREFIX, "lib", "libhdf5.so"))
+    for name in ("libhdf5.so", os.path.basename(real).split(".so")[0] + ".so." +
+                 os.path.basename(real).split(".so.")[1].split(".")[0]):
+        link = os.path.join(H5_LINKS, name)
+        if os.path.islink(link) or os.path.exists(link):
+            os.remove(link)
+        os.symlink(real, link)
+    inc = [f"-I{os.path.join(ROOT, 'include')}", f"-I{HDF5_PREFIX}/include"]
+    h5 = [f"-L{H5_LINKS}", "-lhdf5"]
+    rp = "-Wl,-rpath,$ORIGIN:$ORIGIN/_hdf5"
+    _gxx(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-shared", "-pthread", *inc, IO_SRC,
+          f"-L{HERE}", "-lpic3d_amd", *h5, rp, "-o", IO_OUT], verbose)
+    cinc = [*inc, f"-I{MPI_INC}", "-DMPICH_SKIP_MPICXX"]
+    _gxx(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-shared", *cinc, COMPAT_IO_SRC,
+          f"-L{HERE}", "-lpic3d_io", "-lpic3d_compat", "-lpic3d_amd", rp, "-o", COMPAT_IO_OUT], verbose)
+    _gxx(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-DCOMPAT_MAIN_HDF5", *cinc,
+          DRIVER_SRC, f"-L{HERE}", "-lpic3d_compat_io", "-lpic3d_io", "-lpic3d_compat", "-lpic3d_amd",
+          f"-Wl,-rpath-link,{H5_LINKS}", "-Wl,-rpath,$ORIGIN/../../../pic3d_amd", "-o", DRIVER_H5_OUT],
+         verbose)
+    return IO_OUT
 
 
 if __name__ == "__main__":

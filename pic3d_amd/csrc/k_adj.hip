@@ -498,7 +498,9 @@ __device__ __forceinline__ T deposit_adj(const int* base, const T* f0, const T* 
 // fr(c, k, j, i): field c at a buffer node; lj(c, k, j, i): lambda_J;
 // lf(c, k, j, i, v): adds v to lambda_F.  lx, lu: in = adjoint after the
 // push, out = before it.  Returns this step's d objective / d w.
-template <int M, typename T, typename FR, typename LJ, typename LF>
+// SKIP_FAST: a move of a cell or more leaves the particle untouched (the
+// FP32 push) instead of moving it without a deposit (the FP64 push).
+template <int M, typename T, bool SKIP_FAST, typename FR, typename LJ, typename LF>
 __device__ __forceinline__ T adj_particle(const int* base, const T* frac, const T* u0v, T w,
                                           const AdjK<T>& K, FR fr, LJ lj, LF lf, T* lx, T* lu) {
     constexpr int H = M / 2;
@@ -565,7 +567,9 @@ __device__ __forceinline__ T adj_particle(const int* base, const T* frac, const 
     for (int a = 0; a < 3; a++) f1[a] = frac[a] + v1[a] / gam * K.dtd[a];
     T lx0[3] = {T(0), T(0), T(0)};
     T gw = T(0);
-    if (fabs(f1[0] - frac[0]) < T(1) && fabs(f1[1] - frac[1]) < T(1) && fabs(f1[2] - frac[2]) < T(1)) {
+    const bool moved = fabs(f1[0] - frac[0]) < T(1) && fabs(f1[1] - frac[1]) < T(1) && fabs(f1[2] - frac[2]) < T(1);
+    if (SKIP_FAST && !moved) return T(0);
+    if (moved) {
         T g[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
         gw = deposit_adj<M, T>(base, frac, f1, K, lj, g);
 #pragma unroll
@@ -632,66 +636,137 @@ __host__ __device__ inline AdjK<double> adj_consts64(const PushConsts& P) {
     return K;
 }
 
-// FP64 particle view -> one particle's adjoint (position in global cells)
-template <int M, typename FR, typename LJ, typename LF>
-__device__ __forceinline__ void adj_slot64(const AdjPush& A, int64_t p, double x, double y, double z,
-                                           const AdjK<double>& K, int m, FR fr, LJ lj, LF lf) {
-    const double org = (double)(1 - m);
-    const int base[3] = {0, 0, 0};
-    const double frac[3] = {x - org, y - org, z - org};
-    const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
-    double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
-    double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
-    const double gw = adj_particle<M, double>(base, frac, u0, A.w[p], K, fr, lj, lf, lx, lu);
-    A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
-    A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
-    A.lw[p] += gw;
-}
+// FP32 state at the start of the push (slot order): linear cell (-1 =
+// removed) + in-cell offsets, fp32 momenta and weights; fp32 fields and
+// adjoints, the weight adjoint accumulated in FP64.
+struct AdjPush32 {
+    const int32_t* cell;
+    const float *fx, *fy, *fz, *ux, *uy, *uz, *w;
+    const float* F[6];
+    const float* lJ[3];
+    float* lF[6];
+    float *lx, *ly, *lz, *lux, *luy, *luz;  // lx..lz: adjoint of the offsets (cell units)
+    double* lw;
+    int64_t n;
+};
 
-// separable adjoint push, any particle order (HBM atomics)
-template <int M>
-__global__ __launch_bounds__(256) void k_adj_push64_sep(AdjPush A, Grid g, PushConsts P) {
-    const AdjK<double> K = adj_consts64(P);
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.n;
-         p += (int64_t)gridDim.x * blockDim.x) {
+// Particle views of the adjoint push, one per precision.  locate(p):
+// whether slot p is pushed this step, its cell per axis and its position
+// as a buffer node plus a fraction; adjoint<M>(p, ...): that particle's
+// adjoint through adj_particle, written back to the slot.
+struct AdjView64 {
+    using T = double;
+    AdjPush A;
+    PushConsts P;
+    AdjK<double> K;
+    int m;
+    __device__ __forceinline__ bool locate(int64_t p, int* cell, int* base, double* frac) const {
         const double w = A.w[p];
-        if (!(w == w)) continue;
+        if (!(w == w)) return false;  // dead slot
         double x = A.x[p], y = A.y[p], z = A.z[p];
-        if (!wrap_check(x, y, z, P)) continue;
-        adj_slot64<M>(
-            A, p, x, y, z, K, g.m,
-            [&](int c, int k, int j, int i) { return A.F[c][g.fi(k, j, i)]; },
-            [&](int c, int k, int j, int i) { return A.lJ[c][g.ji(k, j, i)]; },
-            [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); });
+        if (!wrap_check(x, y, z, P)) return false;  // not pushed: the adjoint passes through
+        const double org = (double)(1 - m);
+        const double v[3] = {x, y, z};
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            cell[a] = min((int)v[a], (int)P.N[a] - 1);
+            base[a] = 0;
+            frac[a] = v[a] - org;
+        }
+        return true;
+    }
+    template <int M, typename FR, typename LJ, typename LF>
+    __device__ __forceinline__ void adjoint(int64_t p, const int* base, const double* frac, FR fr, LJ lj,
+                                            LF lf) const {
+        const double u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
+        double lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
+        double lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
+        const double gw = adj_particle<M, double, false>(base, frac, u0, A.w[p], K, fr, lj, lf, lx, lu);
+        A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
+        A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
+        A.lw[p] += gw;
+    }
+};
+
+struct AdjView32 {
+    using T = float;
+    AdjPush32 A;
+    AdjK<float> K;
+    int n[3];
+    int m;
+    __device__ __forceinline__ bool locate(int64_t p, int* cell, int* base, float* frac) const {
+        const int c = A.cell[p];
+        if (c < 0) return false;  // removed at an open face
+        cell[0] = c % n[0];
+        cell[1] = (c / n[0]) % n[1];
+        cell[2] = c / (n[0] * n[1]);
+#pragma unroll
+        for (int a = 0; a < 3; a++) base[a] = cell[a] + m - 1;
+        frac[0] = A.fx[p]; frac[1] = A.fy[p]; frac[2] = A.fz[p];
+        return true;
+    }
+    template <int M, typename FR, typename LJ, typename LF>
+    __device__ __forceinline__ void adjoint(int64_t p, const int* base, const float* frac, FR fr, LJ lj,
+                                            LF lf) const {
+        const float u0[3] = {A.ux[p], A.uy[p], A.uz[p]};
+        float lx[3] = {A.lx[p], A.ly[p], A.lz[p]};
+        float lu[3] = {A.lux[p], A.luy[p], A.luz[p]};
+        const float gw = adj_particle<M, float, true>(base, frac, u0, A.w[p], K, fr, lj, lf, lx, lu);
+        A.lx[p] = lx[0]; A.ly[p] = lx[1]; A.lz[p] = lx[2];
+        A.lux[p] = lu[0]; A.luy[p] = lu[1]; A.luz[p] = lu[2];
+        A.lw[p] += (double)gw;
+    }
+};
+
+// separable adjoint push through HBM (gathers from the field arrays,
+// global atomics into lambda_F): every slot (list = null, any order), or
+// the slots of a list (the tiled kernel's strays)
+template <int M, typename V>
+__global__ __launch_bounds__(256) void k_adj_push_hbm(V v, Grid g, const int64_t* __restrict__ list,
+                                                      const int64_t* __restrict__ nlist) {
+    using T = typename V::T;
+    const int64_t cnt = list ? *nlist : v.A.n;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cnt;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = list ? list[q] : q;
+        int cell[3], base[3];
+        T frac[3];
+        if (!v.locate(p, cell, base, frac)) continue;
+        v.template adjoint<M>(
+            p, base, frac, [&](int c, int k, int j, int i) { return v.A.F[c][g.fi(k, j, i)]; },
+            [&](int c, int k, int j, int i) { return v.A.lJ[c][g.ji(k, j, i)]; },
+            [&](int c, int k, int j, int i, T x) { hbm_add(&v.A.lF[c][g.fi(k, j, i)], x); });
     }
 }
 
 // LDS-tiled adjoint push: one workgroup per (tile, chunk) work item of the
-// frozen tile grouping.  lambda_J of the tile's deposit box is staged in
-// LDS; lambda_F of the tile's gather box accumulates in LDS (ds_add_f64)
-// and is flushed with one global atomic per non-zero point.  Particles
-// that left their tile take the HBM path.
-template <int M, int TX, int TY, int TZ>
+// frozen tile grouping.  lambda_J of the tile's deposit box and the
+// fields of its gather box are staged in LDS; lambda_F of the gather box
+// accumulates in LDS (LACC: double, ds_add_f64; float, ds_add_f32) and is
+// flushed with one global atomic per non-zero point.  Particles that left
+// their tile take the HBM path (k_adj_push_hbm over the stray list).
+template <int M, int TX, int TY, int TZ, typename T, typename LACC>
 struct AdjBox {
     static constexpr int H = M / 2;
     static constexpr int AX = TX + M + 3, AY = TY + M + 3, AZ = TZ + M + 3, AVOL = AX * AY * AZ;
     static constexpr int FX = TX + M + 1, FY = TY + M + 1, FZ = TZ + M + 1, FVOL = FX * FY * FZ;
-    static constexpr size_t LDS = (size_t)(3 * AVOL + 12 * FVOL) * sizeof(double);
+    // accumulators first (8-byte aligned), then lambda_J and the fields
+    static constexpr size_t LDS = (size_t)6 * FVOL * sizeof(LACC) + (size_t)(3 * AVOL + 6 * FVOL) * sizeof(T);
 };
 
-template <int M, int TX, int TY, int TZ>
-__global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, PushConsts K, Tiling tl,
-                                                          const int64_t* __restrict__ tile_off,
-                                                          const int64_t* __restrict__ work,
-                                                          const int64_t* __restrict__ nwork, int chunk,
-                                                          int64_t* __restrict__ stray_list,
-                                                          int64_t* __restrict__ stray_n) {
-    using B = AdjBox<M, TX, TY, TZ>;
-    const AdjK<double> KA = adj_consts64(K);
+template <int M, int TX, int TY, int TZ, typename V, typename LACC>
+__global__ __launch_bounds__(256) void k_adj_push_tiled(V v, Grid g, Tiling tl,
+                                                        const int64_t* __restrict__ tile_off,
+                                                        const int64_t* __restrict__ work,
+                                                        const int64_t* __restrict__ nwork, int chunk,
+                                                        int64_t* __restrict__ stray_list,
+                                                        int64_t* __restrict__ stray_n) {
+    using T = typename V::T;
+    using B = AdjBox<M, TX, TY, TZ, T, LACC>;
     extern __shared__ double lds_adj[];
-    double* lj = lds_adj;                 // 3 * AVOL: lambda_J of the deposit box
-    double* lfb = lds_adj + 3 * B::AVOL;  // 6 * FVOL: lambda_F accumulator
-    double* fld = lfb + 6 * B::FVOL;      // 6 * FVOL: the fields of the gather box
+    LACC* lfb = reinterpret_cast<LACC*>(lds_adj);    // 6 * FVOL: lambda_F accumulator
+    T* lj = reinterpret_cast<T*>(lfb + 6 * B::FVOL);  // 3 * AVOL: lambda_J of the deposit box
+    T* fld = lj + 3 * B::AVOL;                        // 6 * FVOL: the fields of the gather box
     const int64_t wi = blockIdx.x;
     if (wi >= *nwork) return;
     const int64_t item = work[wi];
@@ -710,14 +785,14 @@ __global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, Pus
         const int lx = r % B::AX, ly = (r / B::AX) % B::AY, lz = r / (B::AX * B::AY);
         // the box of a partial last tile reaches past the array: clamp (unused)
         const int gx = min(aox + lx, g.je[0] - 2), gy = min(aoy + ly, g.je[1] - 2), gz = min(aoz + lz, g.je[2] - 2);
-        lj[e] = A.lJ[c][g.ji(gz, gy, gx)];
+        lj[e] = v.A.lJ[c][g.ji(gz, gy, gx)];
     }
     for (int e = threadIdx.x; e < 6 * B::FVOL; e += blockDim.x) {
-        lfb[e] = 0.0;
+        lfb[e] = LACC(0);
         const int c = e / B::FVOL, r = e % B::FVOL;
         const int lx = r % B::FX, ly = (r / B::FX) % B::FY, lz = r / (B::FX * B::FY);
         const int gx = min(fox + lx, g.fe[0] - 1), gy = min(foy + ly, g.fe[1] - 1), gz = min(foz + lz, g.fe[2] - 1);
-        fld[e] = A.F[c][g.fi(gz, gy, gx)];
+        fld[e] = v.A.F[c][g.fi(gz, gy, gx)];
     }
     __syncthreads();
     uint32_t strays = 0;
@@ -725,30 +800,28 @@ __global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, Pus
     const int len = (int)(end - beg);
     for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
         const int64_t p = beg + visit_slot(t, len);
-        const double w = A.w[p];
-        if (!(w == w)) continue;
-        double x = A.x[p], y = A.y[p], z = A.z[p];
-        if (!wrap_check(x, y, z, K)) continue;
-        const int cx = min((int)x, g.n[0] - 1), cy = min((int)y, g.n[1] - 1), cz = min((int)z, g.n[2] - 1);
-        if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
-            (unsigned)(cz - T0z) >= (unsigned)TZ) {
+        int cell[3], base[3];
+        T frac[3];
+        if (!v.locate(p, cell, base, frac)) continue;
+        if ((unsigned)(cell[0] - T0x) >= (unsigned)TX || (unsigned)(cell[1] - T0y) >= (unsigned)TY ||
+            (unsigned)(cell[2] - T0z) >= (unsigned)TZ) {
             strays |= 1u << it;
             continue;
         }
-        adj_slot64<M>(
-            A, p, x, y, z, KA, m,
+        v.template adjoint<M>(
+            p, base, frac,
             [&](int c, int k, int j, int i) {
                 return fld[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)];
             },
             [&](int c, int k, int j, int i) {
                 return lj[c * B::AVOL + ((k - aoz) * B::AY + (j - aoy)) * B::AX + (i - aox)];
             },
-            [&](int c, int k, int j, int i, double v) {
-                lds_add(&lfb[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)], v);
+            [&](int c, int k, int j, int i, T x) {
+                lds_add(&lfb[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)], (LACC)x);
             });
     }
     // particles that left their tile: appended to the list the HBM kernel
-    // (k_adj_push64_list) processes after this one
+    // processes after this one
     if (strays) {
         it = 0;
         for (int t = threadIdx.x; t < len; t += blockDim.x, it++)
@@ -759,32 +832,12 @@ __global__ __launch_bounds__(256) void k_adj_push64_tiled(AdjPush A, Grid g, Pus
     }
     __syncthreads();
     for (int e = threadIdx.x; e < 6 * B::FVOL; e += blockDim.x) {
-        const double v = lfb[e];
-        if (v == 0.0) continue;
+        const LACC x = lfb[e];
+        if (x == LACC(0)) continue;
         const int c = e / B::FVOL, r = e % B::FVOL;
         const int gx = fox + r % B::FX, gy = foy + (r / B::FX) % B::FY, gz = foz + r / (B::FX * B::FY);
         if (gx >= g.fe[0] || gy >= g.fe[1] || gz >= g.fe[2]) continue;
-        hbm_add(&A.lF[c][g.fi(gz, gy, gx)], v);
-    }
-}
-
-// the tiled kernel's strays, through HBM
-template <int M>
-__global__ __launch_bounds__(256) void k_adj_push64_list(AdjPush A, Grid g, PushConsts K,
-                                                         const int64_t* __restrict__ list,
-                                                         const int64_t* __restrict__ n) {
-    const AdjK<double> KA = adj_consts64(K);
-    const int64_t cnt = *n;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < cnt;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = list[q];
-        double x = A.x[p], y = A.y[p], z = A.z[p];
-        wrap_check(x, y, z, K);
-        adj_slot64<M>(
-            A, p, x, y, z, KA, g.m,
-            [&](int c, int k, int j, int i) { return A.F[c][g.fi(k, j, i)]; },
-            [&](int c, int k, int j, int i) { return A.lJ[c][g.ji(k, j, i)]; },
-            [&](int c, int k, int j, int i, double v) { hbm_add(&A.lF[c][g.fi(k, j, i)], v); });
+        hbm_add(&v.A.lF[c][g.fi(gz, gy, gx)], (T)x);
     }
 }
 
@@ -808,6 +861,25 @@ __global__ __launch_bounds__(256) void k_adj_objective(AdjPush A, double mdv, do
     }
 }
 
+// FP32: the same from the fp32 state (removed slots carry no adjoint)
+__global__ __launch_bounds__(256) void k_adj_objective32(AdjPush32 A, double mdv, double gout) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < A.n;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        double lu[3] = {0.0, 0.0, 0.0}, lw = 0.0;
+        if (A.cell[p] >= 0) {
+            const double w = A.w[p], ux = A.ux[p], uy = A.uy[p], uz = A.uz[p];
+            const double gam = ::sqrt(1.0 + (ux * ux + uy * uy + uz * uz));
+            lu[0] = gout * mdv * w * ux / gam;
+            lu[1] = gout * mdv * w * uy / gam;
+            lu[2] = gout * mdv * w * uz / gam;
+            lw = gout * mdv * (gam - 1.0);
+        }
+        A.lx[p] = 0.0f; A.ly[p] = 0.0f; A.lz[p] = 0.0f;
+        A.lux[p] = (float)lu[0]; A.luy[p] = (float)lu[1]; A.luz[p] = (float)lu[2];
+        A.lw[p] = lw;
+    }
+}
+
 // ---------------------------------------------------------------------
 // field adjoints (gathers over the adjoint arrays)
 // ---------------------------------------------------------------------
@@ -816,17 +888,17 @@ __device__ __forceinline__ bool inner(const Grid& g, int k, int j, int i) {
 }
 
 // transpose of E += c (curl B - J) (k_e_half): lB += .., lJ -= c lE
-__global__ __launch_bounds__(256) void k_adj_e(const double* __restrict__ lEx, const double* __restrict__ lEy,
-                                               const double* __restrict__ lEz, double* __restrict__ lBx,
-                                               double* __restrict__ lBy, double* __restrict__ lBz,
-                                               double* __restrict__ lJx, double* __restrict__ lJy,
-                                               double* __restrict__ lJz, Grid g, double c, double dx,
-                                               double dy, double dz) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_e(const T* __restrict__ lEx, const T* __restrict__ lEy,
+                                               const T* __restrict__ lEz, T* __restrict__ lBx,
+                                               T* __restrict__ lBy, T* __restrict__ lBz,
+                                               T* __restrict__ lJx, T* __restrict__ lJy,
+                                               T* __restrict__ lJz, Grid g, T c, T dx, T dy, T dz) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.fsize) return;
     const int i = (int)(t % g.fe[0]), j = (int)((t / g.fe[0]) % g.fe[1]), k = (int)(t / ((int64_t)g.fe[0] * g.fe[1]));
-    auto L = [&](const double* a, int kk, int jj, int ii) -> double {
-        return inner(g, kk, jj, ii) ? a[g.fi(kk, jj, ii)] : 0.0;
+    auto L = [&](const T* a, int kk, int jj, int ii) -> T {
+        return inner(g, kk, jj, ii) ? a[g.fi(kk, jj, ii)] : T(0);
     };
     lBz[t] += c * (L(lEx, k, j, i) / dy - L(lEx, k, j + 1, i) / dy - L(lEy, k, j, i) / dx + L(lEy, k, j, i + 1) / dx);
     lBy[t] += c * (-L(lEx, k, j, i) / dz + L(lEx, k + 1, j, i) / dz + L(lEz, k, j, i) / dx - L(lEz, k, j, i + 1) / dx);
@@ -840,15 +912,16 @@ __global__ __launch_bounds__(256) void k_adj_e(const double* __restrict__ lEx, c
 }
 
 // transpose of B -= c curl E (k_b_half): lE += ..
-__global__ __launch_bounds__(256) void k_adj_b(double* __restrict__ lEx, double* __restrict__ lEy,
-                                               double* __restrict__ lEz, const double* __restrict__ lBx,
-                                               const double* __restrict__ lBy, const double* __restrict__ lBz,
-                                               Grid g, double c, double dx, double dy, double dz) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_b(T* __restrict__ lEx, T* __restrict__ lEy,
+                                               T* __restrict__ lEz, const T* __restrict__ lBx,
+                                               const T* __restrict__ lBy, const T* __restrict__ lBz,
+                                               Grid g, T c, T dx, T dy, T dz) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= g.fsize) return;
     const int i = (int)(t % g.fe[0]), j = (int)((t / g.fe[0]) % g.fe[1]), k = (int)(t / ((int64_t)g.fe[0] * g.fe[1]));
-    auto L = [&](const double* a, int kk, int jj, int ii) -> double {
-        return inner(g, kk, jj, ii) ? a[g.fi(kk, jj, ii)] : 0.0;
+    auto L = [&](const T* a, int kk, int jj, int ii) -> T {
+        return inner(g, kk, jj, ii) ? a[g.fi(kk, jj, ii)] : T(0);
     };
     lEx[t] += c * (-L(lBy, k - 1, j, i) / dz + L(lBy, k, j, i) / dz + L(lBz, k, j - 1, i) / dy - L(lBz, k, j, i) / dy);
     lEy[t] += c * (L(lBx, k - 1, j, i) / dz - L(lBx, k, j, i) / dz - L(lBz, k, j, i - 1) / dx + L(lBz, k, j, i) / dx);
@@ -857,8 +930,9 @@ __global__ __launch_bounds__(256) void k_adj_b(double* __restrict__ lEx, double*
 
 // transpose of the halo wrap of three components along one axis
 // (k_wrap): the ghost's adjoint moves onto its source
-__global__ __launch_bounds__(256) void k_adj_wrap(double* __restrict__ a0, double* __restrict__ a1,
-                                                  double* __restrict__ a2, Grid g, int axis) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_wrap(T* __restrict__ a0, T* __restrict__ a1,
+                                                  T* __restrict__ a2, Grid g, int axis) {
     const int m = g.m, N = g.n[axis];
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int e0, e1;
@@ -875,16 +949,17 @@ __global__ __launch_bounds__(256) void k_adj_wrap(double* __restrict__ a0, doubl
     if (axis == 0) { od = g.fi(v, u, dst); os = g.fi(v, u, src); }
     else if (axis == 1) { od = g.fi(v, dst, u); os = g.fi(v, src, u); }
     else { od = g.fi(dst, v, u); os = g.fi(src, v, u); }
-    double* A[3] = {a0, a1, a2};
+    T* A[3] = {a0, a1, a2};
     for (int q = 0; q < 3; q++) {
         A[q][os] += A[q][od];
-        A[q][od] = 0.0;
+        A[q][od] = T(0);
     }
 }
 
 // transpose of the current fold (k_fold): phase 1 = sync, phase 0 = accumulate
-__global__ __launch_bounds__(256) void k_adj_fold(double* __restrict__ a0, double* __restrict__ a1,
-                                                  double* __restrict__ a2, Grid g, int axis, int phase) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_fold(T* __restrict__ a0, T* __restrict__ a1,
+                                                  T* __restrict__ a2, Grid g, int axis, int phase) {
     const int m = g.m, N = g.n[axis];
     const int st = m, end = m + N - 1;
     int e0, e1;
@@ -903,14 +978,14 @@ __global__ __launch_bounds__(256) void k_adj_fold(double* __restrict__ a0, doubl
     };
     const int64_t i_lo_int = idx(st + l), i_hi_gh = idx(end + 1 + l);
     const int64_t i_hi_int = idx(end - m + l), i_lo_gh = idx(-1 + l);
-    double* A[3] = {a0, a1, a2};
+    T* A[3] = {a0, a1, a2};
     for (int q = 0; q < 3; q++) {
-        double* D = A[q];
+        T* D = A[q];
         if (phase == 1) {
             D[i_hi_int] += D[i_lo_gh];
-            D[i_lo_gh] = 0.0;
+            D[i_lo_gh] = T(0);
             D[i_lo_int] += D[i_hi_gh];
-            D[i_hi_gh] = 0.0;
+            D[i_hi_gh] = T(0);
         } else {
             D[i_hi_gh] += D[i_lo_int];
             D[i_lo_gh] += D[i_hi_int];
@@ -921,10 +996,11 @@ __global__ __launch_bounds__(256) void k_adj_fold(double* __restrict__ a0, doubl
 // transpose of the laser / Silver-Mueller writes (k_laser) and the
 // derivative of the objective with respect to this step's x-min envelope
 // value; g_row(y, z) = Gpulse(t, 0, y, z) / envelope.
-__global__ __launch_bounds__(256) void k_adj_laser(double* __restrict__ lEy, double* __restrict__ lEz,
-                                                   double* __restrict__ lBx, double* __restrict__ lBy,
-                                                   double* __restrict__ lBz, double* __restrict__ lJy,
-                                                   double* __restrict__ lJz, Grid g, LaserArgs L,
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_laser(T* __restrict__ lEy, T* __restrict__ lEz,
+                                                   T* __restrict__ lBx, T* __restrict__ lBy,
+                                                   T* __restrict__ lBz, T* __restrict__ lJy,
+                                                   T* __restrict__ lJz, Grid g, LaserArgs L,
                                                    double* __restrict__ genv) {
     const int j = g.m + blockIdx.x * blockDim.x + threadIdx.x;
     const int k = g.m + blockIdx.y;
@@ -941,17 +1017,17 @@ __global__ __launch_bounds__(256) void k_adj_laser(double* __restrict__ lEy, dou
         {
             const int64_t o = g.fi(k, j, m), gl = g.fi(k, j, m - 1);
             const double a = lBz[gl], b = lBy[gl];
-            lBz[gl] = 0.0;
-            lBy[gl] = 0.0;
-            hbm_add(&lEy[o], -2.0 * alpha * a);
-            hbm_add(&lBx[o], -alpha * dtdz * a + alpha * dtdy * b);
-            hbm_add(&lBx[g.fi(k - 1, j, m)], alpha * dtdz * a);
-            hbm_add(&lBx[g.fi(k, j - 1, m)], -alpha * dtdy * b);
-            hbm_add(&lBz[o], -alpha * beta * a);
-            hbm_add(&lBy[o], -alpha * beta * b);
-            hbm_add(&lEz[o], 2.0 * alpha * b);
-            hbm_add(&lJy[g.ji(k, j, m)], -alpha * dt * a);
-            hbm_add(&lJz[g.ji(k, j, m)], alpha * dt * b);
+            lBz[gl] = T(0);
+            lBy[gl] = T(0);
+            hbm_add(&lEy[o], (T)(-2.0 * alpha * a));
+            hbm_add(&lBx[o], (T)(-alpha * dtdz * a + alpha * dtdy * b));
+            hbm_add(&lBx[g.fi(k - 1, j, m)], (T)(alpha * dtdz * a));
+            hbm_add(&lBx[g.fi(k, j - 1, m)], (T)(-alpha * dtdy * b));
+            hbm_add(&lBz[o], (T)(-alpha * beta * a));
+            hbm_add(&lBy[o], (T)(-alpha * beta * b));
+            hbm_add(&lEz[o], (T)(2.0 * alpha * b));
+            hbm_add(&lJy[g.ji(k, j, m)], (T)(-alpha * dt * a));
+            hbm_add(&lJz[g.ji(k, j, m)], (T)(alpha * dt * b));
             if (L.left) {
                 // S_l = P_l = envelope * g_row for the enabled polarisations
                 LaserArgs U = L;
@@ -967,17 +1043,17 @@ __global__ __launch_bounds__(256) void k_adj_laser(double* __restrict__ lEy, dou
         {
             const int64_t o = g.fi(k, j, N), gr = g.fi(k, j, N + 1);
             const double a = lBz[gr], b = lBy[gr];
-            lBz[gr] = 0.0;
-            lBy[gr] = 0.0;
-            hbm_add(&lEy[o], 2.0 * alpha * a);
-            hbm_add(&lBx[o], -alpha * dtdz * a + alpha * dtdy * b);
-            hbm_add(&lBx[g.fi(k - 1, j, N)], alpha * dtdz * a);
-            hbm_add(&lBx[g.fi(k, j - 1, N)], -alpha * dtdy * b);
-            hbm_add(&lBz[o], -alpha * beta * a);
-            hbm_add(&lBy[o], -alpha * beta * b);
-            hbm_add(&lEz[o], -2.0 * alpha * b);
-            hbm_add(&lJy[g.ji(k, j, N)], alpha * dt * a);
-            hbm_add(&lJz[g.ji(k, j, N)], -alpha * dt * b);
+            lBz[gr] = T(0);
+            lBy[gr] = T(0);
+            hbm_add(&lEy[o], (T)(2.0 * alpha * a));
+            hbm_add(&lBx[o], (T)(-alpha * dtdz * a + alpha * dtdy * b));
+            hbm_add(&lBx[g.fi(k - 1, j, N)], (T)(alpha * dtdz * a));
+            hbm_add(&lBx[g.fi(k, j - 1, N)], (T)(-alpha * dtdy * b));
+            hbm_add(&lBz[o], (T)(-alpha * beta * a));
+            hbm_add(&lBy[o], (T)(-alpha * beta * b));
+            hbm_add(&lEz[o], (T)(-2.0 * alpha * b));
+            hbm_add(&lJy[g.ji(k, j, N)], (T)(alpha * dt * a));
+            hbm_add(&lJz[g.ji(k, j, N)], (T)(-alpha * dt * b));
         }
     }
     // block sum of ge
@@ -1023,17 +1099,63 @@ __global__ __launch_bounds__(256) void k_adj_set_w(double* __restrict__ w, const
         if (w[p] == w[p]) w[p] = win[p];
 }
 
+// FP32 state: sum w (gamma - 1) over live slots, in FP64
+__global__ __launch_bounds__(256) void k_adj_kinetic32(const int32_t* __restrict__ cell, const float* __restrict__ w,
+                                                       const float* __restrict__ ux, const float* __restrict__ uy,
+                                                       const float* __restrict__ uz, int64_t n,
+                                                       double* __restrict__ out) {
+    double s = 0.0;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        if (cell[p] < 0) continue;
+        const double x = ux[p], y = uy[p], z = uz[p];
+        s += (double)w[p] * (::sqrt(1.0 + (x * x + y * y + z * z)) - 1.0);
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) hbm_add(out, red[0]);
+}
+
+__global__ __launch_bounds__(256) void k_adj_set_w32(const int32_t* __restrict__ cell, float* __restrict__ w,
+                                                     const double* __restrict__ win, int64_t n) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        if (cell[p] >= 0) w[p] = (float)win[p];
+}
+
 // ---------------------------------------------------------------------
 // checkpoint / recompute driver
 // ---------------------------------------------------------------------
+// the particle adjoints of a differentiated run (one precision)
+template <typename T> struct DiffLam {
+    T* l[6];
+    double* w;
+};
+
 struct DiffState {
     int s = -1;
+    int f32 = 0;                     // FP32 context: fp32 state, fp32 adjoints
+    int kind = 0;                    // tile shape kind of the run's grouping
     int64_t slots = 0;               // particle slots of species s (live and dead)
     int first = 1, nsteps = 0, K = 1;
     int forward_done = 0;
     int saved_push_kernel = 0;
     int saved_sort_interval = 1;
-    int kernel = 2;                  // push kernel of the run (1 direct, 2 tiled, never re-sorted)
+    int kernel = 2;                  // push kernel of the run (1 direct, 2 tiled)
+    // Tiled runs sort the species every sort_every steps (before the push)
+    // with a keep-dead sort, which preserves the slot count: the forward
+    // records each sort's permutation and tile offsets, the recompute
+    // replays them (so forward and recompute agree on every slot's
+    // particle), and the adjoint carries the particle adjoints back
+    // through each permutation.
+    int sort_every = 1;
+    int32_t* perms = nullptr;        // one permutation (slots entries) per sort step
+    int64_t* toffs = nullptr;        // tile offsets (ntiles + 2) after each sort step
+    int nsorts = 0;
+    bool sorts_at(int i) const { return kernel == 2 && i % sort_every == 0; }
     std::vector<double> env;         // envelope per step (empty: the analytic one)
     double objective = 0.0;
     // snapshots: [0] the run's initial state, [i] the state before step first + i*K
@@ -1044,16 +1166,26 @@ struct DiffState {
     char* seg = nullptr;
     int seg_steps = 0;
     size_t step_bytes = 0;
-    // adjoint state
-    double* lF[6] = {};
-    double* lJ[3] = {};
-    double* lp[7] = {};              // x y z ux uy uz w
+    // adjoint state, in the precision of the context
+    void* lF[6] = {};
+    void* lJ[3] = {};
+    void* lp[6] = {};                // x y z (FP32: the offsets) ux uy uz
+    double* lw = nullptr;            // accumulated d objective / d w (FP64)
+    void* lp_tmp[6] = {};            // targets of the un-permutation
+    double* lw_tmp = nullptr;
     double* genv = nullptr;          // per step (device)
     int64_t* stray = nullptr;        // tiled adjoint: slots outside their tile (+ count)
     double* red = nullptr;           // reduction scratch
+    size_t es() const { return f32 ? 4 : 8; }  // bytes of a field / particle value
+    // particle columns of a snapshot: FP64 x y z ux uy uz w;
+    // FP32 cell fx fy fz ux uy uz w
+    int ncol() const { return f32 ? 8 : 7; }
+    int wcol() const { return f32 ? 7 : 6; }
 };
 
 namespace {
+
+size_t up256(size_t b) { return (b + 255) / 256 * 256; }
 
 int dalloc(pic_ctx* c, void** p, size_t bytes) {
     if (hipMalloc(p, std::max<size_t>(bytes, 256)) != hipSuccess) {
@@ -1065,42 +1197,137 @@ int dalloc(pic_ctx* c, void** p, size_t bytes) {
 
 int fill0(pic_ctx* c, void* p, size_t bytes) { return launch_fill(c, p, (int64_t)bytes); }
 
-double* pcol(pic_ctx* c, int s, int f) {  // particle array f of species s (current buffer)
+// particle column f of species s (current buffer), in snapshot order
+void* pcol(pic_ctx* c, int s, int f) {
     Species& S = c->sp[s];
-    double* a[7] = {S.x[S.cur], S.y[S.cur], S.z[S.cur], S.ux[S.cur], S.uy[S.cur], S.uz[S.cur], S.w[S.cur]};
+    const int b = S.cur;
+    if (c->fp32) {
+        void* a[8] = {S.cell[b], S.fx[b], S.fy[b], S.fz[b], S.vx[b], S.vy[b], S.vz[b], S.fw[b]};
+        return a[f];
+    }
+    void* a[7] = {S.x[b], S.y[b], S.z[b], S.ux[b], S.uy[b], S.uz[b], S.w[b]};
     return a[f];
 }
 
-// state <-> snapshot: 6 fields, 3 currents, 7 particle arrays, counters
+// state <-> snapshot: 6 fields, 3 currents, the particle columns, counters
 int copy_state(pic_ctx* c, DiffState& D, char* snap, bool save) {
     const Grid& g = c->g;
-    const size_t fb = g.fsize * 8, jb = g.jsize * 8, pb = D.slots * 8;
+    const size_t es = D.es(), fb = g.fsize * es, jb = g.jsize * es, pb = D.slots * es;
     size_t off = 0;
     auto mv = [&](void* live, size_t bytes) -> int {
         void* dst = save ? (void*)(snap + off) : live;
         const void* src = save ? (const void*)live : (const void*)(snap + off);
         if (bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
             return set_err(c, PIC_ERR_HIP, "differentiated run: snapshot copy failed");
-        off += (bytes + 255) / 256 * 256;
+        off += up256(bytes);
         return PIC_OK;
     };
     int r = PIC_OK;
     for (int f = 0; f < 6 && !r; f++) r = mv(c->F[f], fb);
     for (int f = 0; f < 3 && !r; f++) r = mv(c->J[f], jb);
-    for (int f = 0; f < 7 && !r; f++) r = mv(pcol(c, D.s, f), pb);
+    for (int f = 0; f < D.ncol() && !r; f++) r = mv(pcol(c, D.s, f), pb);
     if (!r) r = mv(c->sp[D.s].d_cnt, CNT_N * 8);
+    if (!r) r = mv(c->sp[D.s].tile_off, (size_t)(c->tl.ntiles + 2) * 8);
     return r;
 }
 
 size_t snapshot_bytes(const pic_ctx* c, const DiffState& D) {
-    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-    return 6 * up(c->g.fsize * 8) + 3 * up(c->g.jsize * 8) + 7 * up(D.slots * 8) + up(CNT_N * 8);
+    const size_t es = D.es();
+    return 6 * up256(c->g.fsize * es) + 3 * up256(c->g.jsize * es) + D.ncol() * up256(D.slots * es) +
+           up256(CNT_N * 8) + up256((size_t)(c->tl.ntiles + 2) * 8);
+}
+
+// FP64/FP32 particle columns moved by a sort: the snapshot columns, the
+// ids and (FP64, when kept) x_old
+struct GatherCols {
+    const void* src[12];
+    void* dst[12];
+    int es[12];
+    int n;
+};
+
+__global__ __launch_bounds__(256) void k_gather_cols(GatherCols G, const int32_t* __restrict__ perm, int64_t slots) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < slots;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = perm[q];
+        for (int k = 0; k < G.n; k++) {
+            if (G.es[k] == 8) ((int64_t*)G.dst[k])[q] = ((const int64_t*)G.src[k])[p];
+            else ((int32_t*)G.dst[k])[q] = ((const int32_t*)G.src[k])[p];
+        }
+    }
+}
+
+// the adjoint of a permutation: lambda_before[perm[q]] = lambda_after[q]
+template <typename T>
+__global__ __launch_bounds__(256) void k_adj_unpermute(DiffLam<T> a, DiffLam<T> b, const int32_t* __restrict__ perm,
+                                                       int64_t slots) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < slots;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = perm[q];
+        for (int k = 0; k < 6; k++) b.l[k][p] = a.l[k][q];
+        b.w[p] = a.w[q];
+    }
+}
+
+// Replay of a recorded sort: the columns permuted into the other buffer,
+// which becomes current, and the tile offsets restored.
+int replay_sort(pic_ctx* c, DiffState& D, const int32_t* perm, const int64_t* toff) {
+    Species& S = c->sp[D.s];
+    GatherCols G{};
+    const int a = S.cur, b = 1 - S.cur;
+    auto add = [&](const void* x, void* y, int es) {
+        if (x && y) { G.src[G.n] = x; G.dst[G.n] = y; G.es[G.n] = es; G.n++; }
+    };
+    if (c->fp32) {
+        add(S.cell[a], S.cell[b], 4); add(S.fx[a], S.fx[b], 4); add(S.fy[a], S.fy[b], 4); add(S.fz[a], S.fz[b], 4);
+        add(S.vx[a], S.vx[b], 4); add(S.vy[a], S.vy[b], 4); add(S.vz[a], S.vz[b], 4); add(S.fw[a], S.fw[b], 4);
+    } else {
+        add(S.x[a], S.x[b], 8); add(S.y[a], S.y[b], 8); add(S.z[a], S.z[b], 8); add(S.ux[a], S.ux[b], 8);
+        add(S.uy[a], S.uy[b], 8); add(S.uz[a], S.uz[b], 8); add(S.w[a], S.w[b], 8);
+        if (c->keep_old) { add(S.xo[a], S.xo[b], 8); add(S.yo[a], S.yo[b], 8); add(S.zo[a], S.zo[b], 8); }
+    }
+    add(S.id[a], S.id[b], 8);
+    if (D.slots) {
+        k_gather_cols<<<(int)std::min<int64_t>(blocks_for(D.slots, 256), 8192), 256, 0, c->stream>>>(G, perm, D.slots);
+        PIC_LAUNCH_CHECK(c);
+    }
+    PIC_HIP(c, hipMemcpyAsync(S.tile_off, toff, (size_t)(c->tl.ntiles + 2) * 8, hipMemcpyDeviceToDevice, c->stream));
+    S.cur = b;
+    S.sorted = 1;
+    S.pushes_since_sort = 0;
+    return PIC_OK;
+}
+
+// offset of particle column f in a snapshot
+size_t snap_col(const pic_ctx* c, const DiffState& D, int f) {
+    const size_t es = D.es();
+    return 6 * up256(c->g.fsize * es) + 3 * up256(c->g.jsize * es) + f * up256(D.slots * es);
 }
 
 // one step of run_main_time_loop (simulation.cpp:159-183) with the
 // envelope value e; 'save' (may be null) receives the particle state and
 // the fields at the push
-int diff_step(pic_ctx* c, DiffState& D, int st, char* save) {
+// The sort before step i's push: recorded (forward) or replayed (recompute).
+int diff_sort(pic_ctx* c, DiffState& D, int i, bool replay) {
+    if (!D.sorts_at(i)) return PIC_OK;
+    const int k = i / D.sort_every;
+    int32_t* perm = D.perms + (size_t)k * D.slots;
+    int64_t* toff = D.toffs + (size_t)k * (c->tl.ntiles + 2);
+    if (replay) return replay_sort(c, D, perm, toff);
+    int r = launch_sort(c, D.s, perm);
+    if (r) return r;
+    PIC_HIP(c, hipMemcpyAsync(toff, c->sp[D.s].tile_off, (size_t)(c->tl.ntiles + 2) * 8, hipMemcpyDeviceToDevice,
+                              c->stream));
+    return PIC_OK;
+}
+
+// bytes of one step's saved push inputs: particle columns, fields, tile offsets
+size_t step_state_bytes(const pic_ctx* c, const DiffState& D) {
+    const size_t es = D.es();
+    return D.ncol() * up256(D.slots * es) + 6 * up256(c->g.fsize * es) + up256((size_t)(c->tl.ntiles + 2) * 8);
+}
+
+int diff_step(pic_ctx* c, DiffState& D, int st, char* save, bool replay) {
     int r;
     c->env_on = D.env.empty() ? 0 : 1;
     c->env_val = D.env.empty() ? 0.0 : D.env[st - D.first];
@@ -1109,13 +1336,17 @@ int diff_step(pic_ctx* c, DiffState& D, int st, char* save) {
     if (r) return r;
     if ((r = pic_fields_first(c))) return r;
     if ((r = pic_clear_currents(c))) return r;
+    if ((r = diff_sort(c, D, st - D.first, replay))) return r;
     if (save) {
-        const size_t pb = (D.slots * 8 + 255) / 256 * 256, fb = (c->g.fsize * 8 + 255) / 256 * 256;
-        for (int f = 0; f < 7; f++)
+        const size_t es = D.es(), pb = up256(D.slots * es), fb = up256(c->g.fsize * es);
+        for (int f = 0; f < D.ncol(); f++)
             if (D.slots)
-                PIC_HIP(c, hipMemcpyAsync(save + f * pb, pcol(c, D.s, f), D.slots * 8, hipMemcpyDeviceToDevice, c->stream));
+                PIC_HIP(c, hipMemcpyAsync(save + f * pb, pcol(c, D.s, f), D.slots * es, hipMemcpyDeviceToDevice, c->stream));
         for (int f = 0; f < 6; f++)
-            PIC_HIP(c, hipMemcpyAsync(save + 7 * pb + f * fb, c->F[f], c->g.fsize * 8, hipMemcpyDeviceToDevice, c->stream));
+            PIC_HIP(c, hipMemcpyAsync(save + D.ncol() * pb + f * fb, c->F[f], c->g.fsize * es,
+                                      hipMemcpyDeviceToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(save + D.ncol() * pb + 6 * fb, c->sp[D.s].tile_off,
+                                  (size_t)(c->tl.ntiles + 2) * 8, hipMemcpyDeviceToDevice, c->stream));
     }
     if ((r = launch_push(c, D.s))) return r;
     if ((r = pic_particle_boundary(c, D.s, nullptr))) return r;
@@ -1123,140 +1354,229 @@ int diff_step(pic_ctx* c, DiffState& D, int st, char* save) {
     return pic_fields_second(c);
 }
 
-int adj_wrap3(pic_ctx* c, double* const* a) {
+template <typename T> int adj_wrap3(pic_ctx* c, void* const* a) {
     const Grid& g = c->g;
     for (int ax = 2; ax >= 0; ax--) {
         if (!c->p.periodic[ax]) continue;
         const int e0 = ax == 0 ? g.fe[1] : g.fe[0], e1 = ax == 2 ? g.fe[1] : g.fe[2];
         const int64_t total = (int64_t)2 * g.m * e0 * e1;
-        k_adj_wrap<<<blocks_for(total, 256), 256, 0, c->stream>>>(a[0], a[1], a[2], g, ax);
+        k_adj_wrap<T><<<blocks_for(total, 256), 256, 0, c->stream>>>((T*)a[0], (T*)a[1], (T*)a[2], g, ax);
         PIC_LAUNCH_CHECK(c);
     }
     return PIC_OK;
 }
 
-int adj_e(pic_ctx* c, DiffState& D, double coef) {
+template <typename T> int adj_e(pic_ctx* c, DiffState& D, double coef) {
     const Grid& g = c->g;
-    k_adj_e<<<blocks_for(g.fsize, 256), 256, 0, c->stream>>>(D.lF[0], D.lF[1], D.lF[2], D.lF[3], D.lF[4],
-                                                              D.lF[5], D.lJ[0], D.lJ[1], D.lJ[2], g, coef,
-                                                              c->p.dx, c->p.dy, c->p.dz);
+    T* const* l = (T* const*)D.lF;
+    T* const* j = (T* const*)D.lJ;
+    k_adj_e<T><<<blocks_for(g.fsize, 256), 256, 0, c->stream>>>(l[0], l[1], l[2], l[3], l[4], l[5], j[0], j[1],
+                                                                 j[2], g, (T)coef, (T)c->p.dx, (T)c->p.dy,
+                                                                 (T)c->p.dz);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
 
-int adj_b(pic_ctx* c, DiffState& D, double coef) {
+template <typename T> int adj_b(pic_ctx* c, DiffState& D, double coef) {
     const Grid& g = c->g;
-    k_adj_b<<<blocks_for(g.fsize, 256), 256, 0, c->stream>>>(D.lF[0], D.lF[1], D.lF[2], D.lF[3], D.lF[4],
-                                                              D.lF[5], g, coef, c->p.dx, c->p.dy, c->p.dz);
+    T* const* l = (T* const*)D.lF;
+    k_adj_b<T><<<blocks_for(g.fsize, 256), 256, 0, c->stream>>>(l[0], l[1], l[2], l[3], l[4], l[5], g, (T)coef,
+                                                                 (T)c->p.dx, (T)c->p.dy, (T)c->p.dz);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
 
-int adj_fold(pic_ctx* c, DiffState& D) {
+template <typename T> int adj_fold(pic_ctx* c, DiffState& D) {
     const Grid& g = c->g;
     for (int phase = 1; phase >= 0; phase--)
         for (int ax = 2; ax >= 0; ax--) {
             if (!c->p.periodic[ax]) continue;
             const int e0 = ax == 0 ? g.je[1] : g.je[0], e1 = ax == 2 ? g.je[1] : g.je[2];
             const int64_t total = (int64_t)(g.m + 1) * e0 * e1;
-            k_adj_fold<<<blocks_for(total, 256), 256, 0, c->stream>>>(D.lJ[0], D.lJ[1], D.lJ[2], g, ax, phase);
+            k_adj_fold<T><<<blocks_for(total, 256), 256, 0, c->stream>>>((T*)D.lJ[0], (T*)D.lJ[1], (T*)D.lJ[2], g,
+                                                                         ax, phase);
             PIC_LAUNCH_CHECK(c);
         }
     return PIC_OK;
 }
 
+// FP64 particle view of the push at one step: the saved state (save) or
+// the live one (null)
 AdjPush adj_view(pic_ctx* c, DiffState& D, const char* save) {
     AdjPush A;
-    const size_t pb = (D.slots * 8 + 255) / 256 * 256, fb = (c->g.fsize * 8 + 255) / 256 * 256;
+    const size_t pb = up256(D.slots * 8), fb = up256(c->g.fsize * 8);
     const double* P[7];
-    for (int f = 0; f < 7; f++) P[f] = save ? (const double*)(save + f * pb) : pcol(c, D.s, f);
+    for (int f = 0; f < 7; f++) P[f] = save ? (const double*)(save + f * pb) : (const double*)pcol(c, D.s, f);
     A.x = P[0]; A.y = P[1]; A.z = P[2]; A.ux = P[3]; A.uy = P[4]; A.uz = P[5]; A.w = P[6];
     for (int f = 0; f < 6; f++) A.F[f] = save ? (const double*)(save + 7 * pb + f * fb) : (const double*)c->F[f];
-    for (int f = 0; f < 3; f++) A.lJ[f] = D.lJ[f];
-    for (int f = 0; f < 6; f++) A.lF[f] = D.lF[f];
-    A.lx = D.lp[0]; A.ly = D.lp[1]; A.lz = D.lp[2]; A.lux = D.lp[3]; A.luy = D.lp[4]; A.luz = D.lp[5];
-    A.lw = D.lp[6];
+    for (int f = 0; f < 3; f++) A.lJ[f] = (const double*)D.lJ[f];
+    for (int f = 0; f < 6; f++) A.lF[f] = (double*)D.lF[f];
+    double* const* lp = (double* const*)D.lp;
+    A.lx = lp[0]; A.ly = lp[1]; A.lz = lp[2]; A.lux = lp[3]; A.luy = lp[4]; A.luz = lp[5];
+    A.lw = D.lw;
     A.n = D.slots;
     return A;
 }
 
-template <int M>
-int launch_adj_tiled(pic_ctx* c, DiffState& D, const AdjPush& A, const PushConsts& K) {
-    using TS = TileShape<0, M>;
-    using B = AdjBox<M, TS::X, TS::Y, TS::Z>;
+// FP32 particle view, likewise
+AdjPush32 adj_view32(pic_ctx* c, DiffState& D, const char* save) {
+    AdjPush32 A;
+    const size_t pb = up256(D.slots * 4), fb = up256(c->g.fsize * 4);
+    const void* P[8];
+    for (int f = 0; f < 8; f++) P[f] = save ? (const void*)(save + f * pb) : (const void*)pcol(c, D.s, f);
+    A.cell = (const int32_t*)P[0];
+    A.fx = (const float*)P[1]; A.fy = (const float*)P[2]; A.fz = (const float*)P[3];
+    A.ux = (const float*)P[4]; A.uy = (const float*)P[5]; A.uz = (const float*)P[6]; A.w = (const float*)P[7];
+    for (int f = 0; f < 6; f++) A.F[f] = save ? (const float*)(save + 8 * pb + f * fb) : (const float*)c->F[f];
+    for (int f = 0; f < 3; f++) A.lJ[f] = (const float*)D.lJ[f];
+    for (int f = 0; f < 6; f++) A.lF[f] = (float*)D.lF[f];
+    float* const* lp = (float* const*)D.lp;
+    A.lx = lp[0]; A.ly = lp[1]; A.lz = lp[2]; A.lux = lp[3]; A.luy = lp[4]; A.luz = lp[5];
+    A.lw = D.lw;
+    A.n = D.slots;
+    return A;
+}
+
+AdjK<float> adj_consts32(const pic_ctx* c, const Species& S) {
+    AdjK<float> K;
+    K.q = (float)S.q;
+    K.qm = (float)(S.q / S.m * (c->p.dt / 2.0));
+    const double d[3] = {c->p.dx, c->p.dy, c->p.dz};
+    for (int a = 0; a < 3; a++) {
+        K.dtd[a] = (float)(c->p.dt / d[a]);
+        K.jc[a] = (float)(S.q * d[a] / c->p.dt);
+    }
+    return K;
+}
+
+int tile_kind(const pic_ctx* c) { return c->fp32 ? (c->fix32 ? 2 : 1) : 0; }
+
+template <int M, int KIND, typename V, typename LACC = double>
+int launch_adj_tiled(pic_ctx* c, DiffState& D, const V& v) {
+    using TS = TileShape<KIND, M>;
+    using B = AdjBox<M, TS::X, TS::Y, TS::Z, typename V::T, LACC>;
     Species& S = c->sp[D.s];
     int r = build_work(c, D.s, PIC_PUSH_CHUNK);
     if (r) return r;
-    PIC_HIP(c, hipFuncSetAttribute((const void*)k_adj_push64_tiled<M, TS::X, TS::Y, TS::Z>,
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_adj_push_tiled<M, TS::X, TS::Y, TS::Z, V, LACC>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
     const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
     int64_t* cnt = D.stray + std::max<int64_t>(D.slots, 1);
     if ((r = launch_fill(c, cnt, 16))) return r;
-    k_adj_push64_tiled<M, TS::X, TS::Y, TS::Z><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
-        A, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, D.stray, cnt);
+    k_adj_push_tiled<M, TS::X, TS::Y, TS::Z, V, LACC><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
+        v, c->g, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, D.stray, cnt);
     PIC_LAUNCH_CHECK(c);
-    k_adj_push64_list<M><<<1024, 256, 0, c->stream>>>(A, c->g, K, D.stray, cnt);
+    k_adj_push_hbm<M, V><<<1024, 256, 0, c->stream>>>(v, c->g, D.stray, cnt);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
 
 // adjoint push: the LDS-tiled kernel when the run's forward is tiled, the
-// separable HBM kernel for the direct forward, or (PIC_OPT_ADJ_PUSH = 1)
-// the dual-number kernel, an independent evaluation of the same derivatives
-int launch_adj_push(pic_ctx* c, DiffState& D, const AdjPush& A, const PushConsts& K) {
+// separable HBM kernel for the direct forward (or PIC_OPT_ADJ_PUSH = 2),
+// and (FP64, PIC_OPT_ADJ_PUSH = 1) the dual-number kernel, an independent
+// evaluation of the same derivatives
+int launch_adj_push(pic_ctx* c, DiffState& D, const char* save) {
     const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(D.slots, 1), 256), 8192);
     const bool q = c->g.m == 2;
+    const bool tiled = D.kernel == 2 && (c->adj_push == 0 || (c->adj_push == 3 && D.f32));
+    if (tiled && tile_kind(c) != D.kind)
+        return set_err(c, PIC_ERR_STATE, "differentiated run: the deposit accumulator option changed during the run");
+    if (D.f32) {
+        AdjView32 v{adj_view32(c, D, save), adj_consts32(c, c->sp[D.s]), {c->g.n[0], c->g.n[1], c->g.n[2]}, c->g.m};
+        if (tiled && c->adj_push == 3) {  // fp32 lambda_F accumulators in LDS
+            if (D.kind == 2) return q ? launch_adj_tiled<2, 2, AdjView32, float>(c, D, v)
+                                      : launch_adj_tiled<4, 2, AdjView32, float>(c, D, v);
+            return q ? launch_adj_tiled<2, 1, AdjView32, float>(c, D, v) : launch_adj_tiled<4, 1, AdjView32, float>(c, D, v);
+        }
+        if (tiled) {
+            if (D.kind == 2) return q ? launch_adj_tiled<2, 2>(c, D, v) : launch_adj_tiled<4, 2>(c, D, v);
+            return q ? launch_adj_tiled<2, 1>(c, D, v) : launch_adj_tiled<4, 1>(c, D, v);
+        }
+        if (q) k_adj_push_hbm<2><<<blocks, 256, 0, c->stream>>>(v, c->g, nullptr, nullptr);
+        else k_adj_push_hbm<4><<<blocks, 256, 0, c->stream>>>(v, c->g, nullptr, nullptr);
+        PIC_LAUNCH_CHECK(c);
+        return PIC_OK;
+    }
+    const PushConsts K = push_consts64(c, c->sp[D.s]);
+    const AdjPush A = adj_view(c, D, save);
     if (c->adj_push == 1) {
         if (q) k_adj_push64<2><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
         else k_adj_push64<4><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
-    } else if (D.kernel == 2 && c->adj_push == 0) {
-        return q ? launch_adj_tiled<2>(c, D, A, K) : launch_adj_tiled<4>(c, D, A, K);
-    } else {
-        if (q) k_adj_push64_sep<2><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
-        else k_adj_push64_sep<4><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
+        PIC_LAUNCH_CHECK(c);
+        return PIC_OK;
     }
+    AdjView64 v{A, K, adj_consts64(K), c->g.m};
+    if (tiled) return q ? launch_adj_tiled<2, 0>(c, D, v) : launch_adj_tiled<4, 0>(c, D, v);
+    if (q) k_adj_push_hbm<2><<<blocks, 256, 0, c->stream>>>(v, c->g, nullptr, nullptr);
+    else k_adj_push_hbm<4><<<blocks, 256, 0, c->stream>>>(v, c->g, nullptr, nullptr);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
 
 // reverse of diff_step for step st, from the saved push-time state
-int adj_step(pic_ctx* c, DiffState& D, int st, const char* save) {
+template <typename T> int adj_step_t(pic_ctx* c, DiffState& D, int st, const char* save) {
     const double h = c->p.dt / 2.0;
-    double* lE[3] = {D.lF[0], D.lF[1], D.lF[2]};
-    double* lB[3] = {D.lF[3], D.lF[4], D.lF[5]};
+    void* lE[3] = {D.lF[0], D.lF[1], D.lF[2]};
+    void* lB[3] = {D.lF[3], D.lF[4], D.lF[5]};
     int r;
     // fields_second: B then E
-    if ((r = adj_wrap3(c, lE)) || (r = adj_e(c, D, h))) return r;
-    if ((r = adj_wrap3(c, lB)) || (r = adj_b(c, D, h))) return r;
+    if ((r = adj_wrap3<T>(c, lE)) || (r = adj_e<T>(c, D, h))) return r;
+    if ((r = adj_wrap3<T>(c, lB)) || (r = adj_b<T>(c, D, h))) return r;
     // fold, boundary (dead slots carry no adjoint), push
-    if ((r = adj_fold(c, D))) return r;
-    const PushConsts K = push_consts64(c, c->sp[D.s]);
-    const AdjPush A = adj_view(c, D, save);
-    if ((r = launch_adj_push(c, D, A, K))) return r;
+    if ((r = adj_fold<T>(c, D))) return r;
+    if (D.kernel == 2) {  // the tile grouping of this step's push
+        const size_t pb = up256(D.slots * sizeof(T)), fb = up256(c->g.fsize * sizeof(T));
+        PIC_HIP(c, hipMemcpyAsync(c->sp[D.s].tile_off, save + D.ncol() * pb + 6 * fb,
+                                  (size_t)(c->tl.ntiles + 2) * 8, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if ((r = launch_adj_push(c, D, save))) return r;
+    // the sort before the push
+    const int i = st - D.first;
+    if (D.sorts_at(i) && D.slots) {
+        const int32_t* perm = D.perms + (size_t)(i / D.sort_every) * D.slots;
+        DiffLam<T> a{{(T*)D.lp[0], (T*)D.lp[1], (T*)D.lp[2], (T*)D.lp[3], (T*)D.lp[4], (T*)D.lp[5]}, D.lw};
+        DiffLam<T> b{{(T*)D.lp_tmp[0], (T*)D.lp_tmp[1], (T*)D.lp_tmp[2], (T*)D.lp_tmp[3], (T*)D.lp_tmp[4],
+                      (T*)D.lp_tmp[5]}, D.lw_tmp};
+        k_adj_unpermute<T><<<(int)std::min<int64_t>(blocks_for(D.slots, 256), 8192), 256, 0, c->stream>>>(
+            a, b, perm, D.slots);
+        PIC_LAUNCH_CHECK(c);
+        for (int f = 0; f < 6; f++) std::swap(D.lp[f], D.lp_tmp[f]);
+        std::swap(D.lw, D.lw_tmp);
+    }
     // clear: the currents before it get their adjoint from this step's
     // E half step and laser boundary only
     for (int f = 0; f < 3; f++)
-        if ((r = fill0(c, D.lJ[f], c->g.jsize * 8))) return r;
+        if ((r = fill0(c, D.lJ[f], c->g.jsize * sizeof(T)))) return r;
     // fields_first: E then B
-    if ((r = adj_wrap3(c, lB)) || (r = adj_b(c, D, h))) return r;
-    if ((r = adj_wrap3(c, lE)) || (r = adj_e(c, D, h))) return r;
+    if ((r = adj_wrap3<T>(c, lB)) || (r = adj_b<T>(c, D, h))) return r;
+    if ((r = adj_wrap3<T>(c, lE)) || (r = adj_e<T>(c, D, h))) return r;
     // laser boundary (x open only)
     if (!c->p.periodic[0]) {
         LaserArgs L = laser_args(c, st, 1, c->p.laser_right);
         L.env_on = D.env.empty() ? 0 : 1;
         L.env_val = D.env.empty() ? 0.0 : D.env[st - D.first];
         dim3 grid(blocks_for(c->g.n[1], 256), c->g.n[2]);
-        k_adj_laser<<<grid, 256, 0, c->stream>>>(D.lF[1], D.lF[2], D.lF[3], D.lF[4], D.lF[5], D.lJ[1], D.lJ[2],
-                                                 c->g, L, D.genv + (st - D.first));
+        T* const* l = (T* const*)D.lF;
+        k_adj_laser<T><<<grid, 256, 0, c->stream>>>(l[1], l[2], l[3], l[4], l[5], (T*)D.lJ[1], (T*)D.lJ[2], c->g, L,
+                                                    D.genv + (st - D.first));
         PIC_LAUNCH_CHECK(c);
     }
     return PIC_OK;
+}
+
+int adj_step(pic_ctx* c, DiffState& D, int st, const char* save) {
+    return D.f32 ? adj_step_t<float>(c, D, st, save) : adj_step_t<double>(c, D, st, save);
 }
 
 void free_adjoint(DiffState& D) {
     for (auto& p : D.lF) { if (p) hipFree(p); p = nullptr; }
     for (auto& p : D.lJ) { if (p) hipFree(p); p = nullptr; }
     for (auto& p : D.lp) { if (p) hipFree(p); p = nullptr; }
+    for (auto& p : D.lp_tmp) { if (p) hipFree(p); p = nullptr; }
+    if (D.lw) hipFree(D.lw);
+    D.lw = nullptr;
+    if (D.lw_tmp) hipFree(D.lw_tmp);
+    D.lw_tmp = nullptr;
     if (D.genv) hipFree(D.genv);
     D.genv = nullptr;
     if (D.stray) hipFree(D.stray);
@@ -1272,6 +1592,11 @@ void free_checkpoints(DiffState& D, bool keep_initial) {
     D.ck.resize(keep_initial && !D.ck.empty() ? 1 : 0);
     if (D.fin) hipFree(D.fin);
     D.fin = nullptr;
+    if (D.perms) hipFree(D.perms);
+    D.perms = nullptr;
+    if (D.toffs) hipFree(D.toffs);
+    D.toffs = nullptr;
+    D.nsorts = 0;
 }
 
 }  // namespace
@@ -1298,7 +1623,6 @@ extern "C" {
 
 int pic_diff_begin(pic_ctx* c, int s) {
     if (!c || s < 0 || s >= PIC_MAX_SPECIES) return PIC_ERR_ARG;
-    if (c->fp32) return set_err(c, PIC_ERR_STATE, "the differentiated run is FP64 only");
     Species& S = c->sp[s];
     if (!S.declared || S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is not a mobile species", s);
     for (int o = 0; o < PIC_MAX_SPECIES; o++)
@@ -1320,10 +1644,13 @@ int pic_diff_begin(pic_ctx* c, int s) {
     c->diff = new DiffState();
     DiffState& D = *c->diff;
     D.s = s;
+    D.f32 = c->fp32;
+    D.kind = tile_kind(c);
     D.slots = cnt[CNT_SLOTS];
     D.saved_push_kernel = c->push_kernel;
     D.saved_sort_interval = c->sort_interval;
     D.kernel = kernel;
+    D.sort_every = std::max(1, c->sort_interval);
     D.ck_bytes = snapshot_bytes(c, D);
     void* p = nullptr;
     int r = dalloc(c, &p, D.ck_bytes);
@@ -1349,14 +1676,38 @@ int pic_diff_initial(pic_ctx* c, double* x, double* y, double* z, double* w) {
     if (!c->diff) return set_err(c, PIC_ERR_STATE, "no differentiated run (pic_diff_begin)");
     DiffState& D = *c->diff;
     PIC_HIP(c, hipSetDevice(c->device));
-    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-    const size_t base = 6 * up(c->g.fsize * 8) + 3 * up(c->g.jsize * 8), pb = up(D.slots * 8);
     double* dst[4] = {x, y, z, w};
-    const int col[4] = {0, 1, 2, 6};
-    for (int i = 0; i < 4; i++)
-        if (dst[i] && D.slots)
-            PIC_HIP(c, hipMemcpyAsync(dst[i], D.ck[0] + base + col[i] * pb, D.slots * 8, hipMemcpyDeviceToHost, c->stream));
+    if (!D.f32) {
+        const int col[4] = {0, 1, 2, 6};
+        for (int i = 0; i < 4; i++)
+            if (dst[i] && D.slots)
+                PIC_HIP(c, hipMemcpyAsync(dst[i], D.ck[0] + snap_col(c, D, col[i]), D.slots * 8, hipMemcpyDeviceToHost,
+                                          c->stream));
+        PIC_HIP(c, hipStreamSynchronize(c->stream));
+        return PIC_OK;
+    }
+    // FP32: global position = cell + offset; a removed slot has a NaN weight
+    std::vector<int32_t> cell(D.slots);
+    std::vector<float> v[4];
+    const int col[4] = {1, 2, 3, 7};
+    if (D.slots) {
+        PIC_HIP(c, hipMemcpyAsync(cell.data(), D.ck[0] + snap_col(c, D, 0), D.slots * 4, hipMemcpyDeviceToHost,
+                                  c->stream));
+        for (int i = 0; i < 4; i++) {
+            v[i].resize(D.slots);
+            PIC_HIP(c, hipMemcpyAsync(v[i].data(), D.ck[0] + snap_col(c, D, col[i]), D.slots * 4,
+                                      hipMemcpyDeviceToHost, c->stream));
+        }
+    }
     PIC_HIP(c, hipStreamSynchronize(c->stream));
+    const int nx = c->g.n[0], ny = c->g.n[1];
+    for (int64_t p = 0; p < D.slots; p++) {
+        const int cl = cell[p];
+        const int cc[3] = {cl % nx, (cl / nx) % ny, cl / (nx * ny)};
+        for (int a = 0; a < 3; a++)
+            if (dst[a]) dst[a][p] = cl < 0 ? 0.0 : cc[a] + (double)v[a][p];
+        if (w) w[p] = cl < 0 ? __builtin_nan("") : (double)v[3][p];
+    }
     return PIC_OK;
 }
 
@@ -1376,13 +1727,26 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
     if (envelope) D.env.assign(envelope, envelope + nsteps);
     else D.env.clear();
     int r = copy_state(c, D, D.ck[0], false);
+    if (!r && D.kernel == 2 && nsteps > 0) {  // the run's sort records
+        if (D.slots > INT32_MAX)
+            return set_err(c, PIC_ERR_ARG, "differentiated run: %lld slots exceed the int32 sort records",
+                           (long long)D.slots);
+        D.nsorts = (nsteps + D.sort_every - 1) / D.sort_every;
+        r = dalloc(c, (void**)&D.perms, (size_t)D.nsorts * std::max<int64_t>(D.slots, 1) * 4);
+        if (!r) r = dalloc(c, (void**)&D.toffs, (size_t)D.nsorts * (c->tl.ntiles + 2) * 8);
+    }
     if (!r && weights && D.slots) {  // new initial weights: into the state and the snapshot
         double* tmp = nullptr;
         r = dalloc(c, (void**)&tmp, D.slots * 8);
         if (!r && hipMemcpyAsync(tmp, weights, D.slots * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
             r = set_err(c, PIC_ERR_HIP, "pic_diff_forward: weight upload failed");
         if (!r) {
-            k_adj_set_w<<<blocks_for(D.slots, 256), 256, 0, c->stream>>>(pcol(c, D.s, 6), tmp, D.slots);
+            if (D.f32)
+                k_adj_set_w32<<<blocks_for(D.slots, 256), 256, 0, c->stream>>>(
+                    (const int32_t*)pcol(c, D.s, 0), (float*)pcol(c, D.s, D.wcol()), tmp, D.slots);
+            else
+                k_adj_set_w<<<blocks_for(D.slots, 256), 256, 0, c->stream>>>((double*)pcol(c, D.s, D.wcol()), tmp,
+                                                                           D.slots);
             if (hipGetLastError() != hipSuccess) r = set_err(c, PIC_ERR_HIP, "pic_diff_forward: weight kernel failed");
         }
         if (!r) r = copy_state(c, D, D.ck[0], true);
@@ -1401,7 +1765,7 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
                 r = copy_state(c, D, D.ck.back(), true);
             }
         }
-        if (!r) r = diff_step(c, D, st, nullptr);
+        if (!r) r = diff_step(c, D, st, nullptr, false);
     }
     c->push_kernel = D.saved_push_kernel;
     c->sort_interval = D.saved_sort_interval;
@@ -1412,8 +1776,15 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
     Species& S = c->sp[D.s];
     if ((r = fill0(c, D.red, 8))) return r;
     if (D.slots) {
-        k_adj_kinetic<<<(int)std::min<int64_t>(blocks_for(D.slots, 256), 2048), 256, 0, c->stream>>>(
-            pcol(c, D.s, 6), pcol(c, D.s, 3), pcol(c, D.s, 4), pcol(c, D.s, 5), D.slots, D.red);
+        const int blocks = (int)std::min<int64_t>(blocks_for(D.slots, 256), 2048);
+        if (D.f32)
+            k_adj_kinetic32<<<blocks, 256, 0, c->stream>>>(
+                (const int32_t*)pcol(c, D.s, 0), (const float*)pcol(c, D.s, 7), (const float*)pcol(c, D.s, 4),
+                (const float*)pcol(c, D.s, 5), (const float*)pcol(c, D.s, 6), D.slots, D.red);
+        else
+            k_adj_kinetic<<<blocks, 256, 0, c->stream>>>(
+                (const double*)pcol(c, D.s, 6), (const double*)pcol(c, D.s, 3), (const double*)pcol(c, D.s, 4),
+                (const double*)pcol(c, D.s, 5), D.slots, D.red);
         PIC_LAUNCH_CHECK(c);
     }
     double sum = 0.0;
@@ -1433,35 +1804,41 @@ int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, 
     PIC_HIP(c, hipSetDevice(c->device));
     PIC_HIP(c, hipStreamSynchronize(c->stream));
     const Grid& g = c->g;
+    const size_t es = D.es();
+    const int64_t ns = std::max<int64_t>(D.slots, 1);
     int r = copy_state(c, D, D.fin, false);  // the final state of the forward
     free_adjoint(D);
-    for (int f = 0; f < 6 && !r; f++) r = dalloc(c, (void**)&D.lF[f], g.fsize * 8);
-    for (int f = 0; f < 3 && !r; f++) r = dalloc(c, (void**)&D.lJ[f], g.jsize * 8);
-    for (int f = 0; f < 7 && !r; f++) r = dalloc(c, (void**)&D.lp[f], std::max<int64_t>(D.slots, 1) * 8);
+    for (int f = 0; f < 6 && !r; f++) r = dalloc(c, &D.lF[f], g.fsize * es);
+    for (int f = 0; f < 3 && !r; f++) r = dalloc(c, &D.lJ[f], g.jsize * es);
+    for (int f = 0; f < 6 && !r; f++) r = dalloc(c, &D.lp[f], ns * es);
+    if (!r) r = dalloc(c, (void**)&D.lw, ns * 8);
+    const bool sorts = D.kernel == 2;
+    for (int f = 0; f < 6 && !r && sorts; f++) r = dalloc(c, &D.lp_tmp[f], ns * es);
+    if (!r && sorts) r = dalloc(c, (void**)&D.lw_tmp, ns * 8);
     if (!r) r = dalloc(c, (void**)&D.genv, std::max(D.nsteps, 1) * 8);
-    if (!r) r = dalloc(c, (void**)&D.stray, (std::max<int64_t>(D.slots, 1) + 1) * 8);
-    D.step_bytes = 7 * ((D.slots * 8 + 255) / 256 * 256) + 6 * ((g.fsize * 8 + 255) / 256 * 256);
+    if (!r) r = dalloc(c, (void**)&D.stray, (ns + 1) * 8);
+    D.step_bytes = step_state_bytes(c, D);
     D.seg_steps = std::min(D.K, std::max(D.nsteps, 1));
     if (!r) r = dalloc(c, (void**)&D.seg, D.step_bytes * D.seg_steps);
-    for (int f = 0; f < 6 && !r; f++) r = fill0(c, D.lF[f], g.fsize * 8);
-    for (int f = 0; f < 3 && !r; f++) r = fill0(c, D.lJ[f], g.jsize * 8);
+    for (int f = 0; f < 6 && !r; f++) r = fill0(c, D.lF[f], g.fsize * es);
+    for (int f = 0; f < 3 && !r; f++) r = fill0(c, D.lJ[f], g.jsize * es);
     if (!r) r = fill0(c, D.genv, std::max(D.nsteps, 1) * 8);
     if (r) return r;
     // adjoint of the objective at the final state (the live state now)
     Species& S = c->sp[D.s];
-    {
-        const AdjPush A = adj_view(c, D, nullptr);
-        k_adj_objective<<<blocks_for(std::max<int64_t>(D.slots, 1), 256), 256, 0, c->stream>>>(
-            A, S.m * c->p.dx * c->p.dy * c->p.dz, grad_objective);
-        PIC_LAUNCH_CHECK(c);
-    }
+    const double mdv = S.m * c->p.dx * c->p.dy * c->p.dz;
+    if (D.f32)
+        k_adj_objective32<<<blocks_for(ns, 256), 256, 0, c->stream>>>(adj_view32(c, D, nullptr), mdv, grad_objective);
+    else
+        k_adj_objective<<<blocks_for(ns, 256), 256, 0, c->stream>>>(adj_view(c, D, nullptr), mdv, grad_objective);
+    PIC_LAUNCH_CHECK(c);
     c->push_kernel = D.kernel;
     c->sort_interval = 1 << 30;
     const int nseg = (D.nsteps + D.K - 1) / D.K;
     for (int sg = nseg - 1; sg >= 0 && !r; sg--) {
         const int a = D.first + sg * D.K, b = std::min(a + D.K, D.first + D.nsteps);
         r = copy_state(c, D, D.ck[sg], false);
-        for (int st = a; st < b && !r; st++) r = diff_step(c, D, st, D.seg + (size_t)(st - a) * D.step_bytes);
+        for (int st = a; st < b && !r; st++) r = diff_step(c, D, st, D.seg + (size_t)(st - a) * D.step_bytes, true);
         for (int st = b - 1; st >= a && !r; st--) r = adj_step(c, D, st, D.seg + (size_t)(st - a) * D.step_bytes);
     }
     c->push_kernel = D.saved_push_kernel;
@@ -1472,7 +1849,7 @@ int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, 
     if (grad_envelope && D.nsteps)
         PIC_HIP(c, hipMemcpyAsync(grad_envelope, D.genv, D.nsteps * 8, hipMemcpyDeviceToHost, c->stream));
     if (grad_weights && D.slots)
-        PIC_HIP(c, hipMemcpyAsync(grad_weights, D.lp[6], D.slots * 8, hipMemcpyDeviceToHost, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(grad_weights, D.lw, D.slots * 8, hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(c, hipStreamSynchronize(c->stream));
     return PIC_OK;
 }

@@ -42,8 +42,9 @@ struct Soa64 {
     int64_t* id;
 };
 
+// keep_dead: removed slots counted in bin ntiles instead of dropped
 __global__ __launch_bounds__(256) void k_hist64(Soa64 A, const int64_t* __restrict__ cnt,
-                                                 int64_t* __restrict__ tile_cnt, TileArgs T) {
+                                                 int64_t* __restrict__ tile_cnt, TileArgs T, int keep_dead) {
     const int64_t n = cnt[CNT_SLOTS];
     const int64_t base = (int64_t)blockIdx.x * blockDim.x;
     for (int64_t b = base; b < n; b += (int64_t)gridDim.x * blockDim.x) {
@@ -54,6 +55,7 @@ __global__ __launch_bounds__(256) void k_hist64(Soa64 A, const int64_t* __restri
             const double w = A.w[p];
             valid = (w == w);
             if (valid) key = tile_of64(A.x[p], A.y[p], A.z[p], T);
+            else if (keep_dead) { valid = true; key = T.tl.ntiles; }
         }
         int head, len;
         wave_runs(key, valid, head, len);
@@ -109,9 +111,11 @@ static Soa64 soa64(pic_ctx* c, Species& S, int b) {
                  S.xo[b], S.yo[b], S.zo[b], S.id[b]};
 }
 
+// perm != null: removed slots go after the last tile (bin ntiles) and
+// perm[q] records the source slot of slot q
 __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const int64_t* __restrict__ cnt,
                                                         int64_t* __restrict__ tile_cur, TileArgs T,
-                                                        int keep_old) {
+                                                        int keep_old, int32_t* __restrict__ perm) {
     const int64_t n = cnt[CNT_N - 1];
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = b + threadIdx.x;
@@ -121,10 +125,9 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
         if (p < n) {
             w = A.w[p];
             valid = (w == w);
-            if (valid) {
-                x = A.x[p]; y = A.y[p]; z = A.z[p];
-                key = tile_of64(x, y, z, T);
-            }
+            x = A.x[p]; y = A.y[p]; z = A.z[p];
+            if (valid) key = tile_of64(x, y, z, T);
+            else if (perm) { valid = true; key = T.tl.ntiles; }
         }
         int head, len;
         wave_runs(key, valid, head, len);
@@ -138,13 +141,14 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
             B.ux[q] = A.ux[p]; B.uy[q] = A.uy[p]; B.uz[q] = A.uz[p];
             B.id[q] = A.id[p];
             if (keep_old) { B.xo[q] = A.xo[p]; B.yo[q] = A.yo[p]; B.zo[q] = A.zo[p]; }
+            if (perm) perm[q] = (int32_t)p;
         }
     }
 }
 
-int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int blocks) {
+int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int blocks, int32_t* perm) {
     k_scatter64_dev<<<blocks, 256, 0, c->stream>>>(A, B, S.d_cnt, c->tile_cur, T,
-                                                   c->keep_old && S.xo[0] && S.xo[1]);
+                                                   c->keep_old && S.xo[0] && S.xo[1], perm);
     PIC_LAUNCH_CHECK(c);
     S.cur = 1 - S.cur;
     S.sorted = 1;
@@ -152,33 +156,36 @@ int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int b
     return PIC_OK;
 }
 
-int sort32(pic_ctx* c, int s);  // k_push32.hip
+int sort32(pic_ctx* c, int s, int32_t* perm);  // k_push32.hip
 
-int launch_scan_tiles(pic_ctx* c, Species& S) {
-    k_scan_tiles<<<1, 1024, 0, c->stream>>>(c->tile_cnt, S.tile_off, c->tile_cur, c->tl.ntiles, S.d_cnt);
+// nbins = ntiles, or ntiles + 1 with the removed slots' bin (keep-dead sort)
+int launch_scan_tiles(pic_ctx* c, Species& S, int nbins) {
+    k_scan_tiles<<<1, 1024, 0, c->stream>>>(c->tile_cnt, S.tile_off, c->tile_cur, nbins, S.d_cnt);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
 
-// Counting sort by tile into the scratch buffer; drops removed particles.
-// The slot count on the host side is not needed: kernels read it from
-// d_cnt, and the scatter grid covers the species capacity.
-int launch_sort(pic_ctx* c, int s) {
+// Counting sort by tile into the scratch buffer; drops removed particles
+// (or, with perm, keeps them after the last tile).  The slot count on the
+// host side is not needed: kernels read it from d_cnt, and the scatter grid
+// covers the species capacity.
+int launch_sort(pic_ctx* c, int s, int32_t* perm) {
     Species& S = c->sp[s];
-    if (c->fp32) return sort32(c, s);
-    int r = launch_fill(c, c->tile_cnt, (int64_t)c->tl.ntiles * 8);
+    if (c->fp32) return sort32(c, s, perm);
+    const int nbins = c->tl.ntiles + (perm ? 1 : 0);
+    int r = launch_fill(c, c->tile_cnt, (int64_t)nbins * 8);
     if (r) return r;
     TileArgs T = tile_args(c);
     const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 32);
     Soa64 A = soa64(c, S, S.cur), B = soa64(c, S, 1 - S.cur);
-    k_hist64<<<blocks, 256, 0, c->stream>>>(A, S.d_cnt, c->tile_cnt, T);
+    k_hist64<<<blocks, 256, 0, c->stream>>>(A, S.d_cnt, c->tile_cnt, T, perm ? 1 : 0);
     PIC_LAUNCH_CHECK(c);
     // the scatter needs the pre-sort slot count: keep it in d_cnt[CNT_N-1]
     PIC_HIP(c, hipMemcpyAsync(S.d_cnt + (CNT_N - 1), S.d_cnt + CNT_SLOTS, 8,
                               hipMemcpyDeviceToDevice, c->stream));
-    r = launch_scan_tiles(c, S);
+    r = launch_scan_tiles(c, S, nbins);
     if (r) return r;
-    return launch_scatter64(c, S, A, B, T, blocks);
+    return launch_scatter64(c, S, A, B, T, blocks, perm);
 }
 
 // ---------------------------------------------------------------------

@@ -502,29 +502,33 @@ __device__ __forceinline__ int tile_of32(int cell, const TileArgs32& A) {
     return ((cz / A.tl.t[2]) * A.tl.nt[1] + cy / A.tl.t[1]) * A.tl.nt[0] + cx / A.tl.t[0];
 }
 
+// keep_dead: removed slots counted in bin ntiles instead of dropped
 __global__ __launch_bounds__(256) void k_hist32(const int32_t* __restrict__ cellv,
                                                  const int64_t* __restrict__ cnt,
-                                                 int64_t* __restrict__ tile_cnt, TileArgs32 T) {
+                                                 int64_t* __restrict__ tile_cnt, TileArgs32 T, int keep_dead) {
     const int64_t n = cnt[CNT_SLOTS];
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = b + threadIdx.x;
         const int cell = p < n ? cellv[p] : -1;
-        const bool valid = cell >= 0;
-        const int key = valid ? tile_of32(cell, T) : -1;
+        const bool valid = cell >= 0 || (keep_dead && p < n);
+        const int key = cell >= 0 ? tile_of32(cell, T) : (valid ? T.tl.ntiles : -1);
         int head, len;
         wave_runs(key, valid, head, len);
         if (valid && head == lane_id()) atomicAdd((unsigned long long*)&tile_cnt[key], (unsigned long long)len);
     }
 }
 
+// perm != null: removed slots go after the last tile (bin ntiles) and
+// perm[q] records the source slot of slot q
 __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int64_t* __restrict__ cnt,
-                                                    int64_t* __restrict__ tile_cur, TileArgs32 T) {
+                                                    int64_t* __restrict__ tile_cur, TileArgs32 T,
+                                                    int32_t* __restrict__ perm) {
     const int64_t n = cnt[CNT_N - 1];
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = b + threadIdx.x;
         const int cell = p < n ? A.cell[p] : -1;
-        const bool valid = cell >= 0;
-        const int key = valid ? tile_of32(cell, T) : -1;
+        const bool valid = cell >= 0 || (perm && p < n);
+        const int key = cell >= 0 ? tile_of32(cell, T) : (valid ? T.tl.ntiles : -1);
         int head, len;
         wave_runs(key, valid, head, len);
         int64_t base = 0;
@@ -538,27 +542,29 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
             Bo.ux[q] = A.ux[p]; Bo.uy[q] = A.uy[p]; Bo.uz[q] = A.uz[p];
             Bo.w[q] = A.w[p];
             Bo.id[q] = A.id[p];
+            if (perm) perm[q] = (int32_t)p;
         }
     }
 }
 
-int sort32(pic_ctx* c, int s) {
+int sort32(pic_ctx* c, int s, int32_t* perm) {
     Species& S = c->sp[s];
-    int r = launch_fill(c, c->tile_cnt, (int64_t)c->tl.ntiles * 8);
+    const int nbins = c->tl.ntiles + (perm ? 1 : 0);
+    int r = launch_fill(c, c->tile_cnt, (int64_t)nbins * 8);
     if (r) return r;
     TileArgs32 T;
     for (int a = 0; a < 3; a++) T.n[a] = c->g.n[a];
     T.tl = c->tl;
     const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 32);
-    k_hist32<<<blocks, 256, 0, c->stream>>>(S.cell[S.cur], S.d_cnt, c->tile_cnt, T);
+    k_hist32<<<blocks, 256, 0, c->stream>>>(S.cell[S.cur], S.d_cnt, c->tile_cnt, T, perm ? 1 : 0);
     PIC_LAUNCH_CHECK(c);
     PIC_HIP(c, hipMemcpyAsync(S.d_cnt + (CNT_N - 1), S.d_cnt + CNT_SLOTS, 8,
                               hipMemcpyDeviceToDevice, c->stream));
-    extern int launch_scan_tiles(pic_ctx*, Species&);
-    r = launch_scan_tiles(c, S);
+    extern int launch_scan_tiles(pic_ctx*, Species&, int);
+    r = launch_scan_tiles(c, S, nbins);
     if (r) return r;
     k_scatter32<<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), soa32(S, 1 - S.cur), S.d_cnt,
-                                               c->tile_cur, T);
+                                               c->tile_cur, T, perm);
     PIC_LAUNCH_CHECK(c);
     S.cur = 1 - S.cur;
     S.sorted = 1;
@@ -733,7 +739,7 @@ int launch_push32(pic_ctx* c, int s) {
     {
         PhaseTimer t(c, PIC_T_SORT);
         if (!S.sorted || S.pushes_since_sort >= c->sort_interval) {
-            int r = sort32(c, s);
+            int r = sort32(c, s, nullptr);
             if (r) return r;
         }
         int r = build_work(c, s, PIC_PUSH_CHUNK);

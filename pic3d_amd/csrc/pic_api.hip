@@ -307,6 +307,12 @@ int pic_ctx_create(const pic_params* p, int device, pic_ctx** out) {
     return PIC_OK;
 }
 
+int pic_ctx_params(const pic_ctx* c, pic_params* out) {
+    if (!c || !out) return PIC_ERR_ARG;
+    *out = c->p;
+    return PIC_OK;
+}
+
 int pic_ctx_destroy(pic_ctx* c) {
     if (!c) return PIC_OK;
     hipSetDevice(c->device);
@@ -319,6 +325,7 @@ int pic_ctx_destroy(pic_ctx* c) {
     for (int f = 0; f < 3; f++) if (c->J[f]) hipFree(c->J[f]);
     dfree(c->tile_cnt); dfree(c->tile_cur); dfree(c->work); dfree(c->nwork); dfree(c->diag_buf);
     dfree(c->jtmp);
+    dfree(c->stage);
     diff_free(c);
     for (auto& e : c->tev) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -374,7 +381,7 @@ int pic_set_option(pic_ctx* c, int option, int64_t value) {
                 for (int s = 0; s < PIC_MAX_SPECIES && !r; s++)
                     if (c->sp[s].declared) {
                         dfree(c->sp[s].tile_off);
-                        r = dmalloc(c, &c->sp[s].tile_off, c->tl.ntiles + 1);
+                        r = dmalloc(c, &c->sp[s].tile_off, c->tl.ntiles + 2);
                     }
                 if (r) return r;
             }
@@ -392,7 +399,7 @@ int pic_set_option(pic_ctx* c, int option, int64_t value) {
             return PIC_OK;
         }
         case PIC_OPT_ADJ_PUSH:
-            if (value < 0 || value > 2) return set_err(c, PIC_ERR_ARG, "adjoint push kernel must be 0, 1 or 2");
+            if (value < 0 || value > 3) return set_err(c, PIC_ERR_ARG, "adjoint push kernel must be 0, 1, 2 or 3");
             c->adj_push = (int)value;
             return PIC_OK;
         case PIC_OPT_SORT_INTERVAL:
@@ -419,7 +426,7 @@ int pic_species_set(pic_ctx* c, int s, double charge, double mass, int immobile,
     S.cap = immobile ? 0 : capacity;
     int r = PIC_OK;
     if (!r) r = dmalloc(c, &S.d_cnt, CNT_N);
-    if (!r) r = dmalloc(c, &S.tile_off, c->tl.ntiles + 1);
+    if (!r) r = dmalloc(c, &S.tile_off, c->tl.ntiles + 2);
     if (!r) r = launch_fill(c, S.d_cnt, CNT_N * 8);
     for (int b = 0; b < 2 && !r && S.cap > 0; b++) {
         if (c->fp32) {

@@ -61,7 +61,8 @@ struct Species {
     // [2..7] lost per face, [8..13] wrapped per face (cumulative),
     // [14..19] lost and [20..25] wrapped since the last boundary call
     int64_t* d_cnt = nullptr;
-    int64_t* tile_off = nullptr;  // ntiles + 1 tile offsets after the last sort
+    int64_t* tile_off = nullptr;  // ntiles + 1 tile offsets after the last sort (+1: the
+                                  // end of the removed slots a keep-dead sort appends)
     int sorted = 0;               // tile_off valid for the current order
 };
 
@@ -101,6 +102,8 @@ struct pic_ctx {
     int keep_old = 0;
     int fix32 = 0;                // FP32: int32 fixed-point LDS accumulators
     void* jtmp = nullptr;         // J-sized scratch of the current filter (lazy)
+    void* stage = nullptr;        // output staging buffer (lazy, k_stage.hip)
+    size_t stage_bytes = 0;
     // x-min envelope override of the laser source (differentiated runs)
     int env_on = 0;
     double env_val = 0.0;
@@ -146,7 +149,11 @@ int launch_deposit(pic_ctx* c, int s, int64_t n, const double* const* d6, const 
 void diff_free(pic_ctx* c);
 int launch_push(pic_ctx* c, int s);
 int launch_boundary(pic_ctx* c, int s);
-int launch_sort(pic_ctx* c, int s);
+// Counting sort of species s by tile.  perm == null: removed particles are
+// dropped.  perm != null (differentiated runs): removed slots are kept
+// after the last tile, so the slot count is unchanged, and perm[q] = the
+// slot that moved to slot q.
+int launch_sort(pic_ctx* c, int s, int32_t* perm = nullptr);
 int build_work(pic_ctx* c, int s, int chunk);
 int launch_loader(pic_ctx* c, int s, const pic_loader* l, int64_t n, const int64_t* d_prefix);
 int launch_diag(pic_ctx* c, pic_diag_out* out);

@@ -171,23 +171,25 @@ def diff_small(n: int = 64, ppc: int = 8, steps: int = 200, seed: int = 3) -> Wo
                     notes=f"{n}^3 laser-slab, lambda 16 cells, {ppc} ppc in x[{n / 4},{7 * n / 8}), mem 2")
 
 
-def diff_ramp(nx: int = 128, ny: int = 64, nz: int = 64, ppc: int = 8, steps: int = 400,
+def diff_ramp(nx: int = 128, ny: int = 64, nz: int = 64, ppc_target: int = 57, steps: int = 400,
               seed: int = 4, precision: int = PIC_FP32) -> Workload:
     """Differentiable case at size: quartic shapes, the ramp-target density
     profile scaled to the box (target 20 n_c over nx/16 cells behind an
-    exponential ramp of scale nx/12.8 cells, cut at 0.01 n_c), ppc at the
-    target density, a0 = 5."""
+    exponential ramp of scale nx/12.8 cells, cut at 0.01 n_c), equal-weight
+    electrons sampled with density, a0 = 5.  ppc_target = 57 at the target
+    gives 8 electrons per cell on average (4.17M at 128x64x64)."""
     p = reference_params(nx, ny, nz)
     p.mem = 4
     p.precision = precision
     x_t0 = 0.75 * nx
     _laser(p, 5.0, 20.0, x_t0)
     e = Species(-1.0, 1.0, loader=make_loader(
-        PIC_PROFILE_RAMP_TARGET, ppc, 20.0, 0.005, seed,
+        PIC_PROFILE_RAMP_TARGET, ppc_target, 20.0, 0.005, seed,
         x0=x_t0, x1=x_t0 + nx / 16.0, scale_len=nx / 12.8, cutoff=0.01))
     ions = Species(1.0, 1836.0, immobile=True)
     return Workload("diff_ramp", p, [e, ions], steps,
-                    notes=f"{nx}x{ny}x{nz} ramp+target, {ppc} ppc at the target, mem 4")
+                    notes=f"{nx}x{ny}x{nz} ramp+target, {ppc_target} ppc at the target "
+                          f"(8 on average), mem 4")
 
 
 WORKLOADS = {"thermal": thermal, "laser_slab": laser_slab, "ramp_target": ramp_target,

@@ -112,14 +112,17 @@ extern const double alpha0 = 55.0 / 96.0, alpha1 = 5.0 / 24.0, alpha2 = 5.0 / 4.
 extern const int g_data_size = 2849;
 double g_data[2849];
 
-// ---- recorders in place of io_utils.cpp ----
+// ---- recorders in place of io_utils.cpp (COMPAT_MAIN_HDF5: the real
+// writers of libpic3d_compat_io.so, and the recorder counts nothing) ----
 static int g_outputs = 0;
+#ifndef COMPAT_MAIN_HDF5
 void output_data_hdf5(int step, int, int, MPI_Comm, hid_t) {
     (void)step;
     g_outputs++;
 }
 void output_particles_hdf5(int, int, int, MPI_Comm, const std::vector<std::vector<particle>>&,
                            const int*) {}
+#endif
 
 static void write3(FILE* f, double*** a, int lo, int hiz, int hiy, int hix) {
     for (int k = lo; k <= hiz; k++)

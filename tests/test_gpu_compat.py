@@ -27,7 +27,7 @@ STEP_FIXTURES = [n for n in golden_names() if n.startswith(("thermal", "laser"))
 PRE_FIXTURES = [n for n in golden_names() if n.startswith("preloop")]
 
 
-def run_driver(g, steps, output_interval):
+def run_driver(g, steps, output_interval, driver=DRIVER, cwd=None):
     p = g["params"]
     kv = {"nx": p.nx, "ny": p.ny, "nz": p.nz, "mem": p.mem, "px": p.periodic[0],
           "py": p.periodic[1], "pz": p.periodic[2], "steps": steps,
@@ -52,8 +52,8 @@ def run_driver(g, steps, output_interval):
                     np.ascontiguousarray(pp[k], np.float64).tofile(f)
             if n:
                 np.ascontiguousarray(pp["id"], np.int64).tofile(f)
-    r = subprocess.run([DRIVER, fin, fout], env=dict(os.environ, PIC_COMPAT_CFG=cfg),
-                       capture_output=True, text=True, timeout=300)
+    r = subprocess.run([driver, fin, fout], env=dict(os.environ, PIC_COMPAT_CFG=cfg),
+                       capture_output=True, text=True, timeout=300, cwd=cwd)
     assert r.returncode == 0, r.stderr[-2000:]
     raw = open(fout, "rb").read()
     m = p.mem
@@ -106,3 +106,46 @@ def test_compat_preloop_order(name):
         assert np.array_equal(pd[k], ref[k]), k
     for k in ("jx", "jy", "jz", "rho0"):
         assert rel_err(fields[k], g["fields"][k]) <= 1e-12, k
+
+
+DRIVER_H5 = os.path.join(ROOT, "tests", "compat", "bin", "compat_main_h5")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER_H5), reason="built only where the image's HDF5 is")
+@pytest.mark.parametrize("name", [n for n in STEP_FIXTURES if n.startswith("thermal")][:1] +
+                         [n for n in STEP_FIXTURES if n.startswith("laser")][:1])
+def test_compat_hdf5_outputs(name):
+    """main.cpp's sequence with the real writers (libpic3d_compat_io): the
+    files of every output step exist; the last one holds the final state
+    (fields staged from the device, particles from the vectors)."""
+    import _h5
+    from pic3d_amd.io import PARTICLE_DTYPE
+    print(f"compat hdf5 {name}", flush=True)
+    g = load_golden(name)
+    steps = int([c for c in g["cmds"] if c.startswith("steps:")][0].split(":")[2])
+    every = 2 if steps % 2 == 0 else 1
+    td = tempfile.mkdtemp(prefix="compat_h5_")
+    fields, pd, _, stdout = run_driver(g, steps, every, driver=DRIVER_H5, cwd=td)
+    p, m = g["params"], g["params"].mem
+    for st in range(0, steps + 1, every):
+        assert os.path.exists(os.path.join(td, "output", f"output_step_{st:04d}.h5")), st
+    assert os.path.exists(os.path.join(td, "output", "particles_step_0000.h5"))
+    with _h5.File(os.path.join(td, "output", f"output_step_{steps:04d}.h5")) as f:
+        for k in ("ex", "ey", "ez", "bx", "by", "bz"):
+            ref = fields[k][m:m + p.nz, m:m + p.ny, m:m + p.nx]
+            assert np.array_equal(f.read("/" + k), ref), k
+        for k, src in (("jx", "jx"), ("jy", "jy"), ("jz", "jz"), ("rho_electron", "rho0"),
+                       ("rho_proton", "rho1")):
+            ref = fields[src][m + 1:m + 1 + p.nz, m + 1:m + 1 + p.ny, m + 1:m + 1 + p.nx]
+            assert np.array_equal(f.read("/" + k), ref), k
+        assert np.array_equal(f.read("/rho"), f.read("/rho_electron") + f.read("/rho_proton"))
+    with _h5.File(os.path.join(td, "output", f"particles_step_{steps:04d}.h5")) as f:
+        rec = f.read("/particles", PARTICLE_DTYPE)
+        assert f.int_attr("/particles", "timestep") == steps
+        assert f.int_attr("/particles", "total_particles") == len(pd["x"])
+        assert f.int_attr("/particles", "num_processes") == 1
+    assert np.array_equal(rec["id"], pd["id"].astype(np.int32))
+    for k, src in (("x", "x"), ("y", "y"), ("z", "z"), ("x_old", "x_old"), ("px", "ux"),
+                   ("py", "uy"), ("pz", "uz"), ("weight", "w"), ("gamma", "gamma")):
+        assert np.array_equal(rec[k], pd[src]), k
+    assert f"Particle data written to output/particles_step_{steps:04d}.h5" in stdout
