@@ -166,41 +166,47 @@ def run_gpu(args, precision: str) -> dict:
 
 
 def run_adjoint(args) -> dict:
-    """Differentiation form: gradient of the final electron kinetic energy
-    with respect to every step's x-min laser envelope value and every
-    initial weight (FP64), on the laser-slab case at 1/8 of its cells
-    (128x64x64, same shapes and ppc, the slab moved to x in [8, 88) so the pulse reaches it within the run: 5.2M electrons).
-    Times one plain forward step (tiled push, re-sorted every step), the
-    differentiated forward (checkpoint every K steps) and the backward
-    (recompute + adjoint kernels), per step."""
+    """Differentiation form on the differentiable case at size: 128x64x64
+    cells, 8 electrons per cell on average (4.17M) over an exponential ramp
+    onto a 20 n_c target, quartic shapes, a0 = 5, gradient of the final
+    electron kinetic energy with respect to every step's x-min envelope
+    value and every initial weight, checkpoint every 40 steps.  For each
+    precision: the plain forward step (tiled push, re-sorted every step),
+    the differentiated forward and the backward (recompute + adjoint
+    kernels), per step, over the same steps; the FP32 objective against the
+    FP64 one."""
     from pic3d_amd import Simulation, workloads as W
+    from pic3d_amd._abi import PIC_FP32, PIC_FP64
     from pic3d_amd.diff import DiffRun
     from pic3d_amd.simulation import setup
-    wl = W.laser_slab(nx=128, ny=64, nz=64, slab=(8.0, 88.0))
-    sim = Simulation(wl.params)
-    setup(sim, wl, device_loader=True)
-    sim.preloop()
-    sim.step(1, 2)
-    sim.sync()
-    t = time.perf_counter()
-    sim.step(3, 5)
-    sim.sync()
-    fwd = (time.perf_counter() - t) / 5
-    nsteps, K = args.adjoint_steps, 10
-    run = DiffRun(sim, 0, first_step=8, nsteps=nsteps, checkpoint_every=K)
-    t = time.perf_counter()
-    obj = run.forward()
-    tf = (time.perf_counter() - t) / nsteps
-    t = time.perf_counter()
-    ge, gw = run.backward()
-    tb = (time.perf_counter() - t) / nsteps
-    run.close()
-    sim.close()
-    return {"workload": "laser_slab 128x64x64, mem 4, 16 ppc in x[8,88) (5.2M e-)",
-            "steps": nsteps, "checkpoint_every": K, "objective": obj,
-            "forward_ms_per_step": fwd * 1e3, "diff_forward_ms_per_step": tf * 1e3,
-            "backward_ms_per_step": tb * 1e3, "backward_over_forward": tb / fwd,
-            "grad_envelope_norm": float((ge ** 2).sum() ** 0.5)}
+    nsteps, K = args.adjoint_steps, 40
+    out = {"workload": "diff_ramp 128x64x64, mem 4, 8 ppc on average (4.17M e-), a0 5",
+           "steps": nsteps, "checkpoint_every": K}
+    for name, prec in (("fp64", PIC_FP64), ("fp32", PIC_FP32)):
+        wl = W.diff_ramp(precision=prec)
+        sim = Simulation(wl.params)
+        setup(sim, wl, device_loader=True)
+        run = DiffRun(sim, 0, first_step=1, nsteps=nsteps, checkpoint_every=K)
+        sim.sync()
+        t = time.perf_counter()
+        sim.step(1, nsteps)  # the plain forward over the same steps
+        sim.sync()
+        fwd = (time.perf_counter() - t) / nsteps
+        t = time.perf_counter()
+        obj = run.forward()  # restarts from the run's initial state
+        tf = (time.perf_counter() - t) / nsteps
+        t = time.perf_counter()
+        ge, gw = run.backward()
+        tb = (time.perf_counter() - t) / nsteps
+        out[name] = {"objective": obj, "forward_ms_per_step": fwd * 1e3,
+                     "diff_forward_ms_per_step": tf * 1e3, "backward_ms_per_step": tb * 1e3,
+                     "backward_over_forward": tb / fwd,
+                     "grad_envelope_norm": float((ge ** 2).sum() ** 0.5)}
+        run.close()
+        sim.close()
+    out["fp32_objective_rel_err"] = abs(out["fp32"]["objective"] - out["fp64"]["objective"]) / \
+        abs(out["fp64"]["objective"])
+    return out
 
 
 def copy_probe(nbytes: int):
@@ -231,7 +237,7 @@ def main():
     ap.add_argument("--ramp-steps", type=int, default=5,
                     help="timed steps of the FP32 ramp-target throughput case reported beside "
                          "the headline (0: skip it)")
-    ap.add_argument("--adjoint-steps", type=int, default=20,
+    ap.add_argument("--adjoint-steps", type=int, default=400,
                     help="steps of the differentiated run (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (read BYTES, write BYTES) "

@@ -161,9 +161,7 @@ enum pic_option {
     PIC_OPT_ADJ_PUSH = 6       /* differentiated runs: 0 (default) the LDS-tiled
                                   adjoint push (HBM kernel after a direct forward);
                                   1 dual-number kernel (FP64; FP32 takes 2);
-                                  2 separable HBM kernel; 3 (FP32) the
-                                  LDS-tiled kernel with fp32 lambda_F
-                                  accumulators (smaller LDS footprint) */
+                                  2 separable HBM kernel */
 };
 int pic_set_option(pic_ctx* ctx, int option, int64_t value);
 

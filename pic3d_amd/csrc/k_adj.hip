@@ -1162,10 +1162,19 @@ struct DiffState {
     std::vector<char*> ck;
     size_t ck_bytes = 0;
     char* fin = nullptr;             // the state at the end of the forward
-    // per-step buffers of the segment being reversed
-    char* seg = nullptr;
+    // per-step buffers of the segments being recomputed and reversed
+    // (double-buffered: segment k - 1 is recomputed on the context's stream
+    // while the adjoint of segment k runs on the adjoint stream)
+    char* segb[2] = {};
     int seg_steps = 0;
     size_t step_bytes = 0;
+    hipStream_t as = nullptr;        // the adjoint stream (default priority)
+    hipEvent_t ev_rec[2] = {};       // segment buffer b recomputed
+    hipEvent_t ev_adj[2] = {};       // segment buffer b reversed (free again)
+    hipEvent_t ev_obj = nullptr;     // the final-state adjoint taken
+    int64_t* atoff = nullptr;        // the adjoint's tile offsets and work list
+    int64_t* awork = nullptr;
+    int64_t* anwork = nullptr;
     // adjoint state, in the precision of the context
     void* lF[6] = {};
     void* lJ[3] = {};
@@ -1188,7 +1197,7 @@ namespace {
 size_t up256(size_t b) { return (b + 255) / 256 * 256; }
 
 int dalloc(pic_ctx* c, void** p, size_t bytes) {
-    if (hipMalloc(p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+    if (hipMalloc(p, std::max<size_t>(up256(bytes), 256)) != hipSuccess) {
         *p = nullptr;
         return set_err(c, PIC_ERR_NOMEM, "differentiated run: device allocation of %zu bytes failed", bytes);
     }
@@ -1196,6 +1205,20 @@ int dalloc(pic_ctx* c, void** p, size_t bytes) {
 }
 
 int fill0(pic_ctx* c, void* p, size_t bytes) { return launch_fill(c, p, (int64_t)bytes); }
+
+__global__ __launch_bounds__(256) void k_adj_zero(uint4* __restrict__ p, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// zero 'bytes' of a dalloc'd buffer (whole 16-byte words) on the adjoint stream
+int zero_adj(pic_ctx* c, DiffState& D, void* p, size_t bytes) {
+    const int64_t n16 = (int64_t)((bytes + 15) / 16);
+    if (n16 == 0) return PIC_OK;
+    k_adj_zero<<<(int)std::min<int64_t>(blocks_for(n16, 256), 8192), 256, 0, D.as>>>((uint4*)p, n16);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
 
 // particle column f of species s (current buffer), in snapshot order
 void* pcol(pic_ctx* c, int s, int f) {
@@ -1354,13 +1377,13 @@ int diff_step(pic_ctx* c, DiffState& D, int st, char* save, bool replay) {
     return pic_fields_second(c);
 }
 
-template <typename T> int adj_wrap3(pic_ctx* c, void* const* a) {
+template <typename T> int adj_wrap3(pic_ctx* c, DiffState& D, void* const* a) {
     const Grid& g = c->g;
     for (int ax = 2; ax >= 0; ax--) {
         if (!c->p.periodic[ax]) continue;
         const int e0 = ax == 0 ? g.fe[1] : g.fe[0], e1 = ax == 2 ? g.fe[1] : g.fe[2];
         const int64_t total = (int64_t)2 * g.m * e0 * e1;
-        k_adj_wrap<T><<<blocks_for(total, 256), 256, 0, c->stream>>>((T*)a[0], (T*)a[1], (T*)a[2], g, ax);
+        k_adj_wrap<T><<<blocks_for(total, 256), 256, 0, D.as>>>((T*)a[0], (T*)a[1], (T*)a[2], g, ax);
         PIC_LAUNCH_CHECK(c);
     }
     return PIC_OK;
@@ -1370,7 +1393,7 @@ template <typename T> int adj_e(pic_ctx* c, DiffState& D, double coef) {
     const Grid& g = c->g;
     T* const* l = (T* const*)D.lF;
     T* const* j = (T* const*)D.lJ;
-    k_adj_e<T><<<blocks_for(g.fsize, 256), 256, 0, c->stream>>>(l[0], l[1], l[2], l[3], l[4], l[5], j[0], j[1],
+    k_adj_e<T><<<blocks_for(g.fsize, 256), 256, 0, D.as>>>(l[0], l[1], l[2], l[3], l[4], l[5], j[0], j[1],
                                                                  j[2], g, (T)coef, (T)c->p.dx, (T)c->p.dy,
                                                                  (T)c->p.dz);
     PIC_LAUNCH_CHECK(c);
@@ -1380,7 +1403,7 @@ template <typename T> int adj_e(pic_ctx* c, DiffState& D, double coef) {
 template <typename T> int adj_b(pic_ctx* c, DiffState& D, double coef) {
     const Grid& g = c->g;
     T* const* l = (T* const*)D.lF;
-    k_adj_b<T><<<blocks_for(g.fsize, 256), 256, 0, c->stream>>>(l[0], l[1], l[2], l[3], l[4], l[5], g, (T)coef,
+    k_adj_b<T><<<blocks_for(g.fsize, 256), 256, 0, D.as>>>(l[0], l[1], l[2], l[3], l[4], l[5], g, (T)coef,
                                                                  (T)c->p.dx, (T)c->p.dy, (T)c->p.dz);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
@@ -1393,7 +1416,7 @@ template <typename T> int adj_fold(pic_ctx* c, DiffState& D) {
             if (!c->p.periodic[ax]) continue;
             const int e0 = ax == 0 ? g.je[1] : g.je[0], e1 = ax == 2 ? g.je[1] : g.je[2];
             const int64_t total = (int64_t)(g.m + 1) * e0 * e1;
-            k_adj_fold<T><<<blocks_for(total, 256), 256, 0, c->stream>>>((T*)D.lJ[0], (T*)D.lJ[1], (T*)D.lJ[2], g,
+            k_adj_fold<T><<<blocks_for(total, 256), 256, 0, D.as>>>((T*)D.lJ[0], (T*)D.lJ[1], (T*)D.lJ[2], g,
                                                                          ax, phase);
             PIC_LAUNCH_CHECK(c);
         }
@@ -1456,17 +1479,17 @@ int launch_adj_tiled(pic_ctx* c, DiffState& D, const V& v) {
     using TS = TileShape<KIND, M>;
     using B = AdjBox<M, TS::X, TS::Y, TS::Z, typename V::T, LACC>;
     Species& S = c->sp[D.s];
-    int r = build_work(c, D.s, PIC_PUSH_CHUNK);
+    int r = build_work_on(c, D.atoff, PIC_PUSH_CHUNK, D.awork, D.anwork, D.as);
     if (r) return r;
     PIC_HIP(c, hipFuncSetAttribute((const void*)k_adj_push_tiled<M, TS::X, TS::Y, TS::Z, V, LACC>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
     const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
     int64_t* cnt = D.stray + std::max<int64_t>(D.slots, 1);
-    if ((r = launch_fill(c, cnt, 16))) return r;
-    k_adj_push_tiled<M, TS::X, TS::Y, TS::Z, V, LACC><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
-        v, c->g, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, D.stray, cnt);
+    if ((r = zero_adj(c, D, cnt, 16))) return r;
+    k_adj_push_tiled<M, TS::X, TS::Y, TS::Z, V, LACC><<<(unsigned)max_items, 256, B::LDS, D.as>>>(
+        v, c->g, c->tl, D.atoff, D.awork, D.anwork, PIC_PUSH_CHUNK, D.stray, cnt);
     PIC_LAUNCH_CHECK(c);
-    k_adj_push_hbm<M, V><<<1024, 256, 0, c->stream>>>(v, c->g, D.stray, cnt);
+    k_adj_push_hbm<M, V><<<1024, 256, 0, D.as>>>(v, c->g, D.stray, cnt);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
@@ -1478,37 +1501,32 @@ int launch_adj_tiled(pic_ctx* c, DiffState& D, const V& v) {
 int launch_adj_push(pic_ctx* c, DiffState& D, const char* save) {
     const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(D.slots, 1), 256), 8192);
     const bool q = c->g.m == 2;
-    const bool tiled = D.kernel == 2 && (c->adj_push == 0 || (c->adj_push == 3 && D.f32));
+    const bool tiled = D.kernel == 2 && c->adj_push == 0;
     if (tiled && tile_kind(c) != D.kind)
         return set_err(c, PIC_ERR_STATE, "differentiated run: the deposit accumulator option changed during the run");
     if (D.f32) {
         AdjView32 v{adj_view32(c, D, save), adj_consts32(c, c->sp[D.s]), {c->g.n[0], c->g.n[1], c->g.n[2]}, c->g.m};
-        if (tiled && c->adj_push == 3) {  // fp32 lambda_F accumulators in LDS
-            if (D.kind == 2) return q ? launch_adj_tiled<2, 2, AdjView32, float>(c, D, v)
-                                      : launch_adj_tiled<4, 2, AdjView32, float>(c, D, v);
-            return q ? launch_adj_tiled<2, 1, AdjView32, float>(c, D, v) : launch_adj_tiled<4, 1, AdjView32, float>(c, D, v);
-        }
         if (tiled) {
             if (D.kind == 2) return q ? launch_adj_tiled<2, 2>(c, D, v) : launch_adj_tiled<4, 2>(c, D, v);
             return q ? launch_adj_tiled<2, 1>(c, D, v) : launch_adj_tiled<4, 1>(c, D, v);
         }
-        if (q) k_adj_push_hbm<2><<<blocks, 256, 0, c->stream>>>(v, c->g, nullptr, nullptr);
-        else k_adj_push_hbm<4><<<blocks, 256, 0, c->stream>>>(v, c->g, nullptr, nullptr);
+        if (q) k_adj_push_hbm<2><<<blocks, 256, 0, D.as>>>(v, c->g, nullptr, nullptr);
+        else k_adj_push_hbm<4><<<blocks, 256, 0, D.as>>>(v, c->g, nullptr, nullptr);
         PIC_LAUNCH_CHECK(c);
         return PIC_OK;
     }
     const PushConsts K = push_consts64(c, c->sp[D.s]);
     const AdjPush A = adj_view(c, D, save);
     if (c->adj_push == 1) {
-        if (q) k_adj_push64<2><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
-        else k_adj_push64<4><<<blocks, 256, 0, c->stream>>>(A, c->g, K);
+        if (q) k_adj_push64<2><<<blocks, 256, 0, D.as>>>(A, c->g, K);
+        else k_adj_push64<4><<<blocks, 256, 0, D.as>>>(A, c->g, K);
         PIC_LAUNCH_CHECK(c);
         return PIC_OK;
     }
     AdjView64 v{A, K, adj_consts64(K), c->g.m};
     if (tiled) return q ? launch_adj_tiled<2, 0>(c, D, v) : launch_adj_tiled<4, 0>(c, D, v);
-    if (q) k_adj_push_hbm<2><<<blocks, 256, 0, c->stream>>>(v, c->g, nullptr, nullptr);
-    else k_adj_push_hbm<4><<<blocks, 256, 0, c->stream>>>(v, c->g, nullptr, nullptr);
+    if (q) k_adj_push_hbm<2><<<blocks, 256, 0, D.as>>>(v, c->g, nullptr, nullptr);
+    else k_adj_push_hbm<4><<<blocks, 256, 0, D.as>>>(v, c->g, nullptr, nullptr);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
@@ -1520,14 +1538,14 @@ template <typename T> int adj_step_t(pic_ctx* c, DiffState& D, int st, const cha
     void* lB[3] = {D.lF[3], D.lF[4], D.lF[5]};
     int r;
     // fields_second: B then E
-    if ((r = adj_wrap3<T>(c, lE)) || (r = adj_e<T>(c, D, h))) return r;
-    if ((r = adj_wrap3<T>(c, lB)) || (r = adj_b<T>(c, D, h))) return r;
+    if ((r = adj_wrap3<T>(c, D, lE)) || (r = adj_e<T>(c, D, h))) return r;
+    if ((r = adj_wrap3<T>(c, D, lB)) || (r = adj_b<T>(c, D, h))) return r;
     // fold, boundary (dead slots carry no adjoint), push
     if ((r = adj_fold<T>(c, D))) return r;
     if (D.kernel == 2) {  // the tile grouping of this step's push
         const size_t pb = up256(D.slots * sizeof(T)), fb = up256(c->g.fsize * sizeof(T));
-        PIC_HIP(c, hipMemcpyAsync(c->sp[D.s].tile_off, save + D.ncol() * pb + 6 * fb,
-                                  (size_t)(c->tl.ntiles + 2) * 8, hipMemcpyDeviceToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(D.atoff, save + D.ncol() * pb + 6 * fb, (size_t)(c->tl.ntiles + 2) * 8,
+                                  hipMemcpyDeviceToDevice, D.as));
     }
     if ((r = launch_adj_push(c, D, save))) return r;
     // the sort before the push
@@ -1537,7 +1555,7 @@ template <typename T> int adj_step_t(pic_ctx* c, DiffState& D, int st, const cha
         DiffLam<T> a{{(T*)D.lp[0], (T*)D.lp[1], (T*)D.lp[2], (T*)D.lp[3], (T*)D.lp[4], (T*)D.lp[5]}, D.lw};
         DiffLam<T> b{{(T*)D.lp_tmp[0], (T*)D.lp_tmp[1], (T*)D.lp_tmp[2], (T*)D.lp_tmp[3], (T*)D.lp_tmp[4],
                       (T*)D.lp_tmp[5]}, D.lw_tmp};
-        k_adj_unpermute<T><<<(int)std::min<int64_t>(blocks_for(D.slots, 256), 8192), 256, 0, c->stream>>>(
+        k_adj_unpermute<T><<<(int)std::min<int64_t>(blocks_for(D.slots, 256), 8192), 256, 0, D.as>>>(
             a, b, perm, D.slots);
         PIC_LAUNCH_CHECK(c);
         for (int f = 0; f < 6; f++) std::swap(D.lp[f], D.lp_tmp[f]);
@@ -1546,10 +1564,10 @@ template <typename T> int adj_step_t(pic_ctx* c, DiffState& D, int st, const cha
     // clear: the currents before it get their adjoint from this step's
     // E half step and laser boundary only
     for (int f = 0; f < 3; f++)
-        if ((r = fill0(c, D.lJ[f], c->g.jsize * sizeof(T)))) return r;
+        if ((r = zero_adj(c, D, D.lJ[f], c->g.jsize * sizeof(T)))) return r;
     // fields_first: E then B
-    if ((r = adj_wrap3<T>(c, lB)) || (r = adj_b<T>(c, D, h))) return r;
-    if ((r = adj_wrap3<T>(c, lE)) || (r = adj_e<T>(c, D, h))) return r;
+    if ((r = adj_wrap3<T>(c, D, lB)) || (r = adj_b<T>(c, D, h))) return r;
+    if ((r = adj_wrap3<T>(c, D, lE)) || (r = adj_e<T>(c, D, h))) return r;
     // laser boundary (x open only)
     if (!c->p.periodic[0]) {
         LaserArgs L = laser_args(c, st, 1, c->p.laser_right);
@@ -1557,7 +1575,7 @@ template <typename T> int adj_step_t(pic_ctx* c, DiffState& D, int st, const cha
         L.env_val = D.env.empty() ? 0.0 : D.env[st - D.first];
         dim3 grid(blocks_for(c->g.n[1], 256), c->g.n[2]);
         T* const* l = (T* const*)D.lF;
-        k_adj_laser<T><<<grid, 256, 0, c->stream>>>(l[1], l[2], l[3], l[4], l[5], (T*)D.lJ[1], (T*)D.lJ[2], c->g, L,
+        k_adj_laser<T><<<grid, 256, 0, D.as>>>(l[1], l[2], l[3], l[4], l[5], (T*)D.lJ[1], (T*)D.lJ[2], c->g, L,
                                                     D.genv + (st - D.first));
         PIC_LAUNCH_CHECK(c);
     }
@@ -1581,9 +1599,9 @@ void free_adjoint(DiffState& D) {
     D.genv = nullptr;
     if (D.stray) hipFree(D.stray);
     D.stray = nullptr;
-    if (D.seg) hipFree(D.seg);
-    D.seg = nullptr;
+    for (auto& b : D.segb) { if (b) hipFree(b); b = nullptr; }
     D.seg_steps = 0;
+    for (int64_t** p : {&D.atoff, &D.awork, &D.anwork}) { if (*p) hipFree(*p); *p = nullptr; }
 }
 
 void free_checkpoints(DiffState& D, bool keep_initial) {
@@ -1606,9 +1624,16 @@ void diff_free(pic_ctx* c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     DiffState& D = *c->diff;
+    if (D.as) hipStreamSynchronize(D.as);
     free_adjoint(D);
     free_checkpoints(D, false);
     if (D.red) hipFree(D.red);
+    for (int b = 0; b < 2; b++) {
+        if (D.ev_rec[b]) hipEventDestroy(D.ev_rec[b]);
+        if (D.ev_adj[b]) hipEventDestroy(D.ev_adj[b]);
+    }
+    if (D.ev_obj) hipEventDestroy(D.ev_obj);
+    if (D.as) hipStreamDestroy(D.as);
     c->push_kernel = D.saved_push_kernel;
     c->sort_interval = D.saved_sort_interval;
     delete c->diff;
@@ -1659,6 +1684,16 @@ int pic_diff_begin(pic_ctx* c, int s) {
         r = copy_state(c, D, D.ck[0], true);
     }
     if (!r) r = dalloc(c, (void**)&D.red, 256);
+    if (!r && hipStreamCreateWithFlags(&D.as, hipStreamNonBlocking) != hipSuccess) {
+        D.as = nullptr;
+        r = set_err(c, PIC_ERR_HIP, "pic_diff_begin: adjoint stream");
+    }
+    for (int b = 0; b < 2 && !r; b++)
+        if (hipEventCreateWithFlags(&D.ev_rec[b], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&D.ev_adj[b], hipEventDisableTiming) != hipSuccess)
+            r = set_err(c, PIC_ERR_HIP, "pic_diff_begin: events");
+    if (!r && hipEventCreateWithFlags(&D.ev_obj, hipEventDisableTiming) != hipSuccess)
+        r = set_err(c, PIC_ERR_HIP, "pic_diff_begin: events");
     if (!r && hipStreamSynchronize(c->stream) != hipSuccess) r = set_err(c, PIC_ERR_HIP, "pic_diff_begin failed");
     if (r) diff_free(c);
     return r;
@@ -1816,10 +1851,14 @@ int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, 
     for (int f = 0; f < 6 && !r && sorts; f++) r = dalloc(c, &D.lp_tmp[f], ns * es);
     if (!r && sorts) r = dalloc(c, (void**)&D.lw_tmp, ns * 8);
     if (!r) r = dalloc(c, (void**)&D.genv, std::max(D.nsteps, 1) * 8);
-    if (!r) r = dalloc(c, (void**)&D.stray, (ns + 1) * 8);
+    if (!r) r = dalloc(c, (void**)&D.stray, (ns + 2) * 8);  // slots + a 16-byte count
     D.step_bytes = step_state_bytes(c, D);
     D.seg_steps = std::min(D.K, std::max(D.nsteps, 1));
-    if (!r) r = dalloc(c, (void**)&D.seg, D.step_bytes * D.seg_steps);
+    const int nseg = (D.nsteps + D.K - 1) / D.K;
+    for (int b = 0; b < std::min(nseg, 2) && !r; b++) r = dalloc(c, (void**)&D.segb[b], D.step_bytes * D.seg_steps);
+    if (!r) r = dalloc(c, (void**)&D.atoff, (size_t)(c->tl.ntiles + 2) * 8);
+    if (!r) r = dalloc(c, (void**)&D.awork, (size_t)std::max<int64_t>(c->work_cap, 1) * 8);
+    if (!r) r = dalloc(c, (void**)&D.anwork, 8);
     for (int f = 0; f < 6 && !r; f++) r = fill0(c, D.lF[f], g.fsize * es);
     for (int f = 0; f < 3 && !r; f++) r = fill0(c, D.lJ[f], g.jsize * es);
     if (!r) r = fill0(c, D.genv, std::max(D.nsteps, 1) * 8);
@@ -1832,20 +1871,53 @@ int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, 
     else
         k_adj_objective<<<blocks_for(ns, 256), 256, 0, c->stream>>>(adj_view(c, D, nullptr), mdv, grad_objective);
     PIC_LAUNCH_CHECK(c);
+    PIC_HIP(c, hipEventRecord(D.ev_obj, c->stream));
+    PIC_HIP(c, hipStreamWaitEvent(D.as, D.ev_obj, 0));
     c->push_kernel = D.kernel;
     c->sort_interval = 1 << 30;
-    const int nseg = (D.nsteps + D.K - 1) / D.K;
+    // Segment k is recomputed on the context's stream into buffer k % 2 and
+    // reversed on the adjoint stream; segment k - 1 is recomputed while
+    // segment k is reversed (its buffer is free once segment k + 1 has been
+    // reversed).
+    auto seg_range = [&](int sg, int& a, int& b) {
+        a = D.first + sg * D.K;
+        b = std::min(a + D.K, D.first + D.nsteps);
+    };
+    auto recompute = [&](int sg) -> int {
+        int a, b;
+        seg_range(sg, a, b);
+        char* buf = D.segb[sg % 2];
+        if (sg + 2 <= nseg - 1) PIC_HIP(c, hipStreamWaitEvent(c->stream, D.ev_adj[sg % 2], 0));
+        int rr = copy_state(c, D, D.ck[sg], false);
+        for (int st = a; st < b && !rr; st++) rr = diff_step(c, D, st, buf + (size_t)(st - a) * D.step_bytes, true);
+        if (!rr) PIC_HIP(c, hipEventRecord(D.ev_rec[sg % 2], c->stream));
+        return rr;
+    };
+    auto reverse = [&](int sg) -> int {
+        int a, b;
+        seg_range(sg, a, b);
+        char* buf = D.segb[sg % 2];
+        PIC_HIP(c, hipStreamWaitEvent(D.as, D.ev_rec[sg % 2], 0));
+        int rr = PIC_OK;
+        for (int st = b - 1; st >= a && !rr; st--) rr = adj_step(c, D, st, buf + (size_t)(st - a) * D.step_bytes);
+        if (!rr) PIC_HIP(c, hipEventRecord(D.ev_adj[sg % 2], D.as));
+        return rr;
+    };
+    if (nseg > 0) r = recompute(nseg - 1);
     for (int sg = nseg - 1; sg >= 0 && !r; sg--) {
-        const int a = D.first + sg * D.K, b = std::min(a + D.K, D.first + D.nsteps);
-        r = copy_state(c, D, D.ck[sg], false);
-        for (int st = a; st < b && !r; st++) r = diff_step(c, D, st, D.seg + (size_t)(st - a) * D.step_bytes, true);
-        for (int st = b - 1; st >= a && !r; st--) r = adj_step(c, D, st, D.seg + (size_t)(st - a) * D.step_bytes);
+        r = reverse(sg);
+        if (!r && sg > 0) r = recompute(sg - 1);
     }
+    const hipError_t e_as = hipStreamSynchronize(D.as);
     c->push_kernel = D.saved_push_kernel;
     c->sort_interval = D.saved_sort_interval;
+    if (!r && e_as != hipSuccess) r = set_err(c, PIC_ERR_HIP, "adjoint stream: %s", hipGetErrorString(e_as));
     // leave the context at the run's initial state
     if (!r) r = copy_state(c, D, D.ck[0], false);
-    if (r) return r;
+    if (r) {
+        hipStreamSynchronize(c->stream);
+        return r;
+    }
     if (grad_envelope && D.nsteps)
         PIC_HIP(c, hipMemcpyAsync(grad_envelope, D.genv, D.nsteps * 8, hipMemcpyDeviceToHost, c->stream));
     if (grad_weights && D.slots)

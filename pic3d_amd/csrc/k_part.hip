@@ -219,8 +219,12 @@ __global__ __launch_bounds__(1024) void k_work(const int64_t* __restrict__ off, 
 }
 
 int build_work(pic_ctx* c, int s, int chunk) {
-    k_work<<<1, 1024, 0, c->stream>>>(c->sp[s].tile_off, c->tl.ntiles, chunk, c->work, c->nwork,
-                                      c->work_cap);
+    return build_work_on(c, c->sp[s].tile_off, chunk, c->work, c->nwork, c->stream);
+}
+
+int build_work_on(pic_ctx* c, const int64_t* tile_off, int chunk, int64_t* work, int64_t* nwork,
+                  hipStream_t stream) {
+    k_work<<<1, 1024, 0, stream>>>(tile_off, c->tl.ntiles, chunk, work, nwork, c->work_cap);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }

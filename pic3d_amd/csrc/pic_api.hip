@@ -399,7 +399,7 @@ int pic_set_option(pic_ctx* c, int option, int64_t value) {
             return PIC_OK;
         }
         case PIC_OPT_ADJ_PUSH:
-            if (value < 0 || value > 3) return set_err(c, PIC_ERR_ARG, "adjoint push kernel must be 0, 1, 2 or 3");
+            if (value < 0 || value > 2) return set_err(c, PIC_ERR_ARG, "adjoint push kernel must be 0, 1 or 2");
             c->adj_push = (int)value;
             return PIC_OK;
         case PIC_OPT_SORT_INTERVAL:

@@ -155,6 +155,9 @@ int launch_boundary(pic_ctx* c, int s);
 // slot that moved to slot q.
 int launch_sort(pic_ctx* c, int s, int32_t* perm = nullptr);
 int build_work(pic_ctx* c, int s, int chunk);
+// the work list of a tile grouping into (work, nwork), capacity c->work_cap
+int build_work_on(pic_ctx* c, const int64_t* tile_off, int chunk, int64_t* work, int64_t* nwork,
+                  hipStream_t stream);
 int launch_loader(pic_ctx* c, int s, const pic_loader* l, int64_t n, const int64_t* d_prefix);
 int launch_diag(pic_ctx* c, pic_diag_out* out);
 int launch_fill(pic_ctx* c, void* p, int64_t bytes);

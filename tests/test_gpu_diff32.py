@@ -164,9 +164,8 @@ def test_diff_ramp_full_size():
 
 @pytest.mark.parametrize("mem", [2, 4])
 def test_fp32_adjoint_kernels_agree(mem):
-    """The FP32 adjoint push kernels -- LDS-tiled with FP64 or fp32
-    lambda_F accumulators, and the HBM kernel -- evaluate the same
-    derivatives: they agree to fp32 rounding."""
+    """The FP32 adjoint push kernels -- LDS-tiled and HBM -- evaluate the
+    same derivatives: they agree to fp32 rounding."""
     from pic3d_amd._abi import PIC_OPT_ADJ_PUSH
     print(f"diff32 kernels agree mem {mem}", flush=True)
     n_t = 8 if mem == 2 else 10
@@ -177,11 +176,11 @@ def test_fp32_adjoint_kernels_agree(mem):
     sim, run = make(wl, parts, PIC_FP32, 2, 25, 6)
     env0 = run.reference_envelope()
     out = {}
-    for k in (0, 2, 3):
+    for k in (0, 2):
         sim.set_option(PIC_OPT_ADJ_PUSH, k)
         run.forward(env0, None)
         out[k] = run.backward()
-    for k in (2, 3):
+    for k in (2,):
         for a, b in zip(out[k], out[0]):
             print(f"  kernel {k}: {rel(a, b):.2e}", flush=True)
             assert rel(a, b) <= 1e-5, (k, rel(a, b))
