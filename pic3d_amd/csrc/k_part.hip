@@ -22,7 +22,8 @@ struct TileArgs {
     Tiling tl;
 };
 
-__device__ __forceinline__ int tile_of64(double x, double y, double z, const TileArgs& A) {
+// sort bin of a particle: its tile
+__device__ __forceinline__ int64_t bin_of64(double x, double y, double z, const TileArgs& A) {
     double p[3] = {x, y, z};
     const bool out = x < 0.0 || x > A.N[0] || y < 0.0 || y > A.N[1] || z < 0.0 || z > A.N[2];
     int t[3];
@@ -50,12 +51,12 @@ __global__ __launch_bounds__(256) void k_hist64(Soa64 A, const int64_t* __restri
     for (int64_t b = base; b < n; b += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = b + threadIdx.x;
         bool valid = false;
-        int key = -1;
+        int64_t key = -1;
         if (p < n) {
             const double w = A.w[p];
             valid = (w == w);
-            if (valid) key = tile_of64(A.x[p], A.y[p], A.z[p], T);
-            else if (keep_dead) { valid = true; key = T.tl.ntiles; }
+            if (valid) key = bin_of64(A.x[p], A.y[p], A.z[p], T);
+            else if (keep_dead) { valid = true; key = T.tl.nbins(); }
         }
         int head, len;
         wave_runs(key, valid, head, len);
@@ -63,36 +64,105 @@ __global__ __launch_bounds__(256) void k_hist64(Soa64 A, const int64_t* __restri
     }
 }
 
-// exclusive scan of ntiles counts (single workgroup); writes off[0..ntiles],
-// the scatter cursors, and the new slot count.
-__global__ __launch_bounds__(1024) void k_scan_tiles(const int64_t* __restrict__ cnt_in,
-                                                      int64_t* __restrict__ off,
-                                                      int64_t* __restrict__ cur, int ntiles,
-                                                      int64_t* __restrict__ slot_cnt) {
-    __shared__ int64_t part[1024];
-    const int T = blockDim.x, t = threadIdx.x;
-    const int per = (ntiles + T - 1) / T;
-    const int lo = min(ntiles, t * per), hi = min(ntiles, lo + per);
-    int64_t s = 0;
-    for (int i = lo; i < hi; i++) s += cnt_in[i];
-    part[t] = s;
+// ---------------------------------------------------------------------
+// exclusive scan of the sort histogram (up to tens of millions of bins):
+// per-block scans of SCAN_B bins, a scan of the block sums, then the block
+// offsets added; the species' tile offsets are the bins' offsets.
+// ---------------------------------------------------------------------
+constexpr int SCAN_T = 1024, SCAN_E = 16;
+constexpr int64_t SCAN_B = (int64_t)SCAN_T * SCAN_E;
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(1024) void k_scan1(const int64_t* __restrict__ in, int64_t* __restrict__ out,
+                                                int64_t n, int64_t* __restrict__ sums) {
+    __shared__ int64_t wsum[SCAN_T / 64];
+    __shared__ int64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    const int lane = lane_id(), w = threadIdx.x / 64;
+    for (int e = 0; e < SCAN_E; e++) {
+        const int64_t i = (int64_t)blockIdx.x * SCAN_B + (int64_t)e * SCAN_T + threadIdx.x;
+        const int64_t v = i < n ? in[i] : 0;
+        const int64_t x = wave_incl_scan(v);
+        __syncthreads();
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        if (w == 0) {
+            int64_t t = lane < SCAN_T / 64 ? wsum[lane] : 0;
+            t = wave_incl_scan(t);
+            if (lane < SCAN_T / 64) wsum[lane] = t;
+        }
+        __syncthreads();
+        if (i < n) out[i] = carry + (w ? wsum[w - 1] : 0) + x - v;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += wsum[SCAN_T / 64 - 1];
+    }
     __syncthreads();
-    for (int d = 1; d < T; d <<= 1) {
-        int64_t v = t >= d ? part[t - d] : 0;
+    if (threadIdx.x == 0) sums[blockIdx.x] = carry;
+}
+
+// exclusive scan of nb block sums in place (one workgroup); total at sums[nb]
+__global__ __launch_bounds__(1024) void k_scan2(int64_t* __restrict__ sums, int64_t nb) {
+    __shared__ int64_t part[1024];
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t lo = min(nb, (int64_t)threadIdx.x * per), hi = min(nb, lo + per);
+    int64_t s = 0;
+    for (int64_t i = lo; i < hi; i++) s += sums[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int64_t v = (int)threadIdx.x >= d ? part[threadIdx.x - d] : 0;
         __syncthreads();
-        part[t] += v;
+        part[threadIdx.x] += v;
         __syncthreads();
     }
-    int64_t run = part[t] - s;
-    for (int i = lo; i < hi; i++) {
-        off[i] = run;
-        cur[i] = run;
-        run += cnt_in[i];
+    int64_t run = part[threadIdx.x] - s;
+    for (int64_t i = lo; i < hi; i++) {
+        const int64_t x = sums[i];
+        sums[i] = run;
+        run += x;
     }
-    if (t == T - 1) {
-        off[ntiles] = part[T - 1];
-        slot_cnt[CNT_SLOTS] = part[T - 1];
-    }
+    if (threadIdx.x == 1023) sums[nb] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_scan3(int64_t* __restrict__ out, int64_t n, const int64_t* __restrict__ sums) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] += sums[i / SCAN_B];
+}
+
+// tile t starts at its bin's offset; with the removed slots' bin,
+// tile_off[ntiles] is that bin's start and tile_off[ntiles+1] the total.
+// The total is the new slot count.
+__global__ __launch_bounds__(256) void k_tile_off(const int64_t* __restrict__ cur, int64_t nbins,
+                                                  const int64_t* __restrict__ total, int ntiles,
+                                                  int64_t* __restrict__ tile_off, int64_t* __restrict__ slot_cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tot = *total;
+    if (t <= ntiles + 1) tile_off[t] = t < nbins ? cur[t] : tot;
+    if (t == 0) slot_cnt[CNT_SLOTS] = tot;
+}
+
+int launch_scan_bins(pic_ctx* c, Species& S, int64_t nbins) {
+    const int64_t nb = (nbins + SCAN_B - 1) / SCAN_B;
+    k_scan1<<<(unsigned)nb, SCAN_T, 0, c->stream>>>(c->tile_cnt, c->tile_cur, nbins, c->scan_sums);
+    PIC_LAUNCH_CHECK(c);
+    k_scan2<<<1, 1024, 0, c->stream>>>(c->scan_sums, nb);
+    PIC_LAUNCH_CHECK(c);
+    k_scan3<<<(int)std::min<int64_t>(blocks_for(nbins, 256), 8192), 256, 0, c->stream>>>(c->tile_cur, nbins,
+                                                                                       c->scan_sums);
+    PIC_LAUNCH_CHECK(c);
+    k_tile_off<<<blocks_for(c->tl.ntiles + 2, 256), 256, 0, c->stream>>>(c->tile_cur, nbins, c->scan_sums + nb,
+                                                                         c->tl.ntiles, S.tile_off, S.d_cnt);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
 }
 
 static TileArgs tile_args(const pic_ctx* c) {
@@ -120,14 +190,14 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = b + threadIdx.x;
         bool valid = false;
-        int key = -1;
+        int64_t key = -1;
         double x = 0, y = 0, z = 0, w = 0;
         if (p < n) {
             w = A.w[p];
             valid = (w == w);
             x = A.x[p]; y = A.y[p]; z = A.z[p];
-            if (valid) key = tile_of64(x, y, z, T);
-            else if (perm) { valid = true; key = T.tl.ntiles; }
+            if (valid) key = bin_of64(x, y, z, T);
+            else if (perm) { valid = true; key = T.tl.nbins(); }
         }
         int head, len;
         wave_runs(key, valid, head, len);
@@ -158,13 +228,6 @@ int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int b
 
 int sort32(pic_ctx* c, int s, int32_t* perm);  // k_push32.hip
 
-// nbins = ntiles, or ntiles + 1 with the removed slots' bin (keep-dead sort)
-int launch_scan_tiles(pic_ctx* c, Species& S, int nbins) {
-    k_scan_tiles<<<1, 1024, 0, c->stream>>>(c->tile_cnt, S.tile_off, c->tile_cur, nbins, S.d_cnt);
-    PIC_LAUNCH_CHECK(c);
-    return PIC_OK;
-}
-
 // Counting sort by tile into the scratch buffer; drops removed particles
 // (or, with perm, keeps them after the last tile).  The slot count on the
 // host side is not needed: kernels read it from d_cnt, and the scatter grid
@@ -172,8 +235,8 @@ int launch_scan_tiles(pic_ctx* c, Species& S, int nbins) {
 int launch_sort(pic_ctx* c, int s, int32_t* perm) {
     Species& S = c->sp[s];
     if (c->fp32) return sort32(c, s, perm);
-    const int nbins = c->tl.ntiles + (perm ? 1 : 0);
-    int r = launch_fill(c, c->tile_cnt, (int64_t)nbins * 8);
+    const int64_t nbins = c->tl.nbins() + (perm ? 1 : 0);
+    int r = launch_fill(c, c->tile_cnt, nbins * 8);
     if (r) return r;
     TileArgs T = tile_args(c);
     const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 32);
@@ -183,7 +246,7 @@ int launch_sort(pic_ctx* c, int s, int32_t* perm) {
     // the scatter needs the pre-sort slot count: keep it in d_cnt[CNT_N-1]
     PIC_HIP(c, hipMemcpyAsync(S.d_cnt + (CNT_N - 1), S.d_cnt + CNT_SLOTS, 8,
                               hipMemcpyDeviceToDevice, c->stream));
-    r = launch_scan_tiles(c, S, nbins);
+    r = launch_scan_bins(c, S, nbins);
     if (r) return r;
     return launch_scatter64(c, S, A, B, T, blocks, perm);
 }

@@ -184,6 +184,7 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
         for (int j = 0; j < L; j++) {
             const float sy0 = Y0[j], dsy = Y1[j] - sy0;
             const float wyz = cx * (sy0 * sz0 + 0.5f * dsy * sz0 + 0.5f * sy0 * dsz + third * dsy * dsz);
+            if (wyz == 0.0f) continue;  // a plane outside both supports: only zeros to add
             float acc = 0.0f;
 #pragma unroll
             for (int i = 0; i < L - 1; i++) {
@@ -203,6 +204,7 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
         for (int i = 0; i < L; i++) {
             const float sx0 = X0[i], dsx = X1[i] - sx0;
             const float wxz = cy * (sx0 * sz0 + 0.5f * dsx * sz0 + 0.5f * sx0 * dsz + third * dsx * dsz);
+            if (wxz == 0.0f) continue;
             float acc = 0.0f;
 #pragma unroll
             for (int j = 0; j < L - 1; j++) {
@@ -223,6 +225,7 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
         for (int i = 0; i < L; i++) {
             const float sx0 = X0[i], dsx = X1[i] - sx0;
             const float wxy = cz * (sx0 * sy0 + 0.5f * dsx * sy0 + 0.5f * sx0 * dsy + third * dsx * dsy);
+            if (wxy == 0.0f) continue;
             float acc = 0.0f;
 #pragma unroll
             for (int k = 0; k < L - 1; k++) {
@@ -497,7 +500,8 @@ struct TileArgs32 {
     Tiling tl;
 };
 
-__device__ __forceinline__ int tile_of32(int cell, const TileArgs32& A) {
+// sort bin of a cell: its tile
+__device__ __forceinline__ int64_t bin_of32(int cell, const TileArgs32& A) {
     const int cx = cell % A.n[0], cy = (cell / A.n[0]) % A.n[1], cz = cell / (A.n[0] * A.n[1]);
     return ((cz / A.tl.t[2]) * A.tl.nt[1] + cy / A.tl.t[1]) * A.tl.nt[0] + cx / A.tl.t[0];
 }
@@ -511,7 +515,7 @@ __global__ __launch_bounds__(256) void k_hist32(const int32_t* __restrict__ cell
         const int64_t p = b + threadIdx.x;
         const int cell = p < n ? cellv[p] : -1;
         const bool valid = cell >= 0 || (keep_dead && p < n);
-        const int key = cell >= 0 ? tile_of32(cell, T) : (valid ? T.tl.ntiles : -1);
+        const int64_t key = cell >= 0 ? bin_of32(cell, T) : (valid ? T.tl.nbins() : -1);
         int head, len;
         wave_runs(key, valid, head, len);
         if (valid && head == lane_id()) atomicAdd((unsigned long long*)&tile_cnt[key], (unsigned long long)len);
@@ -528,7 +532,7 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
         const int64_t p = b + threadIdx.x;
         const int cell = p < n ? A.cell[p] : -1;
         const bool valid = cell >= 0 || (perm && p < n);
-        const int key = cell >= 0 ? tile_of32(cell, T) : (valid ? T.tl.ntiles : -1);
+        const int64_t key = cell >= 0 ? bin_of32(cell, T) : (valid ? T.tl.nbins() : -1);
         int head, len;
         wave_runs(key, valid, head, len);
         int64_t base = 0;
@@ -549,8 +553,8 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
 
 int sort32(pic_ctx* c, int s, int32_t* perm) {
     Species& S = c->sp[s];
-    const int nbins = c->tl.ntiles + (perm ? 1 : 0);
-    int r = launch_fill(c, c->tile_cnt, (int64_t)nbins * 8);
+    const int64_t nbins = c->tl.nbins() + (perm ? 1 : 0);
+    int r = launch_fill(c, c->tile_cnt, nbins * 8);
     if (r) return r;
     TileArgs32 T;
     for (int a = 0; a < 3; a++) T.n[a] = c->g.n[a];
@@ -560,8 +564,7 @@ int sort32(pic_ctx* c, int s, int32_t* perm) {
     PIC_LAUNCH_CHECK(c);
     PIC_HIP(c, hipMemcpyAsync(S.d_cnt + (CNT_N - 1), S.d_cnt + CNT_SLOTS, 8,
                               hipMemcpyDeviceToDevice, c->stream));
-    extern int launch_scan_tiles(pic_ctx*, Species&, int);
-    r = launch_scan_tiles(c, S, nbins);
+    r = launch_scan_bins(c, S, nbins);
     if (r) return r;
     k_scatter32<<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), soa32(S, 1 - S.cur), S.d_cnt,
                                                c->tile_cur, T, perm);

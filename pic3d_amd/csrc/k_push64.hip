@@ -151,6 +151,7 @@ __device__ __forceinline__ void esirkepov(double qw, double xo, double yo, doubl
         for (int j = 0; j < L; j++) {
             const double sy0 = Y0[j], dsy = Y1[j] - sy0;
             const double wyz = sy0 * sz0 + 0.5 * dsy * sz0 + 0.5 * sy0 * dsz + (1.0 / 3.0) * dsy * dsz;
+            if (wyz == 0.0) continue;  // a plane outside both supports: only +-0 to add
             double acc = 0.0;
 #pragma unroll
             for (int i = 0; i < L; i++) {
@@ -169,6 +170,7 @@ __device__ __forceinline__ void esirkepov(double qw, double xo, double yo, doubl
         for (int i = 0; i < L; i++) {
             const double sx0 = X0[i], dsx = X1[i] - sx0;
             const double wxz = sx0 * sz0 + 0.5 * dsx * sz0 + 0.5 * sx0 * dsz + (1.0 / 3.0) * dsx * dsz;
+            if (wxz == 0.0) continue;
             double acc = 0.0;
 #pragma unroll
             for (int j = 0; j < L; j++) {
@@ -188,6 +190,7 @@ __device__ __forceinline__ void esirkepov(double qw, double xo, double yo, doubl
         for (int i = 0; i < L; i++) {
             const double sx0 = X0[i], dsx = X1[i] - sx0;
             const double wxy = sx0 * sy0 + 0.5 * dsx * sy0 + 0.5 * sx0 * dsy + (1.0 / 3.0) * dsx * dsy;
+            if (wxy == 0.0) continue;
             double acc = 0.0;
 #pragma unroll
             for (int k = 0; k < L; k++) {

@@ -49,6 +49,23 @@ void set_tiling(pic_ctx* c) {
     }
 }
 
+int alloc_sort_buffers(pic_ctx* c) {
+    const int64_t bins = c->tl.nbins() + 1;  // + the removed slots' bin
+    if (bins <= c->sort_cap) return PIC_OK;
+    for (int64_t** p : {&c->tile_cnt, &c->tile_cur, &c->scan_sums}) {
+        if (*p) hipFree(*p);
+        *p = nullptr;
+    }
+    c->sort_cap = 0;
+    const int64_t nb = (bins + 16383) / 16384 + 1;  // block sums of the scan (16384 bins per block) + total
+    if (hipMalloc((void**)&c->tile_cnt, (size_t)(bins + 2) * 8) != hipSuccess ||
+        hipMalloc((void**)&c->tile_cur, (size_t)(bins + 2) * 8) != hipSuccess ||
+        hipMalloc((void**)&c->scan_sums, (size_t)nb * 8) != hipSuccess)
+        return set_err(c, PIC_ERR_NOMEM, "sort buffers for %lld bins", (long long)bins);
+    c->sort_cap = bins;
+    return PIC_OK;
+}
+
 int launch_push(pic_ctx* c, int s) {
     extern int launch_push64(pic_ctx*, int);
     extern int launch_push32(pic_ctx*, int);
@@ -294,8 +311,7 @@ int pic_ctx_create(const pic_params* p, int device, pic_ctx** out) {
         c->J[f] = q;
         if (!r) r = launch_fill(c, q, g.jsize * es);
     }
-    if (!r) r = dmalloc(c, &c->tile_cnt, c->tl.ntiles + 1);
-    if (!r) r = dmalloc(c, &c->tile_cur, c->tl.ntiles + 1);
+    if (!r) r = alloc_sort_buffers(c);
     if (!r) r = dmalloc(c, &c->nwork, 8);
     if (!r) r = dmalloc(c, &c->diag_buf, 12 + PIC_SPECTRUM_BINS + 8);
     if (r) return fail(r);
@@ -323,7 +339,8 @@ int pic_ctx_destroy(pic_ctx* c) {
     }
     for (int f = 0; f < 6; f++) if (c->F[f]) hipFree(c->F[f]);
     for (int f = 0; f < 3; f++) if (c->J[f]) hipFree(c->J[f]);
-    dfree(c->tile_cnt); dfree(c->tile_cur); dfree(c->work); dfree(c->nwork); dfree(c->diag_buf);
+    dfree(c->tile_cnt); dfree(c->tile_cur); dfree(c->scan_sums);
+    dfree(c->work); dfree(c->nwork); dfree(c->diag_buf);
     dfree(c->jtmp);
     dfree(c->stage);
     diff_free(c);
@@ -374,10 +391,12 @@ int pic_set_option(pic_ctx* c, int option, int64_t value) {
             const int old_tiles = c->tl.ntiles;
             set_tiling(c);
             for (int s = 0; s < PIC_MAX_SPECIES; s++) c->sp[s].sorted = 0;
-            if (c->tl.ntiles > old_tiles) {  // tile buffers grow with the tile count
-                dfree(c->tile_cnt); dfree(c->tile_cur);
-                int r = dmalloc(c, &c->tile_cnt, c->tl.ntiles + 1);
-                if (!r) r = dmalloc(c, &c->tile_cur, c->tl.ntiles + 1);
+            {
+                int r = alloc_sort_buffers(c);
+                if (r) return r;
+            }
+            if (c->tl.ntiles > old_tiles) {  // tile offsets grow with the tile count
+                int r = PIC_OK;
                 for (int s = 0; s < PIC_MAX_SPECIES && !r; s++)
                     if (c->sp[s].declared) {
                         dfree(c->sp[s].tile_off);

@@ -37,6 +37,8 @@ struct Tiling {
     int t[3];     // tile edge in cells
     int nt[3];    // tiles per axis
     int ntiles;
+    // sort bins: one per tile (a keep-dead sort adds one for removed slots)
+    __host__ __device__ int64_t nbins() const { return (int64_t)ntiles; }
 };
 
 // Per-species state.  FP64: positions in global cell units.  FP32: a linear
@@ -91,8 +93,10 @@ struct pic_ctx {
     double rho_bg = 0.0;
     pic::Species sp[PIC_MAX_SPECIES];
     // scratch
-    int64_t* tile_cnt = nullptr;  // ntiles histogram
-    int64_t* tile_cur = nullptr;  // ntiles scatter cursors
+    int64_t* tile_cnt = nullptr;  // sort histogram (nbins + 1: per cell of every tile, removed slots)
+    int64_t* tile_cur = nullptr;  // sort scatter cursors (nbins + 1)
+    int64_t* scan_sums = nullptr; // block sums of the histogram scan
+    int64_t sort_cap = 0;         // bins tile_cnt / tile_cur hold
     int64_t* work = nullptr;      // push work items (tile << 20 | chunk)
     int64_t work_cap = 0;
     int64_t* nwork = nullptr;     // number of work items (device)
@@ -155,6 +159,11 @@ int launch_boundary(pic_ctx* c, int s);
 // slot that moved to slot q.
 int launch_sort(pic_ctx* c, int s, int32_t* perm = nullptr);
 int build_work(pic_ctx* c, int s, int chunk);
+// Exclusive scan of the sort histogram (nbins bins) into the scatter
+// cursors; the species' tile offsets and slot count from it.
+int launch_scan_bins(pic_ctx* c, Species& S, int64_t nbins);
+// Allocate the sort buffers for the context's tiling (grows only).
+int alloc_sort_buffers(pic_ctx* c);
 // the work list of a tile grouping into (work, nwork), capacity c->work_cap
 int build_work_on(pic_ctx* c, const int64_t* tile_off, int chunk, int64_t* work, int64_t* nwork,
                   hipStream_t stream);
