@@ -73,11 +73,13 @@ __device__ __forceinline__ void shift_down32(float* A) {
 // for quartic shapes the z loop is rolled (register rotation of the z
 // weights) to keep the kernel inside the instruction cache.
 // ---------------------------------------------------------------------
+// The primal weights at the particle (Wxp, Wyp, Wzp) are returned: the
+// deposit's old-position weights are the same values.
 template <int M, typename FR>
 __device__ __forceinline__ void gather32(int Cx, int Cy, int Cz, float fx, float fy, float fz,
-                                         float* E, float* B, FR fr) {
+                                         float* E, float* B, FR fr, float* Wxp, float* Wyp, float* Wzp) {
     constexpr int H = M / 2;
-    float Wxp[M + 1], Wyp[M + 1], Wzp[M + 1], Wxd[M + 1], Wyd[M + 1], Wzd[M + 1];
+    float Wxd[M + 1], Wyd[M + 1], Wzd[M + 1];
     const int ox = fx >= 0.5f, oy = fy >= 0.5f, oz = fz >= 0.5f;
     shape32<M>(fx - (float)ox, Wxp);
     shape32<M>(fy - (float)oy, Wyp);
@@ -141,18 +143,19 @@ __device__ __forceinline__ void boris32(float* u, const float* E, const float* B
 // L = M+2 points per axis; the last point of each cumulative sum (which
 // carries sum(S1) - sum(S0) = 0) is not deposited, and the extra planes
 // along the other axes carry exact zeros.
+// S0x, S0y, S0z: the weights at f0 (as the gather evaluates them)
 template <int M, typename D>
-__device__ __forceinline__ void esirkepov32(float qw, const float* f0, const float* f1,
-                                            const Consts32& K, D& dst) {
+__device__ __forceinline__ void esirkepov32(float qw, const float* f0, const float* f1, const float* S0x,
+                                            const float* S0y, const float* S0z, const Consts32& K, D& dst) {
     constexpr int H = M / 2, L = M + 2;
     constexpr int R = (M == 2) ? L : 1;  // unroll factor of the outer loops
-    float S0[3][M + 1], S1[3][M + 1];
+    const float* S0[3] = {S0x, S0y, S0z};
+    float S1[3][M + 1];
     int b[3], d0[3], d1[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const int i1 = f0[a] >= 0.5f ? 1 : 0;
         const int i2 = (int)floorf(f1[a] + 0.5f);  // -1 .. 2
-        shape32<M>(f0[a] - (float)i1, S0[a]);
         shape32<M>(f1[a] - (float)i2, S1[a]);
         b[a] = min(i1, i2) - H;
         d0[a] = i1 - H - b[a];
@@ -324,15 +327,15 @@ __device__ __forceinline__ bool push32_core(int cx, int cy, int cz, int m, float
                                             float w, const Consts32& K, FR fr, SINK& sink,
                                             int& newcell, unsigned* cnt12) {
     const int Cx = cx + m - 1, Cy = cy + m - 1, Cz = cz + m - 1;  // buffer index of the cell
-    float E[3], Bf[3];
-    gather32<M>(Cx, Cy, Cz, f[0], f[1], f[2], E, Bf, fr);
+    float E[3], Bf[3], W0[3][M + 1];
+    gather32<M>(Cx, Cy, Cz, f[0], f[1], f[2], E, Bf, fr, W0[0], W0[1], W0[2]);
     boris32(u, E, Bf, K.qm);
     const float ig = rsqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
     float d[3] = {u[0] * ig * K.cfx, u[1] * ig * K.cfy, u[2] * ig * K.cfz};
     // a non-finite momentum would put the deposit box outside the arrays
     if (!(fabsf(d[0]) < 1.0f && fabsf(d[1]) < 1.0f && fabsf(d[2]) < 1.0f)) return false;
     float f1[3] = {f[0] + d[0], f[1] + d[1], f[2] + d[2]};
-    esirkepov32<M>(K.q * w, f, f1, K, sink);
+    esirkepov32<M>(K.q * w, f, f1, W0[0], W0[1], W0[2], K, sink);
     newcell = move32(cx, cy, cz, f, d, K, cnt12);
     return true;
 }

@@ -39,15 +39,21 @@ __device__ __forceinline__ void copy_arr(const double* S, double* D) {
 // functor FR(comp, k, j, i).  Each component's sum runs k -> j -> i with
 // terms ((Wx*Wy)*Wz)*F, the reference's order (particle.cpp:207-259); the
 // k loop is rolled to keep the kernel inside the instruction cache.
+// The primal weights at the old position and their centre nodes are
+// returned in (Wxp, Wyp, Wzp, ip, jp, kp): the deposit's old-position
+// weights are the same values (the same function of the same argument).
 template <int M, typename FR>
 __device__ __forceinline__ void gather_boris(double x, double y, double z, double& ux, double& uy,
                                              double& uz, double& xn, double& yn, double& zn,
-                                             const PushConsts& K, int m, FR fr) {
+                                             const PushConsts& K, int m, FR fr, double* Wxp, double* Wyp,
+                                             double* Wzp, int& ip, int& jp, int& kp) {
     constexpr int H = M / 2;
     const double org = (double)(1 - m);
     const double xc = x - org, yc = y - org, zc = z - org;
-    double Wxp[M + 1], Wyp[M + 1], Wzp[M + 1], Wxd[M + 1], Wyd[M + 1], Wzd[M + 1];
-    const int ip = shape_at<M>(xc, Wxp), jp = shape_at<M>(yc, Wyp), kp = shape_at<M>(zc, Wzp);
+    double Wxd[M + 1], Wyd[M + 1], Wzd[M + 1];
+    ip = shape_at<M>(xc, Wxp);
+    jp = shape_at<M>(yc, Wyp);
+    kp = shape_at<M>(zc, Wzp);
     shape_at<M>(xc - 0.5, Wxd);
     shape_at<M>(yc - 0.5, Wyd);
     shape_at<M>(zc - 0.5, Wzd);
@@ -123,15 +129,18 @@ __device__ __forceinline__ void place(const double* S, int d, double* A) {
     }
 }
 
+// deposit with the old-position weights (Sx0, Sy0, Sz0 at nodes i1, j1, k1)
+// given
 template <int M, typename D>
-__device__ __forceinline__ void esirkepov(double qw, double xo, double yo, double zo, double xn,
-                                          double yn, double zn, const PushConsts& K, int m, D& dst) {
+__device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const double* Sy0,
+                                             const double* Sz0, int i1, int j1, int k1, double xn,
+                                             double yn, double zn, const PushConsts& K, int m, D& dst) {
     constexpr int H = M / 2, L = M + 2;
     const double org = (double)(1 - m);
-    double Sx0[M + 1], Sx1[M + 1], Sy0[M + 1], Sy1[M + 1], Sz0[M + 1], Sz1[M + 1];
-    const int i1 = shape_at<M>(xo - org, Sx0), i2 = shape_at<M>(xn - org, Sx1);
-    const int j1 = shape_at<M>(yo - org, Sy0), j2 = shape_at<M>(yn - org, Sy1);
-    const int k1 = shape_at<M>(zo - org, Sz0), k2 = shape_at<M>(zn - org, Sz1);
+    double Sx1[M + 1], Sy1[M + 1], Sz1[M + 1];
+    const int i2 = shape_at<M>(xn - org, Sx1);
+    const int j2 = shape_at<M>(yn - org, Sy1);
+    const int k2 = shape_at<M>(zn - org, Sz1);
     const int bx = min(i1, i2) - H, by = min(j1, j2) - H, bz = min(k1, k2) - H;
     const bool crx = i1 != i2, cry = j1 != j2, crz = k1 != k2;
     double X0[L], X1[L], Y0[L], Y1[L], Z0[L], Z1[L];
@@ -216,6 +225,15 @@ __device__ __forceinline__ void esirkepov(double qw, double xo, double yo, doubl
     }
 }
 
+template <int M, typename D>
+__device__ __forceinline__ void esirkepov(double qw, double xo, double yo, double zo, double xn,
+                                          double yn, double zn, const PushConsts& K, int m, D& dst) {
+    const double org = (double)(1 - m);
+    double Sx0[M + 1], Sy0[M + 1], Sz0[M + 1];
+    const int i1 = shape_at<M>(xo - org, Sx0), j1 = shape_at<M>(yo - org, Sy0), k1 = shape_at<M>(zo - org, Sz0);
+    esirkepov_s0<M>(qw, Sx0, Sy0, Sz0, i1, j1, k1, xn, yn, zn, K, m, dst);
+}
+
 // deposit sinks ---------------------------------------------------------
 struct HbmSink {  // global FP64 atomics on the ghosted J/rho arrays
     double* jp[4];
@@ -251,14 +269,16 @@ __device__ __forceinline__ int push_one_hbm(const P64& P, int64_t p, double wp, 
     const double* fp[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
     auto fr = [&](int c, int k, int j, int i) -> double { return fp[c][g.fi(k, j, i)]; };
     double ux = P.ux[p], uy = P.uy[p], uz = P.uz[p], xn, yn, zn;
-    gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr);
+    double Sx0[M + 1], Sy0[M + 1], Sz0[M + 1];
+    int i1, j1, k1;
+    gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr, Sx0, Sy0, Sz0, i1, j1, k1);
     P.x[p] = xn; P.y[p] = yn; P.z[p] = zn;
     P.ux[p] = ux; P.uy[p] = uy; P.uz[p] = uz;
     if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
     // a non-finite momentum would put the deposit box outside the arrays
     if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) return 1;
     HbmSink d{{F.jx, F.jy, F.jz, F.rho}, g, 0, 0, 0};
-    esirkepov<M>(K.q * wp, x, y, z, xn, yn, zn, K, g.m, d);
+    esirkepov_s0<M>(K.q * wp, Sx0, Sy0, Sz0, i1, j1, k1, xn, yn, zn, K, g.m, d);
     return 0;
 }
 
@@ -384,13 +404,15 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
             continue;
         }
         double ux = P.ux[p], uy = P.uy[p], uz = P.uz[p], xn, yn, zn;
-        gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr);
+        double Sx0[M + 1], Sy0[M + 1], Sz0[M + 1];
+        int i1, j1, k1;
+        gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr, Sx0, Sy0, Sz0, i1, j1, k1);
         P.x[p] = xn; P.y[p] = yn; P.z[p] = zn;
         P.ux[p] = ux; P.uy[p] = uy; P.uz[p] = uz;
         if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
         if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) { oob++; continue; }
         // in-tile particle, move < 1 cell: its whole box lies in the LDS box
-        esirkepov<M>(K.q * wp, x, y, z, xn, yn, zn, K, g.m, sink);
+        esirkepov_s0<M>(K.q * wp, Sx0, Sy0, Sz0, i1, j1, k1, xn, yn, zn, K, g.m, sink);
     }
     if (strays) {
         it = 0;
