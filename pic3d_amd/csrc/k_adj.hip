@@ -799,7 +799,7 @@ __global__ __launch_bounds__(256) void k_adj_push_tiled(V v, Grid g, Tiling tl,
     int it = 0;
     const int len = (int)(end - beg);
     for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
-        const int64_t p = beg + visit_slot(t, len);
+        const int64_t p = beg + t;
         int cell[3], base[3];
         T frac[3];
         if (!v.locate(p, cell, base, frac)) continue;
@@ -827,7 +827,7 @@ __global__ __launch_bounds__(256) void k_adj_push_tiled(V v, Grid g, Tiling tl,
         for (int t = threadIdx.x; t < len; t += blockDim.x, it++)
             if (strays & (1u << it)) {
                 const unsigned long long at = atomicAdd((unsigned long long*)stray_n, 1ull);
-                stray_list[at] = beg + visit_slot(t, len);
+                stray_list[at] = beg + t;
             }
     }
     __syncthreads();

@@ -185,7 +185,8 @@ static Soa64 soa64(pic_ctx* c, Species& S, int b) {
 // perm[q] records the source slot of slot q
 __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const int64_t* __restrict__ cnt,
                                                         int64_t* __restrict__ tile_cur, TileArgs T,
-                                                        int keep_old, int32_t* __restrict__ perm) {
+                                                        int keep_old, int32_t* __restrict__ perm,
+                                                        const int64_t* __restrict__ tile_off, int scramble) {
     const int64_t n = cnt[CNT_N - 1];
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = b + threadIdx.x;
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
             base = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key], (unsigned long long)len);
         base = __shfl(base, head);
         if (valid) {
-            const int64_t q = base + (lane_id() - head);
+            const int64_t q = placed_slot(base + (lane_id() - head), key, T.tl.ntiles, tile_off, scramble);
             B.x[q] = x; B.y[q] = y; B.z[q] = z; B.w[q] = w;
             B.ux[q] = A.ux[p]; B.uy[q] = A.uy[p]; B.uz[q] = A.uz[p];
             B.id[q] = A.id[p];
@@ -217,9 +218,11 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
 }
 
 int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int blocks, int32_t* perm) {
+    const int scramble = !S.scrambled && !perm;
     k_scatter64_dev<<<blocks, 256, 0, c->stream>>>(A, B, S.d_cnt, c->tile_cur, T,
-                                                   c->keep_old && S.xo[0] && S.xo[1], perm);
+                                                   c->keep_old && S.xo[0] && S.xo[1], perm, S.tile_off, scramble);
     PIC_LAUNCH_CHECK(c);
+    if (scramble) S.scrambled = 1;
     S.cur = 1 - S.cur;
     S.sorted = 1;
     S.pushes_since_sort = 0;

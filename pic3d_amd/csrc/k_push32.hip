@@ -402,7 +402,7 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
     int it = 0;
     const int len = (int)(end - beg);
     for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
-        const int64_t p = beg + visit_slot(t, len);
+        const int64_t p = beg + t;
         const int cell = P.cell[p];
         if (cell < 0) continue;
         const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
         it = 0;
         for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
             if (!(strays & (1u << it))) continue;
-            const int64_t p = beg + visit_slot(t, len);
+            const int64_t p = beg + t;
             const int cell = P.cell[p];
             const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
             float f[3] = {P.fx[p], P.fy[p], P.fz[p]};
@@ -529,7 +529,8 @@ __global__ __launch_bounds__(256) void k_hist32(const int32_t* __restrict__ cell
 // perm[q] records the source slot of slot q
 __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int64_t* __restrict__ cnt,
                                                     int64_t* __restrict__ tile_cur, TileArgs32 T,
-                                                    int32_t* __restrict__ perm) {
+                                                    int32_t* __restrict__ perm, const int64_t* __restrict__ tile_off,
+                                                    int scramble) {
     const int64_t n = cnt[CNT_N - 1];
     for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = b + threadIdx.x;
@@ -543,7 +544,7 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
             base = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key], (unsigned long long)len);
         base = __shfl(base, head);
         if (valid) {
-            const int64_t q = base + (lane_id() - head);
+            const int64_t q = placed_slot(base + (lane_id() - head), key, T.tl.ntiles, tile_off, scramble);
             Bo.cell[q] = cell;
             Bo.fx[q] = A.fx[p]; Bo.fy[q] = A.fy[p]; Bo.fz[q] = A.fz[p];
             Bo.ux[q] = A.ux[p]; Bo.uy[q] = A.uy[p]; Bo.uz[q] = A.uz[p];
@@ -569,8 +570,10 @@ int sort32(pic_ctx* c, int s, int32_t* perm) {
                               hipMemcpyDeviceToDevice, c->stream));
     r = launch_scan_bins(c, S, nbins);
     if (r) return r;
+    const int scramble = !S.scrambled && !perm;
     k_scatter32<<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), soa32(S, 1 - S.cur), S.d_cnt,
-                                               c->tile_cur, T, perm);
+                                               c->tile_cur, T, perm, S.tile_off, scramble);
+    if (scramble) S.scrambled = 1;
     PIC_LAUNCH_CHECK(c);
     S.cur = 1 - S.cur;
     S.sorted = 1;

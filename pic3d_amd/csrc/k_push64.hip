@@ -391,7 +391,7 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
     int it = 0;
     const int len = (int)(end - beg);
     for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
-        const int64_t p = beg + visit_slot(t, len);
+        const int64_t p = beg + t;
         const double wp = P.w[p];
         if (!(wp == wp)) continue;
         double x = P.x[p], y = P.y[p], z = P.z[p];
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
         it = 0;
         for (int t = threadIdx.x; t < len; t += blockDim.x, it++)
             if (strays & (1u << it)) {
-                const int64_t p = beg + visit_slot(t, len);
+                const int64_t p = beg + t;
                 oob += push_one_hbm<M>(P, p, P.w[p], F, g, K);
             }
     }

@@ -541,6 +541,7 @@ int pic_particles_set(pic_ctx* c, int s, int64_t n, const double* x, const doubl
     PIC_HIP(c, hipMemcpyAsync(S.d_cnt, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
     PIC_HIP(c, hipStreamSynchronize(c->stream));
     S.sorted = 0;
+    S.scrambled = 0;
     return PIC_OK;
 }
 
@@ -605,6 +606,7 @@ int pic_particles_load(pic_ctx* c, int s, const pic_loader* l) {
     PIC_HIP(c, hipStreamSynchronize(c->stream));
     dfree(d_pre);
     S.sorted = 0;
+    S.scrambled = 0;
     return r;
 }
 

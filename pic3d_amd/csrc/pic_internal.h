@@ -66,6 +66,7 @@ struct Species {
     int64_t* tile_off = nullptr;  // ntiles + 1 tile offsets after the last sort (+1: the
                                   // end of the removed slots a keep-dead sort appends)
     int sorted = 0;               // tile_off valid for the current order
+    int scrambled = 0;            // the in-tile order has been transposed (stored_slot)
 };
 
 // d_cnt layout.  CNT_PEND: FP32 pushes classify boundary crossings as they
@@ -184,18 +185,34 @@ constexpr int PIC_PUSH_CHUNK = 4096;  // particles per tiled-push workgroup
 // the tiled pushes track one bit per particle a thread visits (256 threads)
 static_assert(PIC_PUSH_CHUNK <= 32 * 256, "stray mask holds 32 iterations");
 
-// Visiting order of a tiled push chunk of length len: a 64 x (len/64)
-// transpose of the slots (the tail past the last multiple of 64 in order).
-// Lane l of the wave at visit row q takes slot l * (len/64) + q, so a
-// wavefront's 64 lanes sit len/64 slots apart -- in different cells when a
-// cell holds no more than len/64 particles, which keeps the LDS deposits of
-// one instruction off a shared address (same-address LDS atomics
-// serialise) -- while the following visit rows reuse the same cache lines
-// (slot + 1), so each line is fetched from HBM once.
-__host__ __device__ inline int visit_slot(int t, int len) {
+// Storage order within a tile.  Loaded particles come cell by cell, so
+// the first sort after a load cuts each tile into chunks of PIC_PUSH_CHUNK
+// (one tiled-push workgroup each) and stores the particle of rank r of a
+// chunk of length len at slot stored_slot(r, len), a 64 x (len/64)
+// transpose (the tail past the last multiple of 64 in order).  Later sorts
+// keep the relative order within a tile.  The push visits slots in order
+// (reads and writes coalesced), and lane l of a wave at visit row q holds
+// the particle of original rank l * (len/64) + q: the 64 lanes sit len/64
+// ranks apart -- in different cells when a cell holds no more than len/64
+// particles -- which keeps one instruction's LDS deposits off a shared
+// address (same-address LDS atomics serialise).
+__host__ __device__ inline int stored_slot(int r, int len) {
     const int l64 = len & ~63;
-    if (t >= l64) return t;
-    return (t & 63) * (l64 >> 6) + (t >> 6);
+    if (r >= l64) return r;
+    const int rows = l64 >> 6;
+    return (r % rows) * 64 + r / rows;
+}
+
+// Slot of the particle at sorted position q of tile 'tile' (the sort's
+// cursor value): in order, or (scramble) at its transposed slot within its
+// chunk.  Removed slots (bin ntiles) stay in order.
+__device__ __forceinline__ int64_t placed_slot(int64_t q, int64_t tile, int ntiles,
+                                               const int64_t* __restrict__ tile_off, int scramble) {
+    if (!scramble || tile >= ntiles) return q;
+    const int64_t t0 = tile_off[tile], r = q - t0;
+    const int64_t c0 = r / PIC_PUSH_CHUNK * PIC_PUSH_CHUNK;
+    const int len = (int)min((int64_t)PIC_PUSH_CHUNK, tile_off[tile + 1] - t0 - c0);
+    return t0 + c0 + stored_slot((int)(r - c0), len);
 }
 
 // Tile edges of the sorted push (LDS per workgroup: field tile +
