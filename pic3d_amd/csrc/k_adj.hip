@@ -694,12 +694,11 @@ struct AdjView32 {
     AdjK<float> K;
     int n[3];
     int m;
+    CellSplit cs;
     __device__ __forceinline__ bool locate(int64_t p, int* cell, int* base, float* frac) const {
         const int c = A.cell[p];
         if (c < 0) return false;  // removed at an open face
-        cell[0] = c % n[0];
-        cell[1] = (c / n[0]) % n[1];
-        cell[2] = c / (n[0] * n[1]);
+        cs.split(c, cell[0], cell[1], cell[2]);
 #pragma unroll
         for (int a = 0; a < 3; a++) base[a] = cell[a] + m - 1;
         frac[0] = A.fx[p]; frac[1] = A.fy[p]; frac[2] = A.fz[p];
@@ -1505,7 +1504,8 @@ int launch_adj_push(pic_ctx* c, DiffState& D, const char* save) {
     if (tiled && tile_kind(c) != D.kind)
         return set_err(c, PIC_ERR_STATE, "differentiated run: the deposit accumulator option changed during the run");
     if (D.f32) {
-        AdjView32 v{adj_view32(c, D, save), adj_consts32(c, c->sp[D.s]), {c->g.n[0], c->g.n[1], c->g.n[2]}, c->g.m};
+        AdjView32 v{adj_view32(c, D, save), adj_consts32(c, c->sp[D.s]), {c->g.n[0], c->g.n[1], c->g.n[2]}, c->g.m,
+                    CellSplit(c->g.n[0], c->g.n[1])};
         if (tiled) {
             if (D.kind == 2) return q ? launch_adj_tiled<2, 2>(c, D, v) : launch_adj_tiled<4, 2>(c, D, v);
             return q ? launch_adj_tiled<2, 1>(c, D, v) : launch_adj_tiled<4, 1>(c, D, v);

@@ -20,6 +20,7 @@ struct TileArgs {
     int per[3];
     int n[3];
     Tiling tl;
+    FastDiv td[3];  // tile edges
 };
 
 // sort bin of a particle: its tile
@@ -33,7 +34,7 @@ __device__ __forceinline__ int64_t bin_of64(double x, double y, double z, const 
         if (out && A.per[a]) { if (v < 0.0) v += A.N[a]; if (v > A.N[a]) v -= A.N[a]; }
         int c = (int)floor(v);
         c = c < 0 ? 0 : (c >= A.n[a] ? A.n[a] - 1 : c);
-        t[a] = c / A.tl.t[a];
+        t[a] = A.td[a].div(c);
     }
     return (t[2] * A.tl.nt[1] + t[1]) * A.tl.nt[0] + t[0];
 }
@@ -43,24 +44,50 @@ struct Soa64 {
     int64_t* id;
 };
 
+constexpr int HIST_R64 = 4;  // rows of 64 slots per wave and block iteration
+
 // keep_dead: removed slots counted in bin ntiles instead of dropped
 __global__ __launch_bounds__(256) void k_hist64(Soa64 A, const int64_t* __restrict__ cnt,
                                                  int64_t* __restrict__ tile_cnt, TileArgs T, int keep_dead) {
-    const int64_t n = cnt[CNT_SLOTS];
-    const int64_t base = (int64_t)blockIdx.x * blockDim.x;
-    for (int64_t b = base; b < n; b += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = b + threadIdx.x;
-        bool valid = false;
-        int64_t key = -1;
-        if (p < n) {
-            const double w = A.w[p];
-            valid = (w == w);
-            if (valid) key = bin_of64(A.x[p], A.y[p], A.z[p], T);
-            else if (keep_dead) { valid = true; key = T.tl.nbins(); }
+    constexpr int R = HIST_R64;
+    int64_t lo, hi;
+    block_range<R>(cnt[CNT_SLOTS], lo, hi);
+    const int64_t wseg = (threadIdx.x >> 6) * 64 * R + lane_id();
+    for (int64_t b = lo; b < hi; b += (int64_t)blockDim.x * R) {
+        double x[R], y[R], z[R], w[R];
+        int64_t key[R];
+        bool valid[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t p = b + wseg + r * 64;
+            w[r] = 0.0;
+            if (p < hi) { w[r] = A.w[p]; x[r] = A.x[p]; y[r] = A.y[p]; z[r] = A.z[p]; }
         }
-        int head, len;
-        wave_runs(key, valid, head, len);
-        if (valid && head == lane_id()) atomicAdd((unsigned long long*)&tile_cnt[key], (unsigned long long)len);
+        bool uni = true;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t p = b + wseg + r * 64;
+            valid[r] = false;
+            key[r] = -1;
+            if (p < hi) {
+                valid[r] = (w[r] == w[r]);
+                if (valid[r]) key[r] = bin_of64(x[r], y[r], z[r], T);
+                else if (keep_dead) { valid[r] = true; key[r] = T.tl.nbins(); }
+            }
+            uni = uni && valid[r] && key[r] == key[0];
+        }
+        const int64_t k0 = __shfl(key[0], 0);
+        if (__all(uni && key[0] == k0)) {  // the wave's 64 R slots in one bin
+            if (lane_id() == 0) atomicAdd((unsigned long long*)&tile_cnt[k0], (unsigned long long)(64 * R));
+            continue;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            int head, len;
+            wave_runs(key[r], valid[r], head, len);
+            if (valid[r] && head == lane_id())
+                atomicAdd((unsigned long long*)&tile_cnt[key[r]], (unsigned long long)len);
+        }
     }
 }
 
@@ -171,6 +198,7 @@ static TileArgs tile_args(const pic_ctx* c) {
         T.N[a] = c->g.n[a];
         T.per[a] = c->p.periodic[a];
         T.n[a] = c->g.n[a];
+        T.td[a] = FastDiv(c->tl.t[a]);
     }
     T.tl = c->tl;
     return T;
@@ -182,7 +210,11 @@ static Soa64 soa64(pic_ctx* c, Species& S, int b) {
 }
 
 // perm != null: removed slots go after the last tile (bin ntiles) and
-// perm[q] records the source slot of slot q
+// perm[q] records the source slot of slot q.  Blocks interleave over the
+// slots (grid stride), so the cursor atomics of concurrent waves hand out
+// a tile's places in an order that shuffles its 64-slot groups; the FP64
+// push runs about 2 % faster on that order than on the order-preserving
+// placement the FP32 scatter uses (fewer LDS bank conflicts measured).
 __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const int64_t* __restrict__ cnt,
                                                         int64_t* __restrict__ tile_cur, TileArgs T,
                                                         int keep_old, int32_t* __restrict__ perm,

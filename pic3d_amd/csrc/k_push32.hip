@@ -53,6 +53,7 @@ struct Consts32 {
     int n[3];
     int per[3];
     int m;
+    CellSplit cs;  // cell index -> (cx, cy, cz)
 };
 
 // quadratic/quartic weights for an offset delta = x - round(x) in [-0.5, 0.5)
@@ -405,7 +406,8 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
         const int64_t p = beg + t;
         const int cell = P.cell[p];
         if (cell < 0) continue;
-        const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
+        int cx, cy, cz;
+        K.cs.split(cell, cx, cy, cz);
         if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
             (unsigned)(cz - T0z) >= (unsigned)TZ) {
             strays |= 1u << it;  // moved out of the tile since the last sort
@@ -435,7 +437,8 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
             if (!(strays & (1u << it))) continue;
             const int64_t p = beg + t;
             const int cell = P.cell[p];
-            const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
+            int cx, cy, cz;
+            K.cs.split(cell, cx, cy, cz);
             float f[3] = {P.fx[p], P.fy[p], P.fz[p]};
             float u[3] = {P.ux[p], P.uy[p], P.uz[p]};
             auto fr = [&](int c, int k, int j, int i) -> float { return fsrc[c][g.fi(k, j, i)]; };
@@ -478,7 +481,8 @@ __global__ __launch_bounds__(256) void k_push32_direct(Soa32 P, Fields32 F, Grid
          p += (int64_t)gridDim.x * blockDim.x) {
         const int cell = P.cell[p];
         if (cell < 0) continue;
-        const int cx = cell % K.n[0], cy = (cell / K.n[0]) % K.n[1], cz = cell / (K.n[0] * K.n[1]);
+        int cx, cy, cz;
+        K.cs.split(cell, cx, cy, cz);
         float f[3] = {P.fx[p], P.fy[p], P.fz[p]};
         float u[3] = {P.ux[p], P.uy[p], P.uz[p]};
         auto fr = [&](int c, int k, int j, int i) -> float { return fsrc[c][g.fi(k, j, i)]; };
@@ -501,55 +505,126 @@ __global__ __launch_bounds__(256) void k_push32_direct(Soa32 P, Fields32 F, Grid
 struct TileArgs32 {
     int n[3];
     Tiling tl;
+    CellSplit cs;
+    FastDiv td[3];  // tile edges
 };
 
 // sort bin of a cell: its tile
 __device__ __forceinline__ int64_t bin_of32(int cell, const TileArgs32& A) {
-    const int cx = cell % A.n[0], cy = (cell / A.n[0]) % A.n[1], cz = cell / (A.n[0] * A.n[1]);
-    return ((cz / A.tl.t[2]) * A.tl.nt[1] + cy / A.tl.t[1]) * A.tl.nt[0] + cx / A.tl.t[0];
+    int cx, cy, cz;
+    A.cs.split(cell, cx, cy, cz);
+    return (A.td[2].div(cz) * A.tl.nt[1] + A.td[1].div(cy)) * A.tl.nt[0] + A.td[0].div(cx);
 }
+
+constexpr int SORT_R32 = 4;  // rows of 64 slots per wave and block iteration
 
 // keep_dead: removed slots counted in bin ntiles instead of dropped
 __global__ __launch_bounds__(256) void k_hist32(const int32_t* __restrict__ cellv,
                                                  const int64_t* __restrict__ cnt,
                                                  int64_t* __restrict__ tile_cnt, TileArgs32 T, int keep_dead) {
-    const int64_t n = cnt[CNT_SLOTS];
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = b + threadIdx.x;
-        const int cell = p < n ? cellv[p] : -1;
-        const bool valid = cell >= 0 || (keep_dead && p < n);
-        const int64_t key = cell >= 0 ? bin_of32(cell, T) : (valid ? T.tl.nbins() : -1);
-        int head, len;
-        wave_runs(key, valid, head, len);
-        if (valid && head == lane_id()) atomicAdd((unsigned long long*)&tile_cnt[key], (unsigned long long)len);
+    constexpr int R = SORT_R32;
+    int64_t lo, hi;
+    block_range<R>(cnt[CNT_SLOTS], lo, hi);
+    const int64_t wseg = (threadIdx.x >> 6) * 64 * R + lane_id();
+    for (int64_t b = lo; b < hi; b += (int64_t)blockDim.x * R) {
+        int cell[R];
+        int64_t key[R];
+        bool valid[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t p = b + wseg + r * 64;
+            cell[r] = p < hi ? cellv[p] : -1;
+        }
+        bool uni = true;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t p = b + wseg + r * 64;
+            valid[r] = cell[r] >= 0 || (keep_dead && p < hi);
+            key[r] = cell[r] >= 0 ? bin_of32(cell[r], T) : (valid[r] ? T.tl.nbins() : -1);
+            uni = uni && valid[r] && key[r] == key[0];
+        }
+        const int64_t k0 = __shfl(key[0], 0);
+        if (__all(uni && key[0] == k0)) {  // the wave's 64 R slots in one bin
+            if (lane_id() == 0) atomicAdd((unsigned long long*)&tile_cnt[k0], (unsigned long long)(64 * R));
+            continue;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            int head, len;
+            wave_runs(key[r], valid[r], head, len);
+            if (valid[r] && head == lane_id())
+                atomicAdd((unsigned long long*)&tile_cnt[key[r]], (unsigned long long)len);
+        }
     }
 }
 
 // perm != null: removed slots go after the last tile (bin ntiles) and
-// perm[q] records the source slot of slot q
+// perm[q] records the source slot of slot q.  Each wave moves 64 R
+// consecutive slots per iteration; when they share one bin (the common
+// case for tile-ordered input) one cursor atomic places them contiguously
+// and in order.
 __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int64_t* __restrict__ cnt,
                                                     int64_t* __restrict__ tile_cur, TileArgs32 T,
                                                     int32_t* __restrict__ perm, const int64_t* __restrict__ tile_off,
                                                     int scramble) {
-    const int64_t n = cnt[CNT_N - 1];
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x; b < n; b += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = b + threadIdx.x;
-        const int cell = p < n ? A.cell[p] : -1;
-        const bool valid = cell >= 0 || (perm && p < n);
-        const int64_t key = cell >= 0 ? bin_of32(cell, T) : (valid ? T.tl.nbins() : -1);
-        int head, len;
-        wave_runs(key, valid, head, len);
-        int64_t base = 0;
-        if (valid && head == lane_id())
-            base = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key], (unsigned long long)len);
-        base = __shfl(base, head);
-        if (valid) {
-            const int64_t q = placed_slot(base + (lane_id() - head), key, T.tl.ntiles, tile_off, scramble);
-            Bo.cell[q] = cell;
-            Bo.fx[q] = A.fx[p]; Bo.fy[q] = A.fy[p]; Bo.fz[q] = A.fz[p];
-            Bo.ux[q] = A.ux[p]; Bo.uy[q] = A.uy[p]; Bo.uz[q] = A.uz[p];
-            Bo.w[q] = A.w[p];
-            Bo.id[q] = A.id[p];
+    constexpr int R = SORT_R32;
+    int64_t lo, hi;
+    block_range<R>(cnt[CNT_N - 1], lo, hi);
+    const int64_t wseg = (threadIdx.x >> 6) * 64 * R + lane_id();
+    for (int64_t b = lo; b < hi; b += (int64_t)blockDim.x * R) {
+        int cell[R], head[R];
+        int64_t key[R], base[R], id[R];
+        float v[R][7];
+        bool valid[R];
+        // every row's loads in flight together
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t p = b + wseg + r * 64;
+            cell[r] = -1;
+            if (p < hi) {
+                cell[r] = A.cell[p];
+                v[r][0] = A.fx[p]; v[r][1] = A.fy[p]; v[r][2] = A.fz[p];
+                v[r][3] = A.ux[p]; v[r][4] = A.uy[p]; v[r][5] = A.uz[p];
+                v[r][6] = A.w[p];
+                id[r] = A.id[p];
+            }
+        }
+        bool uni = true;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t p = b + wseg + r * 64;
+            valid[r] = cell[r] >= 0 || (perm && p < hi);
+            key[r] = cell[r] >= 0 ? bin_of32(cell[r], T) : (valid[r] ? T.tl.nbins() : -1);
+            uni = uni && valid[r] && key[r] == key[0];
+        }
+        const int64_t k0 = __shfl(key[0], 0);
+        if (__all(uni && key[0] == k0)) {
+            int64_t b0 = 0;
+            if (lane_id() == 0) b0 = (int64_t)atomicAdd((unsigned long long*)&tile_cur[k0], (unsigned long long)(64 * R));
+            b0 = __shfl(b0, 0);
+#pragma unroll
+            for (int r = 0; r < R; r++) { head[r] = 0; base[r] = b0 + r * 64; }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                int len;
+                wave_runs(key[r], valid[r], head[r], len);
+                base[r] = 0;
+                if (valid[r] && head[r] == lane_id())
+                    base[r] = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key[r]], (unsigned long long)len);
+                base[r] = __shfl(base[r], head[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int64_t p = b + wseg + r * 64;
+            if (!valid[r]) continue;
+            const int64_t q = placed_slot(base[r] + (lane_id() - head[r]), key[r], T.tl.ntiles, tile_off, scramble);
+            Bo.cell[q] = cell[r];
+            Bo.fx[q] = v[r][0]; Bo.fy[q] = v[r][1]; Bo.fz[q] = v[r][2];
+            Bo.ux[q] = v[r][3]; Bo.uy[q] = v[r][4]; Bo.uz[q] = v[r][5];
+            Bo.w[q] = v[r][6];
+            Bo.id[q] = id[r];
             if (perm) perm[q] = (int32_t)p;
         }
     }
@@ -561,8 +636,9 @@ int sort32(pic_ctx* c, int s, int32_t* perm) {
     int r = launch_fill(c, c->tile_cnt, nbins * 8);
     if (r) return r;
     TileArgs32 T;
-    for (int a = 0; a < 3; a++) T.n[a] = c->g.n[a];
+    for (int a = 0; a < 3; a++) { T.n[a] = c->g.n[a]; T.td[a] = FastDiv(c->tl.t[a]); }
     T.tl = c->tl;
+    T.cs = CellSplit(c->g.n[0], c->g.n[1]);
     const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 32);
     k_hist32<<<blocks, 256, 0, c->stream>>>(S.cell[S.cur], S.d_cnt, c->tile_cnt, T, perm ? 1 : 0);
     PIC_LAUNCH_CHECK(c);
@@ -707,6 +783,7 @@ static Consts32 consts32(const pic_ctx* c, const Species& S) {
     K.cjz = (float)(c->p.dz / c->p.dt);
     for (int a = 0; a < 3; a++) { K.n[a] = c->g.n[a]; K.per[a] = c->p.periodic[a]; }
     K.m = c->g.m;
+    K.cs = CellSplit(c->g.n[0], c->g.n[1]);
     const double qw = std::fabs(S.q) * (S.wmax > 0 ? S.wmax : 1.0);
     const double bound[4] = {qw * c->p.dx / c->p.dt, qw * c->p.dy / c->p.dt, qw * c->p.dz / c->p.dt, qw};
     for (int k = 0; k < 4; k++) {

@@ -337,16 +337,25 @@ __global__ __launch_bounds__(256) void k_push64_direct(P64 P, Fields64 F, Grid g
 // LDS accumulator box for a tile whose first cell is (T0x, T0y, T0z) with
 // edges (tx, ty, tz): buffer indices [T0 + m - 2 - H, T0 + t - 1 + m + 1 + H],
 // i.e. t + m + 3 points per axis (particles move < 1 cell per step).
+//
+// Accumulator rows are stored with an x pitch of 12 for 4-wide tiles: the
+// particles of a wave sit in consecutive cells of the tile (rank order
+// within a chunk), and LDS serves 16 lanes of an instruction at a time;
+// with pitch 12 the 16 cells of a 4 x 4 face fall on distinct banks (pitch
+// 11 maps three pairs of them to one bank).  Two workgroups of the quartic
+// kernel still fit in a CU's LDS.
 template <int M, int TX, int TY, int TZ>
 struct AccBox {
     static constexpr int H = M / 2;
     static constexpr int BX = TX + M + 3, BY = TY + M + 3, BZ = TZ + M + 3;
-    static constexpr int VOL = BX * BY * BZ;
+    static constexpr int PX = (TX == 4 && BX <= 12) ? 12 : BX;  // x pitch
+    static constexpr int VOL = PX * BY * BZ;
     // field tile: buffer indices [T0 + m - 1 - H, T0 + T + m - 1 + H]
     static constexpr int FX = TX + M + 1, FY = TY + M + 1, FZ = TZ + M + 1;
     static constexpr int FVOL = FX * FY * FZ;
     static constexpr size_t LDS = (size_t)(6 * FVOL + 4 * VOL) * sizeof(double);
 };
+static_assert(2 * AccBox<4, 4, 4, 4>::LDS <= 160 * 1024, "two quartic workgroups per CU");
 
 template <int M, int TX, int TY, int TZ>
 __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid g, PushConsts K,
@@ -384,7 +393,7 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
     auto fr = [&](int c, int k, int j, int i) -> double {
         return fld[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)];
     };
-    LdsSink<B::BX, B::BY, B::VOL> sink{acc, T0z + g.m - 2 - B::H, T0y + g.m - 2 - B::H,
+    LdsSink<B::PX, B::BY, B::VOL> sink{acc, T0z + g.m - 2 - B::H, T0y + g.m - 2 - B::H,
                                       T0x + g.m - 2 - B::H, 0};
     int64_t oob = 0;
     uint32_t strays = 0;  // iterations whose particle was outside the tile
@@ -430,7 +439,7 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
         const double v = acc[e];
         if (v == 0.0) continue;
         const int c = e / B::VOL, r = e % B::VOL;
-        const int lx = r % B::BX, ly = (r / B::BX) % B::BY, lz = r / (B::BX * B::BY);
+        const int lx = r % B::PX, ly = (r / B::PX) % B::BY, lz = r / (B::PX * B::BY);
         const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
         // a partial last tile's box reaches past the arrays; those points
         // are never deposited into, but keep the flush in bounds regardless

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/pic3d_amd.h"
+#include "fastdiv.h"
 
 namespace pic {
 

@@ -30,4 +30,17 @@ __device__ __forceinline__ void wave_runs(int64_t key, bool valid, int& head_lan
     run_len = next - lane;
 }
 
+// Contiguous slot range [lo, hi) of this block for a sort pass over n
+// slots, in steps of blockDim.x * R.  Blocks resident at one time then
+// work on distant parts of the (mostly tile-ordered) array, so their
+// per-tile counter atomics go to different addresses; with interleaved
+// blocks every wave would update the same few counters.
+template <int R>
+__device__ __forceinline__ void block_range(int64_t n, int64_t& lo, int64_t& hi) {
+    const int64_t step = (int64_t)blockDim.x * R;
+    const int64_t per = ((n + gridDim.x - 1) / gridDim.x + step - 1) / step * step;
+    lo = min(n, (int64_t)blockIdx.x * per);
+    hi = min(n, lo + per);
+}
+
 }  // namespace pic
