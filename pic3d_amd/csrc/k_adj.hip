@@ -25,6 +25,7 @@
 // particle.
 #include <algorithm>
 #include <cmath>
+#include <map>
 
 #include "atomics.h"
 #include "laser.h"
@@ -1160,6 +1161,9 @@ struct DiffState {
     // snapshots: [0] the run's initial state, [i] the state before step first + i*K
     std::vector<char*> ck;
     size_t ck_bytes = 0;
+    // the species' scrambled flag (in-tile storage order) of each snapshot
+    // taken, restored with it
+    std::map<const char*, int> scr;
     char* fin = nullptr;             // the state at the end of the forward
     // per-step buffers of the segments being recomputed and reversed
     // (double-buffered: segment k - 1 is recomputed on the context's stream
@@ -1250,6 +1254,11 @@ int copy_state(pic_ctx* c, DiffState& D, char* snap, bool save) {
     for (int f = 0; f < D.ncol() && !r; f++) r = mv(pcol(c, D.s, f), pb);
     if (!r) r = mv(c->sp[D.s].d_cnt, CNT_N * 8);
     if (!r) r = mv(c->sp[D.s].tile_off, (size_t)(c->tl.ntiles + 2) * 8);
+    if (!r) {
+        int& f = c->sp[D.s].scrambled;
+        if (save) D.scr[snap] = f;
+        else if (D.scr.count(snap)) f = D.scr[snap];
+    }
     return r;
 }
 

@@ -250,7 +250,7 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
 }
 
 int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int blocks, int32_t* perm) {
-    const int scramble = !S.scrambled && !perm;
+    const int scramble = !S.scrambled;
     k_scatter64_dev<<<blocks, 256, 0, c->stream>>>(A, B, S.d_cnt, c->tile_cur, T,
                                                    c->keep_old && S.xo[0] && S.xo[1], perm, S.tile_off, scramble);
     PIC_LAUNCH_CHECK(c);

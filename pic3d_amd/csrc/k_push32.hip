@@ -646,7 +646,7 @@ int sort32(pic_ctx* c, int s, int32_t* perm) {
                               hipMemcpyDeviceToDevice, c->stream));
     r = launch_scan_bins(c, S, nbins);
     if (r) return r;
-    const int scramble = !S.scrambled && !perm;
+    const int scramble = !S.scrambled;
     k_scatter32<<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), soa32(S, 1 - S.cur), S.d_cnt,
                                                c->tile_cur, T, perm, S.tile_off, scramble);
     if (scramble) S.scrambled = 1;
