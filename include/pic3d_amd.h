@@ -199,6 +199,33 @@ int pic_particles_get(pic_ctx* ctx, int s, int64_t* n, double* x, double* y, dou
 int pic_particles_load(pic_ctx* ctx, int s, const pic_loader* l);
 int pic_num_particles(pic_ctx* ctx, int s, int64_t* n);
 
+/* ---- spin precession (particle.cpp:318-368) ---- */
+
+/* Turn on the spin precession that update_particle_position_momentum
+ * applies to electrons (particle.cpp:318-368, for particles whose mass
+ * equals m_e) for every species of mass electron_mass, declared now or
+ * later.  g_table: the n values of g.txt that load_g_data reads
+ * (constants.cpp:114-136; n = 2849 in the reference), indexed as g_factor
+ * does (particle.cpp:603-637); e_schwinger: E_schwinger (constants.cpp:83).
+ * Spins start at 0 (pic_spin_set / pic_spin_fill give them values) and
+ * follow their particles through sorts and boundaries.  Off by default:
+ * the spin feeds back into nothing else.  Not available inside a
+ * differentiated run. */
+int pic_spin_setup(pic_ctx* ctx, const double* g_table, int n, double e_schwinger,
+                   double electron_mass);
+/* Spins of the n live particles of species s, in the order
+ * pic_particles_get returns them (host -> device, device -> host). */
+int pic_spin_set(pic_ctx* ctx, int s, int64_t n, const double* sx, const double* sy,
+                 const double* sz);
+int pic_spin_get(pic_ctx* ctx, int s, int64_t* n, double* sx, double* sy, double* sz);
+/* The same spin for every particle of species s (the initial_sx..sz of
+ * initialize_particles, particle.cpp:50-111). */
+int pic_spin_fill(pic_ctx* ctx, int s, double sx, double sy, double sz);
+/* Precession steps skipped so far because chi fell outside g.txt's range
+ * [0, 100] or was NaN (the reference prints an error and exits there,
+ * particle.cpp:632-636); the spin is left unchanged.  Synchronises. */
+int pic_spin_errors(pic_ctx* ctx, int s, int64_t* n);
+
 /* ---- per-phase hot path (one call = one reference call) ---- */
 
 /* radiating_boundary(rank, 1, laser_right, nprocs, step), boundary.cpp:42-99 */
@@ -329,8 +356,8 @@ int pic_particle_slots(pic_ctx* ctx, int s, int64_t* slots);
  * records io_utils.cpp:218-236 writes), in slot order, into 'records'
  * (room for nslots); *nrec = records written.  id is the low 32 bits of
  * the id; x..z in global cell units; x_old..z_old as kept with
- * PIC_OPT_KEEP_OLD (FP64), else the position; px..pz = u; sx..sz = 0 (the
- * spin is not carried); gamma = sqrt(1 + u.u) (particle.cpp:309); charge
+ * PIC_OPT_KEEP_OLD (FP64), else the position; px..pz = u; sx..sz the spin
+ * (0 for a species without precession, pic_spin_setup); gamma = sqrt(1 + u.u) (particle.cpp:309); charge
  * and mass of the species; weight; rank 0.  Synchronises. */
 int pic_particle_records(pic_ctx* ctx, int s, int64_t first_slot, int64_t nslots, void* records,
                          int64_t* nrec);

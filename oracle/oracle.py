@@ -45,6 +45,11 @@ _SIG = {
     "orc_field_set": (None, [C.c_void_p, C.c_int, _dp]),
     "orc_diag": (None, [C.c_void_p, C.POINTER(PicDiag)]),
     "orc_out_of_bounds": (C.c_int64, [C.c_void_p]),
+    "orc_spin_setup": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double]),
+    "orc_spin_set": (None, [C.c_void_p, C.c_int] + [_dp] * 3),
+    "orc_spin_get": (None, [C.c_void_p, C.c_int] + [_dp] * 3),
+    "orc_spin_errors": (C.c_int64, [C.c_void_p, C.c_int]),
+    "orc_g_factor": (C.c_double, [C.c_void_p, C.c_double, C.POINTER(C.c_int)]),
     "orc_shape2": (None, [C.c_double, _dp]),
     "orc_shape4": (None, [C.c_double, _dp]),
     "orc_gpulse": (C.c_double, [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double]),
@@ -154,6 +159,31 @@ class Oracle:
                                 _i(out["id"]), _d(out["x_old"]), _d(out["y_old"]),
                                 _d(out["z_old"]))
         return out
+
+    # spin precession (particle.cpp:318-368)
+    def spin_setup(self, g_table, e_schwinger: float, m_e: float = 1.0):
+        g = np.ascontiguousarray(g_table, dtype=np.float64)
+        assert lib().orc_spin_setup(self._h, _d(g), len(g), e_schwinger, m_e) == 0
+
+    def spin_set(self, s, sx, sy, sz):
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (sx, sy, sz)]
+        assert all(len(v) == self.num_particles(s) for v in a)
+        lib().orc_spin_set(self._h, s, _d(a[0]), _d(a[1]), _d(a[2]))
+
+    def spin_get(self, s):
+        n = self.num_particles(s)
+        a = [np.empty(n) for _ in range(3)]
+        lib().orc_spin_get(self._h, s, _d(a[0]), _d(a[1]), _d(a[2]))
+        return a
+
+    def spin_errors(self, s) -> int:
+        return int(lib().orc_spin_errors(self._h, s))
+
+    def g_factor(self, x: float):
+        """(value, bad) of particle.cpp:603-637 for one chi."""
+        bad = C.c_int(0)
+        v = lib().orc_g_factor(self._h, x, C.byref(bad))
+        return v, bool(bad.value)
 
     # phases
     def laser_boundary(self, step):

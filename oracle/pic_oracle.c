@@ -32,6 +32,8 @@ typedef struct {
     int64_t n, cap;
     double *x, *y, *z, *xo, *yo, *zo, *ux, *uy, *uz, *w;
     int64_t* id;
+    double *sx, *sy, *sz; /* spin (precessed when spin is on and m == m_e) */
+    int64_t spin_errors;
 } orc_species;
 
 struct orc_ctx {
@@ -49,6 +51,11 @@ struct orc_ctx {
     double* tbuf; /* per-thread J + rho scratch, nthreads * 4 * jsize */
     double rho_bg;
     int64_t oob;
+    /* spin precession, particle.cpp:318-368 */
+    int spin_on;
+    double spin_mass, e_schwinger;
+    double* gtab;
+    int ngtab;
 };
 
 /* E/B index for buffer indices (k, j, i) in [0, n+2m-1]. */
@@ -129,13 +136,17 @@ orc_ctx* orc_create(const pic_params* p, int nthreads) {
 static void species_free(orc_species* s) {
     free(s->x); free(s->y); free(s->z); free(s->xo); free(s->yo); free(s->zo);
     free(s->ux); free(s->uy); free(s->uz); free(s->w); free(s->id);
+    free(s->sx); free(s->sy); free(s->sz);
     s->x = s->y = s->z = s->xo = s->yo = s->zo = s->ux = s->uy = s->uz = s->w = NULL;
+    s->sx = s->sy = s->sz = NULL;
     s->id = NULL;
     s->n = s->cap = 0;
+    s->spin_errors = 0;
 }
 
 void orc_destroy(orc_ctx* c) {
     if (!c) return;
+    free(c->gtab);
     for (int f = 0; f < 6; f++) free(c->F[f]);
     for (int f = 0; f < 3; f++) free(c->J[f]);
     for (int s = 0; s < PIC_MAX_SPECIES; s++) {
@@ -164,6 +175,9 @@ static void species_alloc(orc_species* s, int64_t n) {
     s->xo = (double*)malloc(b); s->yo = (double*)malloc(b); s->zo = (double*)malloc(b);
     s->ux = (double*)malloc(b); s->uy = (double*)malloc(b); s->uz = (double*)malloc(b);
     s->w = (double*)malloc(b);
+    s->sx = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    s->sy = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+    s->sz = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
     s->id = (int64_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
     s->cap = n;
     s->n = n;
@@ -611,6 +625,92 @@ static void deposit_one(const orc_ctx* c, double q, double wgt, double xo, doubl
 
 /* Returns 0 if pushed, 1 if the particle was outside the domain (counted,
  * left untouched: the reference would read outside its arrays). */
+/* ------------------------------------------------------------------ */
+/* spin precession, particle.cpp:318-368, with g_factor (603-637)      */
+/* ------------------------------------------------------------------ */
+
+int orc_spin_setup(orc_ctx* c, const double* g, int n, double e_schwinger, double m_e) {
+    if (n < 1) return PIC_ERR_ARG;
+    free(c->gtab);
+    c->gtab = (double*)malloc((size_t)n * sizeof(double));
+    memcpy(c->gtab, g, (size_t)n * sizeof(double));
+    c->ngtab = n;
+    c->e_schwinger = e_schwinger;
+    c->spin_mass = m_e;
+    c->spin_on = 1;
+    return PIC_OK;
+}
+
+void orc_spin_set(orc_ctx* c, int s, const double* sx, const double* sy, const double* sz) {
+    orc_species* S = &c->sp[s];
+    size_t b = (size_t)S->n * sizeof(double);
+    memcpy(S->sx, sx, b); memcpy(S->sy, sy, b); memcpy(S->sz, sz, b);
+}
+
+void orc_spin_get(const orc_ctx* c, int s, double* sx, double* sy, double* sz) {
+    const orc_species* S = &c->sp[s];
+    size_t b = (size_t)S->n * sizeof(double);
+    memcpy(sx, S->sx, b); memcpy(sy, S->sy, b); memcpy(sz, S->sz, b);
+}
+
+int64_t orc_spin_errors(const orc_ctx* c, int s) { return c->sp[s].spin_errors; }
+
+/* g_factor, particle.cpp:603-637.  Out of [0, 100] (or NaN) the reference
+ * prints and exits: *bad is set.  From chi = 99.9 its index passes the end
+ * of the table; here it stops at the last entry. */
+double orc_g_factor(const orc_ctx* c, double x, int* bad) {
+    int idx;
+    if (x < 0.005 && x >= 0.0) return 2.003219;
+    else if (x >= 0.005 && x < 0.01) idx = (int)((x - 0.005) / 0.0001);
+    else if (x >= 0.01 && x < 1) idx = (int)((x - 0.01) / 0.01) + 50;
+    else if (x >= 1 && x < 10) idx = (int)((x - 1.0) / 0.01) + 150;
+    else if (x >= 10 && x < 20) idx = (int)((x - 10.0) / 0.01) + 1050;
+    else if (x >= 20 && x <= 100) idx = (int)((x - 20.0) / 0.1) + 2050;
+    else { *bad = 1; return 2.0; }
+    if (idx > c->ngtab - 1) idx = c->ngtab - 1;
+    return c->gtab[idx];
+}
+
+/* one precession step of particle p (particle.cpp:321-367); e*, b*: the
+ * gathered fields times qm, as the Boris push holds them; u, gam: after
+ * the push.  Returns 1 where g_factor has no value (spin unchanged). */
+static int spin_one(const orc_ctx* c, orc_species* S, int64_t p, double qm, double exq, double eyq,
+                    double ezq, double bxq, double byq, double bzq, double ux, double uy, double uz,
+                    double gam) {
+    const double ex = exq / qm, ey = eyq / qm, ez = ezq / qm;
+    const double bx = bxq / qm, by = byq / qm, bz = bzq / qm;
+    const double vx = ux / gam, vy = uy / gam, vz = uz / gam;
+    const double px0 = vx * ex, py0 = vy * ey, pz0 = vz * ez;
+    const double fLx = (ex + vy * bz - vz * by);
+    const double fLy = (ey + vz * bx - vx * bz);
+    const double fLz = (ez + vx * by - vy * bx);
+    double chi = fLx * fLx + fLy * fLy + fLz * fLz - (px0 + py0 + pz0) * (px0 + py0 + pz0);
+    chi = 0.0 < chi ? chi : 0.0; /* std::max(0.0, chi) */
+    chi = sqrt(chi) / c->e_schwinger * gam;
+    int bad = 0;
+    const double g = orc_g_factor(c, chi, &bad);
+    if (bad) return 1;
+    const double alpha = -(g / 2.0 - 1.0) * (gam) / (1.0 + gam);
+    const double beta = (g / 2.0 - 1.0) + 1.0 / gam;
+    const double gm = g / 2.0 - gam / (1.0 + gam);
+    const double VB = vx * bx + vy * by + vz * bz;
+    const double ox = alpha * VB * vx + beta * bx + gm * (vy * ez - vz * ey);
+    const double oy = alpha * VB * vy + beta * by + gm * (vz * ex - vx * ez);
+    const double oz = alpha * VB * vz + beta * bz + gm * (vx * ey - vy * ex);
+    const double qmdt2 = S->q / S->m * (c->p.dt / 2.0);
+    const double tx = ox * qmdt2, ty = oy * qmdt2, tz = oz * qmdt2;
+    const double tt = 2.0 / (1.0 + tx * tx + ty * ty + tz * tz);
+    const double sx = S->sx[p], sy = S->sy[p], sz = S->sz[p];
+    const double sxr = sx + (sy * tz - sz * ty);
+    const double syr = sy + (sz * tx - sx * tz);
+    const double szr = sz + (sx * ty - sy * tx);
+    S->sx[p] = sx + (syr * tz - szr * ty) * tt;
+    S->sy[p] = sy + (szr * tx - sxr * tz) * tt;
+    S->sz[p] = sz + (sxr * ty - syr * tx) * tt;
+    return 0;
+}
+
+/* push_one's return: bit 0 = out of bounds, bit 1 = spin step skipped */
 static int push_one(const orc_ctx* c, orc_species* S, int64_t p, double* jx, double* jy,
                     double* jz, double* rho) {
     const pic_params* P = &c->p;
@@ -690,10 +790,13 @@ static int push_one(const orc_ctx* c, orc_species* S, int64_t p, double* jx, dou
     S->xo[p] = x; S->yo[p] = y; S->zo[p] = z;
     S->x[p] = xn; S->y[p] = yn; S->z[p] = zn;
     S->ux[p] = ux; S->uy[p] = uy; S->uz[p] = uz;
+    int spin_bad = 0;
+    if (c->spin_on && S->m == c->spin_mass)
+        spin_bad = spin_one(c, S, p, qm, exp_, eyp, ezp, bxp, byp, bzp, ux, uy, uz, gam);
     /* a non-finite momentum would put the deposit box outside the arrays */
-    if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) return 1;
+    if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) return 1 | (spin_bad << 1);
     deposit_one(c, S->q, S->w[p], x, y, z, xn, yn, zn, jx, jy, jz, rho);
-    return 0;
+    return spin_bad << 1;
 }
 
 /* currents_particle(p, s) for explicit old/new positions and weights,
@@ -720,24 +823,32 @@ void orc_push(orc_ctx* c, int s) {
     orc_species* S = &c->sp[s];
     if (S->immobile || S->n == 0) return;
     if (c->nthreads == 1) {
-        int64_t oob = 0;
-        for (int64_t p = 0; p < S->n; p++)
-            oob += push_one(c, S, p, c->J[0], c->J[1], c->J[2], c->rho[s]);
+        int64_t oob = 0, sbad = 0;
+        for (int64_t p = 0; p < S->n; p++) {
+            const int f = push_one(c, S, p, c->J[0], c->J[1], c->J[2], c->rho[s]);
+            oob += f & 1;
+            sbad += f >> 1;
+        }
         c->oob += oob;
+        S->spin_errors += sbad;
         return;
     }
 #ifdef _OPENMP
     const int nt = c->nthreads;
     const int64_t js = c->jsize;
     if (!c->tbuf) c->tbuf = (double*)malloc((size_t)nt * 4 * (size_t)js * sizeof(double));
-    int64_t oob = 0;
-#pragma omp parallel num_threads(nt) reduction(+ : oob)
+    int64_t oob = 0, sbad = 0;
+#pragma omp parallel num_threads(nt) reduction(+ : oob, sbad)
     {
         int t = omp_get_thread_num();
         double* b = c->tbuf + (size_t)t * 4 * (size_t)js;
         memset(b, 0, 4 * (size_t)js * sizeof(double));
         int64_t lo = S->n * t / nt, hi = S->n * (t + 1) / nt;
-        for (int64_t p = lo; p < hi; p++) oob += push_one(c, S, p, b, b + js, b + 2 * js, b + 3 * js);
+        for (int64_t p = lo; p < hi; p++) {
+            const int f = push_one(c, S, p, b, b + js, b + 2 * js, b + 3 * js);
+            oob += f & 1;
+            sbad += f >> 1;
+        }
 #pragma omp barrier
         /* reduce thread buffers in thread order, split over elements */
         int64_t e0 = js * t / nt, e1 = js * (t + 1) / nt;
@@ -749,6 +860,7 @@ void orc_push(orc_ctx* c, int s) {
             }
     }
     c->oob += oob;
+    S->spin_errors += sbad;
 #endif
 }
 
@@ -759,8 +871,9 @@ void orc_push(orc_ctx* c, int s) {
 
 static void permute(orc_species* S, const int64_t* idx, int64_t n) {
     double* tmp = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
-    double* arrs[10] = {S->x, S->y, S->z, S->xo, S->yo, S->zo, S->ux, S->uy, S->uz, S->w};
-    for (int a = 0; a < 10; a++) {
+    double* arrs[13] = {S->x, S->y, S->z, S->xo, S->yo, S->zo, S->ux, S->uy, S->uz, S->w,
+                        S->sx, S->sy, S->sz};
+    for (int a = 0; a < 13; a++) {
         for (int64_t q = 0; q < n; q++) tmp[q] = arrs[a][idx[q]];
         memcpy(arrs[a], tmp, (size_t)n * sizeof(double));
     }

@@ -58,6 +58,13 @@ void orc_diag(const orc_ctx* c, pic_diag_out* out);
 int64_t orc_out_of_bounds(const orc_ctx* c);
 
 /* Individual shape functions (particle.cpp:8-48), for closed-form tests. */
+/* spin precession, particle.cpp:318-368 (species of mass m_e) */
+int orc_spin_setup(orc_ctx* c, const double* g, int n, double e_schwinger, double m_e);
+void orc_spin_set(orc_ctx* c, int s, const double* sx, const double* sy, const double* sz);
+void orc_spin_get(const orc_ctx* c, int s, double* sx, double* sy, double* sz);
+int64_t orc_spin_errors(const orc_ctx* c, int s);
+double orc_g_factor(const orc_ctx* c, double x, int* bad); /* particle.cpp:603-637 */
+
 void orc_shape2(double x, double* w3);
 void orc_shape4(double x, double* w5);
 /* Gpulse (boundary.cpp:29-40) for the context's laser parameters. */

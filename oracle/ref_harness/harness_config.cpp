@@ -46,6 +46,10 @@ double D(const char* k) {
     return std::strtod(it->second.c_str(), nullptr);
 }
 int I(const char* k) { return (int)D(k); }
+double D_or(const char* k, double dflt) {
+    auto it = cfg().find(k);
+    return it == cfg().end() ? dflt : std::strtod(it->second.c_str(), nullptr);
+}
 }  // namespace
 
 extern const double Pi = D("pi");
@@ -114,7 +118,8 @@ int nc_zst_new, nc_zend_new;
 int nc_zst_out, nc_zend_out;
 double zst, zend;
 
-double E_schwinger = 4.1215e5;
+// constants.cpp:83; a case may lower it so that chi reaches the g.txt table
+double E_schwinger = D_or("E_schwinger", 4.1215e5);
 extern const double alpha0 = 55.0 / 96.0, alpha1 = 5.0 / 24.0, alpha2 = 5.0 / 4.0,
                     alpha3 = 5.0 / 6.0, beta0 = 115.0 / 192.0, beta2 = 5.0 / 8.0;
 

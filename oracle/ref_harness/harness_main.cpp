@@ -18,6 +18,11 @@
 //        xfield:C            exchange_boundary_data(ex|ey|ez|bx|by|bz [C])
 //        xcur:C              exchange_boundary_current_fixed(jx|jy|jz|rho0|rho1 [C])
 //        dep:K               currents_particle(p, K) for every particle of kind K
+//        gfac:PATH           g_factor(x) for the doubles x in PATH, into PATH.out
+//
+// With PIC_REF_SPIN=1 in the environment the input carries sx sy sz (n
+// doubles each) after the ids of each species, and the output sx sy sz
+// gamma after the ids.
 //
 // MPI_PROC_NULL is -1 under MPICH but the reference tests for OpenMPI's -2
 // (boundary.cpp:87,92; mpi_utils.cpp:16-50,80-222).  On an open x axis the
@@ -100,6 +105,8 @@ int main(int argc, char** argv) {
 
     // input: per species int32 mobile, double q, double m, int64 n, then
     // x y z ux uy uz w (n doubles each) and id (n int64)
+    const char* spin_env = std::getenv("PIC_REF_SPIN");
+    const bool spin_io = spin_env && std::atoi(spin_env) == 1;
     FILE* in = std::fopen(argv[1], "rb");
     if (!in) return 3;
     std::vector<std::vector<particle>> P(2);
@@ -119,6 +126,11 @@ int main(int argc, char** argv) {
         }
         std::vector<int64_t> id(cnt);
         if (cnt && std::fread(id.data(), 8, cnt, in) != (size_t)cnt) return 4;
+        std::vector<double> sp[3];
+        for (int f = 0; f < 3 && spin_io; f++) {
+            sp[f].resize(cnt);
+            if (cnt && std::fread(sp[f].data(), 8, cnt, in) != (size_t)cnt) return 4;
+        }
         P[s].resize(cnt);
         for (int64_t p = 0; p < cnt; p++) {
             particle& q_ = P[s][p];
@@ -134,10 +146,35 @@ int main(int argc, char** argv) {
             q_.charge = q;
             q_.mass = m;
             q_.rank = g_rank;
+            if (spin_io) {
+                q_.sx = sp[0][p];
+                q_.sy = sp[1][p];
+                q_.sz = sp[2][p];
+            }
         }
         n[s] = (int)cnt;
     }
     std::fclose(in);
+
+    for (int c = 3; c < argc; c++) {
+        std::string cmd = argv[c];
+        if (cmd.rfind("gfac:", 0) != 0) continue;
+        // g_factor on its own (particle.cpp:603-637): values in range only
+        const std::string path = cmd.substr(5);
+        FILE* gi = std::fopen(path.c_str(), "rb");
+        if (!gi) return 7;
+        std::vector<double> xs;
+        double v;
+        while (std::fread(&v, 8, 1, gi) == 1) xs.push_back(v);
+        std::fclose(gi);
+        FILE* go = std::fopen((path + ".out").c_str(), "wb");
+        if (!go) return 7;
+        for (double x : xs) {
+            const double g = g_factor(x);
+            std::fwrite(&g, 8, 1, go);
+        }
+        std::fclose(go);
+    }
 
     for (int c = 3; c < argc; c++) {
         std::string cmd = argv[c];
@@ -189,6 +226,8 @@ int main(int argc, char** argv) {
         } else if (cmd.rfind("dep:", 0) == 0) {
             int s = std::atoi(cmd.c_str() + 4);
             for (int p = 0; p < n[s]; p++) currents_particle(P[s][p], s);
+        } else if (cmd.rfind("gfac:", 0) == 0) {
+            // done above
         } else {
             std::fprintf(stderr, "harness: unknown command %s\n", cmd.c_str());
             return 5;
@@ -219,6 +258,12 @@ int main(int argc, char** argv) {
             int64_t id = P[s][p].id;
             std::fwrite(&id, 8, 1, out);
         }
+        for (int f = 0; f < 4 && spin_io; f++)
+            for (int64_t p = 0; p < cnt; p++) {
+                const particle& q_ = P[s][p];
+                const double v[4] = {q_.sx, q_.sy, q_.sz, q_.gamma};
+                std::fwrite(&v[f], 8, 1, out);
+            }
     }
     std::fclose(out);
     MPI_Finalize();

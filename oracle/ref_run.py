@@ -46,9 +46,12 @@ def _cfg_text(p: PicParams) -> str:
     return "".join(f"{k} {v}\n" for k, v in kv.items())
 
 
-def run(p: PicParams, species: list, cmds: list, workdir: str | None = None) -> dict:
+def run(p: PicParams, species: list, cmds: list, workdir: str | None = None, spin: bool = False,
+        e_schwinger: float | None = None) -> dict:
     """species: two entries of (mobile, q, m, parts-dict or None).
-    Returns fields (ex..bz, jx, jy, jz, rho0, rho1) and particles per species."""
+    Returns fields (ex..bz, jx, jy, jz, rho0, rho1) and particles per species.
+    spin: the parts carry sx, sy, sz, and the output particles sx, sy, sz
+    and gamma; e_schwinger overrides E_schwinger (constants.cpp:83)."""
     assert p.laser_right == 0, "harness drives radiating_boundary with is_right = 0"
     td = workdir or tempfile.mkdtemp(prefix="picref_")
     cfg = os.path.join(td, "cfg.txt")
@@ -56,6 +59,8 @@ def run(p: PicParams, species: list, cmds: list, workdir: str | None = None) -> 
     fout = os.path.join(td, "out.bin")
     with open(cfg, "w") as f:
         f.write(_cfg_text(p))
+        if e_schwinger is not None:
+            f.write(f"E_schwinger {float(e_schwinger).hex()}\n")
     with open(fin, "wb") as f:
         for mobile, q, m, parts in species:
             n = 0 if parts is None else len(parts["x"])
@@ -67,7 +72,10 @@ def run(p: PicParams, species: list, cmds: list, workdir: str | None = None) -> 
                     np.ascontiguousarray(parts[k], dtype=np.float64).tofile(f)
             if n:
                 np.ascontiguousarray(parts["id"], dtype=np.int64).tofile(f)
-    env = dict(os.environ, PIC_REF_CFG=cfg)
+            if spin and n:
+                for k in ("sx", "sy", "sz"):
+                    np.ascontiguousarray(parts[k], dtype=np.float64).tofile(f)
+    env = dict(os.environ, PIC_REF_CFG=cfg, PIC_REF_SPIN="1" if spin else "0")
     gtxt = os.path.join(REF_ROOT, "g.txt")
     if os.path.exists(gtxt):
         env["PIC_REF_GTXT"] = gtxt
@@ -98,5 +106,20 @@ def run(p: PicParams, species: list, cmds: list, workdir: str | None = None) -> 
             off += 8 * cnt
         d["id"] = np.frombuffer(raw, np.int64, cnt, off).copy()
         off += 8 * cnt
+        if spin:
+            for k in ("sx", "sy", "sz", "gamma"):
+                d[k] = np.frombuffer(raw, np.float64, cnt, off).copy()
+                off += 8 * cnt
         out[f"p{s}"] = d
     return out
+
+
+def g_factor(p: PicParams, xs) -> np.ndarray:
+    """The reference's g_factor (particle.cpp:603-637, with g.txt) at each x
+    (all inside [0, 100]: the reference exits outside)."""
+    td = tempfile.mkdtemp(prefix="picref_g_")
+    path = os.path.join(td, "x.bin")
+    np.ascontiguousarray(xs, dtype=np.float64).tofile(path)
+    r = run(p, [(0, -1.0, 1.0, None), (0, 1.0, 1836.0, None)], [f"gfac:{path}"], workdir=td)
+    del r
+    return np.fromfile(path + ".out", dtype=np.float64)

@@ -107,6 +107,11 @@ SIGNATURES = {
     "pic_particles_get": (C.c_int, [C.c_void_p, C.c_int, _ip] + [_dp] * 7 + [_ip] + [_dp] * 3),
     "pic_particles_load": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PicLoader)]),
     "pic_num_particles": (C.c_int, [C.c_void_p, C.c_int, _ip]),
+    "pic_spin_setup": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double]),
+    "pic_spin_set": (C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 3),
+    "pic_spin_get": (C.c_int, [C.c_void_p, C.c_int, _ip] + [_dp] * 3),
+    "pic_spin_fill": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double]),
+    "pic_spin_errors": (C.c_int, [C.c_void_p, C.c_int, _ip]),
     "pic_laser_boundary": (C.c_int, [C.c_void_p, C.c_int]),
     "pic_fields_first": (C.c_int, [C.c_void_p]),
     "pic_fields_second": (C.c_int, [C.c_void_p]),

@@ -6,6 +6,8 @@
 // the process with a message, as the reference's exit()/MPI_Abort paths do.
 #include "pic3d_compat.h"
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,6 +38,7 @@ bool g_declared[PIC_MAX_SPECIES];
 double g_q[PIC_MAX_SPECIES], g_m[PIC_MAX_SPECIES];
 int64_t g_cap[PIC_MAX_SPECIES];
 std::map<const void*, int> g_kind_of;  // particle vector -> kind (from the push calls)
+bool g_spin_ready = false;            // pic_spin_setup done with the caller's g_data
 
 [[noreturn]] void fail(const char* what, int rc) {
     std::fprintf(stderr, "pic3d_compat: %s failed (%d): %s\n", what, rc,
@@ -117,8 +120,31 @@ void ensure_species(int kind, double q, double m, int64_t n) {
     if (kind >= g_nspecies) g_nspecies = kind + 1;
 }
 
+// The reference precesses the spin of every particle of mass m_e
+// (particle.cpp:318-368) with the g_data table main() loads from g.txt
+// (constants.cpp:114-136) and E_schwinger: hand them to the device once,
+// at the first push (after main's load_g_data).
+void ensure_spin() {
+    if (g_spin_ready) return;
+    chk(pic_spin_setup(ctx(), g_data, g_data_size, E_schwinger, m_e), "pic_spin_setup");
+    g_spin_ready = true;
+}
+
+// g_factor outside its table stops the reference (particle.cpp:632-636);
+// the device skips such a precession step and counts it
+void check_spin(int kind) {
+    if (!g_spin_ready || !g_declared[kind] || g_m[kind] != m_e) return;
+    int64_t bad = 0;
+    chk(pic_spin_errors(ctx(), kind, &bad), "pic_spin_errors");
+    if (bad) {
+        std::printf("Error: x value out of g_data range! (%lld precession steps)\n", (long long)bad);
+        std::exit(1);
+    }
+}
+
 // Upload a particle vector as species 'kind' (device id = vector index).
 void upload(int kind, const std::vector<particle>& P, int64_t n) {
+    ensure_spin();
     if (n == 0) {  // an empty vector: nothing on the device either
         if (g_declared[kind]) {
             double z = 0.0;
@@ -147,12 +173,17 @@ void upload(int kind, const std::vector<particle>& P, int64_t n) {
     chk(pic_particles_set(ctx(), kind, n, a[0].data(), a[1].data(), a[2].data(), a[3].data(),
                           a[4].data(), a[5].data(), a[6].data(), id.data()),
         "pic_particles_set");
+    if (m == m_e) {
+        for (int64_t i = 0; i < n; i++) { a[0][i] = P[i].sx; a[1][i] = P[i].sy; a[2][i] = P[i].sz; }
+        chk(pic_spin_set(ctx(), kind, n, a[0].data(), a[1].data(), a[2].data()), "pic_spin_set");
+    }
 }
 
 struct Down {
-    std::vector<double> x, y, z, ux, uy, uz, w, xo, yo, zo;
+    std::vector<double> x, y, z, ux, uy, uz, w, xo, yo, zo, sx, sy, sz;
     std::vector<int64_t> id;
     int64_t n = 0;
+    bool spin = false;
 };
 Down download(int kind) {
     Down d;
@@ -167,6 +198,12 @@ Down download(int kind) {
                           d.yo.data(), d.zo.data()),
         "pic_particles_get");
     d.n = cap;
+    if (g_declared[kind] && g_m[kind] == m_e && g_spin_ready) {
+        for (auto* v : {&d.sx, &d.sy, &d.sz}) v->resize(std::max<int64_t>(d.n, 1));
+        int64_t ns = std::max<int64_t>(d.n, 1);
+        chk(pic_spin_get(ctx(), kind, &ns, d.sx.data(), d.sy.data(), d.sz.data()), "pic_spin_get");
+        d.spin = true;
+    }
     return d;
 }
 
@@ -178,6 +215,7 @@ void store(particle& q, const Down& d, int64_t j, int rank) {
     q.x_old = d.xo[j]; q.y_old = d.yo[j]; q.z_old = d.zo[j];
     q.px = d.ux[j]; q.py = d.uy[j]; q.pz = d.uz[j];
     q.gamma = std::sqrt(1.0 + (q.px * q.px + q.py * q.py + q.pz * q.pz));
+    if (d.spin) { q.sx = d.sx[j]; q.sy = d.sy[j]; q.sz = d.sz[j]; }
     q.rank = rank;
 }
 
@@ -415,6 +453,7 @@ void run_main_time_loop(std::vector<std::vector<particle>>& particles, int* num_
         int next = timesteps;
         if (output_interval > 0) next = std::min(timesteps, (step / output_interval + 1) * output_interval);
         chk(pic_step(ctx(), step + 1, next - step), "pic_step");
+        for (int k = 0; k < nk && k < PIC_MAX_SPECIES; k++) check_spin(k);
         step = next;
         if (output_interval > 0 && step % output_interval == 0) {
             refresh();
@@ -515,15 +554,16 @@ void initialize_particles(std::vector<particle>& particles, int& num_particles,
     num_particles = n;
 }
 
-// particle.cpp:113-370: wrap, gather, Boris push, Esirkepov deposit of
-// every particle of the vector on the GPU.  The spin precession of
-// particle.cpp:318-368 is not applied (sx, sy, sz are carried unchanged).
+// particle.cpp:113-370: wrap, gather, Boris push, Esirkepov deposit and
+// (particles of mass m_e) spin precession of every particle of the vector
+// on the GPU.
 void update_particle_position_momentum(std::vector<particle>& particles, int num_particles,
                                        int my_rank, int kind) {
     g_kind_of[&particles] = kind;
     if (num_particles <= 0) return;
     upload(kind, particles, num_particles);
     chk(pic_push(ctx(), kind), "pic_push");
+    check_spin(kind);
     Down d = download(kind);
     if (g_p.precision != PIC_FP64) {
         // the FP32 form keeps no x_old: it is the start position after the
@@ -586,7 +626,8 @@ double g_factor(double x) {
         std::printf("Error: x value out of g_data range! x = %.2f\n", x);
         std::exit(1);
     }
-    return g_data[idx];
+    // from x = 99.9 the reference's index passes the end of the table
+    return g_data[std::min(idx, g_data_size - 1)];
 }
 
 // =====================================================================

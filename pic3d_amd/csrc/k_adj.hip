@@ -1659,6 +1659,8 @@ int pic_diff_begin(pic_ctx* c, int s) {
     if (!c || s < 0 || s >= PIC_MAX_SPECIES) return PIC_ERR_ARG;
     Species& S = c->sp[s];
     if (!S.declared || S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is not a mobile species", s);
+    if (S.spin)
+        return set_err(c, PIC_ERR_STATE, "differentiated run: species %d precesses its spin (not differentiated)", s);
     for (int o = 0; o < PIC_MAX_SPECIES; o++)
         if (o != s && c->sp[o].declared && !c->sp[o].immobile && c->sp[o].cap > 0)
             return set_err(c, PIC_ERR_STATE, "differentiated run: species %d is mobile too (one mobile species)", o);

@@ -42,6 +42,7 @@ __device__ __forceinline__ int64_t bin_of64(double x, double y, double z, const 
 struct Soa64 {
     double *x, *y, *z, *ux, *uy, *uz, *w, *xo, *yo, *zo;
     int64_t* id;
+    double *sx, *sy, *sz;  // spin, may be null
 };
 
 constexpr int HIST_R64 = 4;  // rows of 64 slots per wave and block iteration
@@ -206,7 +207,8 @@ static TileArgs tile_args(const pic_ctx* c) {
 
 static Soa64 soa64(pic_ctx* c, Species& S, int b) {
     return Soa64{S.x[b], S.y[b], S.z[b], S.ux[b], S.uy[b], S.uz[b], S.w[b],
-                 S.xo[b], S.yo[b], S.zo[b], S.id[b]};
+                 S.xo[b], S.yo[b], S.zo[b], S.id[b],
+                 (double*)S.spin_v[0][b], (double*)S.spin_v[1][b], (double*)S.spin_v[2][b]};
 }
 
 // perm != null: removed slots go after the last tile (bin ntiles) and
@@ -244,6 +246,7 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
             B.ux[q] = A.ux[p]; B.uy[q] = A.uy[p]; B.uz[q] = A.uz[p];
             B.id[q] = A.id[p];
             if (keep_old) { B.xo[q] = A.xo[p]; B.yo[q] = A.yo[p]; B.zo[q] = A.zo[p]; }
+            if (A.sx) { B.sx[q] = A.sx[p]; B.sy[q] = A.sy[p]; B.sz[q] = A.sz[p]; }
             if (perm) perm[q] = (int32_t)p;
         }
     }

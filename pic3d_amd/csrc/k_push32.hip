@@ -32,10 +32,12 @@ struct Soa32 {
     int32_t* cell;
     float *fx, *fy, *fz, *ux, *uy, *uz, *w;
     int64_t* id;
+    float *sx, *sy, *sz;  // spin (species with precession only)
 };
 
 static Soa32 soa32(Species& S, int b) {
-    return Soa32{S.cell[b], S.fx[b], S.fy[b], S.fz[b], S.vx[b], S.vy[b], S.vz[b], S.fw[b], S.id[b]};
+    return Soa32{S.cell[b], S.fx[b], S.fy[b], S.fz[b], S.vx[b], S.vy[b], S.vz[b], S.fw[b], S.id[b],
+                 (float*)S.spin_v[0][b], (float*)S.spin_v[1][b], (float*)S.spin_v[2][b]};
 }
 
 struct Consts32 {
@@ -54,6 +56,7 @@ struct Consts32 {
     int per[3];
     int m;
     CellSplit cs;  // cell index -> (cx, cy, cz)
+    SpinConsts spin;
 };
 
 // quadratic/quartic weights for an offset delta = x - round(x) in [-0.5, 0.5)
@@ -323,14 +326,21 @@ __device__ __forceinline__ int move32(int cx, int cy, int cz, float* f, const fl
 // one particle: gather (through FR), Boris, deposit (into SINK), move.
 // Returns false when the particle was skipped as out of bounds.
 // ---------------------------------------------------------------------
-template <int M, typename FR, typename SINK>
+//
+// SPIN: the spin sp[3] precesses after the push (particle.cpp:318-368, in
+// FP32); spin_bad counts steps without a g-factor value.
+template <int M, bool SPIN, typename FR, typename SINK>
 __device__ __forceinline__ bool push32_core(int cx, int cy, int cz, int m, float* f, float* u,
                                             float w, const Consts32& K, FR fr, SINK& sink,
-                                            int& newcell, unsigned* cnt12) {
+                                            int& newcell, unsigned* cnt12, float* sp, int& spin_bad) {
     const int Cx = cx + m - 1, Cy = cy + m - 1, Cz = cz + m - 1;  // buffer index of the cell
     float E[3], Bf[3], W0[3][M + 1];
     gather32<M>(Cx, Cy, Cz, f[0], f[1], f[2], E, Bf, fr, W0[0], W0[1], W0[2]);
     boris32(u, E, Bf, K.qm);
+    if constexpr (SPIN) {
+        const float gam = sqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+        if (!spin_precess<float>(E, Bf, K.qm, u[0], u[1], u[2], gam, K.spin, sp[0], sp[1], sp[2])) spin_bad++;
+    }
     const float ig = rsqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
     float d[3] = {u[0] * ig * K.cfx, u[1] * ig * K.cfy, u[2] * ig * K.cfz};
     // a non-finite momentum would put the deposit box outside the arrays
@@ -363,7 +373,7 @@ struct Fields32 {
     float *jx, *jy, *jz, *rho;
 };
 
-template <int M, int TX, int TY, int TZ, typename ACC>
+template <int M, int TX, int TY, int TZ, typename ACC, bool SPIN>
 __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid g, Consts32 K,
                                                       Tiling tl, const int64_t* __restrict__ tile_off,
                                                       const int64_t* __restrict__ work,
@@ -399,6 +409,7 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
     __syncthreads();
     const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
     int64_t oob = 0;
+    int spin_bad = 0;
     uint32_t strays = 0;  // iterations whose particle was outside the tile
     int it = 0;
     const int len = (int)(end - beg);
@@ -425,10 +436,16 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
         LdsSink32<B::AX, B::AY, B::AVOL, ACC> sink{acc, K.inv_ufix,
                                               ((cz + m - 1 - aoz) * B::AY + (cy + m - 1 - aoy)) * B::AX + (cx + m - 1 - aox), 0};
         int nc;
-        if (!push32_core<M>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12)) { oob++; continue; }
+        float sp[3];
+        if (SPIN) { sp[0] = P.sx[p]; sp[1] = P.sy[p]; sp[2] = P.sz[p]; }
+        if (!push32_core<M, SPIN>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12, sp, spin_bad)) {
+            oob++;
+            continue;
+        }
         P.cell[p] = nc;
         P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
         P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
+        if (SPIN) { P.sx[p] = sp[0]; P.sy[p] = sp[1]; P.sz[p] = sp[2]; }
     }
     if (strays) {  // HBM path for the particles outside the tile
         float* jdst[4] = {F.jx, F.jy, F.jz, F.rho};
@@ -444,10 +461,16 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
             auto fr = [&](int c, int k, int j, int i) -> float { return fsrc[c][g.fi(k, j, i)]; };
             HbmSink32 sink{{jdst[0], jdst[1], jdst[2], jdst[3]}, g, cz + m - 1, cy + m - 1, cx + m - 1, 0, 0, 0};
             int nc;
-            if (!push32_core<M>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12)) { oob++; continue; }
+            float sp[3];
+            if (SPIN) { sp[0] = P.sx[p]; sp[1] = P.sy[p]; sp[2] = P.sz[p]; }
+            if (!push32_core<M, SPIN>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12, sp, spin_bad)) {
+                oob++;
+                continue;
+            }
             P.cell[p] = nc;
             P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
             P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
+            if (SPIN) { P.sx[p] = sp[0]; P.sy[p] = sp[1]; P.sz[p] = sp[2]; }
         }
     }
     __syncthreads();
@@ -464,10 +487,11 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
     if (threadIdx.x < 12 && cnt12[threadIdx.x])
         atomicAdd((unsigned long long*)&cnt[CNT_PEND + threadIdx.x], (unsigned long long)cnt12[threadIdx.x]);
     if (oob) atomicAdd((unsigned long long*)&cnt[CNT_OOB], (unsigned long long)oob);
+    if (SPIN && spin_bad) atomicAdd((unsigned long long*)&cnt[CNT_SPIN], (unsigned long long)spin_bad);
 }
 
 // direct push: any particle order, gather from HBM, global atomics
-template <int M>
+template <int M, bool SPIN>
 __global__ __launch_bounds__(256) void k_push32_direct(Soa32 P, Fields32 F, Grid g, Consts32 K,
                                                        int64_t* __restrict__ cnt) {
     __shared__ unsigned cnt12[12];
@@ -477,6 +501,7 @@ __global__ __launch_bounds__(256) void k_push32_direct(Soa32 P, Fields32 F, Grid
     const float* fsrc[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
     const int m = g.m;
     int64_t oob = 0;
+    int spin_bad = 0;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
          p += (int64_t)gridDim.x * blockDim.x) {
         const int cell = P.cell[p];
@@ -488,15 +513,22 @@ __global__ __launch_bounds__(256) void k_push32_direct(Soa32 P, Fields32 F, Grid
         auto fr = [&](int c, int k, int j, int i) -> float { return fsrc[c][g.fi(k, j, i)]; };
         HbmSink32 sink{{F.jx, F.jy, F.jz, F.rho}, g, cz + m - 1, cy + m - 1, cx + m - 1, 0, 0, 0};
         int nc;
-        if (!push32_core<M>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12)) { oob++; continue; }
+        float sp[3];
+        if (SPIN) { sp[0] = P.sx[p]; sp[1] = P.sy[p]; sp[2] = P.sz[p]; }
+        if (!push32_core<M, SPIN>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12, sp, spin_bad)) {
+            oob++;
+            continue;
+        }
         P.cell[p] = nc;
         P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
         P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
+        if (SPIN) { P.sx[p] = sp[0]; P.sy[p] = sp[1]; P.sz[p] = sp[2]; }
     }
     __syncthreads();
     if (threadIdx.x < 12 && cnt12[threadIdx.x])
         atomicAdd((unsigned long long*)&cnt[CNT_PEND + threadIdx.x], (unsigned long long)cnt12[threadIdx.x]);
     if (oob) atomicAdd((unsigned long long*)&cnt[CNT_OOB], (unsigned long long)oob);
+    if (SPIN && spin_bad) atomicAdd((unsigned long long*)&cnt[CNT_SPIN], (unsigned long long)spin_bad);
 }
 
 // ---------------------------------------------------------------------
@@ -574,7 +606,7 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
     for (int64_t b = lo; b < hi; b += (int64_t)blockDim.x * R) {
         int cell[R], head[R];
         int64_t key[R], base[R], id[R];
-        float v[R][7];
+        float v[R][7], sv[R][3];
         bool valid[R];
         // every row's loads in flight together
 #pragma unroll
@@ -587,6 +619,7 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
                 v[r][3] = A.ux[p]; v[r][4] = A.uy[p]; v[r][5] = A.uz[p];
                 v[r][6] = A.w[p];
                 id[r] = A.id[p];
+                if (A.sx) { sv[r][0] = A.sx[p]; sv[r][1] = A.sy[p]; sv[r][2] = A.sz[p]; }
             }
         }
         bool uni = true;
@@ -625,6 +658,7 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
             Bo.ux[q] = v[r][3]; Bo.uy[q] = v[r][4]; Bo.uz[q] = v[r][5];
             Bo.w[q] = v[r][6];
             Bo.id[q] = id[r];
+            if (A.sx) { Bo.sx[q] = sv[r][0]; Bo.sy[q] = sv[r][1]; Bo.sz[q] = sv[r][2]; }
             if (perm) perm[q] = (int32_t)p;
         }
     }
@@ -784,6 +818,7 @@ static Consts32 consts32(const pic_ctx* c, const Species& S) {
     for (int a = 0; a < 3; a++) { K.n[a] = c->g.n[a]; K.per[a] = c->p.periodic[a]; }
     K.m = c->g.m;
     K.cs = CellSplit(c->g.n[0], c->g.n[1]);
+    K.spin = c->spin;
     const double qw = std::fabs(S.q) * (S.wmax > 0 ? S.wmax : 1.0);
     const double bound[4] = {qw * c->p.dx / c->p.dt, qw * c->p.dy / c->p.dt, qw * c->p.dz / c->p.dt, qw};
     for (int k = 0; k < 4; k++) {
@@ -794,17 +829,23 @@ static Consts32 consts32(const pic_ctx* c, const Species& S) {
     return K;
 }
 
-template <int M, int KIND, typename ACC>
-static int launch_tiled32(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
+template <int M, int KIND, typename ACC, bool SPIN>
+static int launch_tiled32_s(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
     constexpr int TX = TileShape<KIND, M>::X, TY = TileShape<KIND, M>::Y, TZ = TileShape<KIND, M>::Z;
     using B = Box32<M, TX, TY, TZ, ACC>;
     const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
-    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push32_tiled<M, TX, TY, TZ, ACC>,
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push32_tiled<M, TX, TY, TZ, ACC, SPIN>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
-    k_push32_tiled<M, TX, TY, TZ, ACC><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
+    k_push32_tiled<M, TX, TY, TZ, ACC, SPIN><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
         soa32(S, S.cur), Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, S.d_cnt);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
+}
+
+template <int M, int KIND, typename ACC>
+static int launch_tiled32(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
+    return S.spin ? launch_tiled32_s<M, KIND, ACC, true>(c, S, Fd, K)
+                  : launch_tiled32_s<M, KIND, ACC, false>(c, S, Fd, K);
 }
 
 int launch_push32(pic_ctx* c, int s) {
@@ -817,8 +858,14 @@ int launch_push32(pic_ctx* c, int s) {
     if (kernel == 1) {
         PhaseTimer t(c, PIC_T_PUSH);
         const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 64);
-        if (c->g.m == 2) k_push32_direct<2><<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), Fd, c->g, K, S.d_cnt);
-        else k_push32_direct<4><<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), Fd, c->g, K, S.d_cnt);
+        const Soa32 P = soa32(S, S.cur);
+        if (S.spin) {
+            if (c->g.m == 2) k_push32_direct<2, true><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K, S.d_cnt);
+            else k_push32_direct<4, true><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K, S.d_cnt);
+        } else {
+            if (c->g.m == 2) k_push32_direct<2, false><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K, S.d_cnt);
+            else k_push32_direct<4, false><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K, S.d_cnt);
+        }
         PIC_LAUNCH_CHECK(c);
         return PIC_OK;
     }

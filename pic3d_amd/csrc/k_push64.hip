@@ -46,7 +46,8 @@ template <int M, typename FR>
 __device__ __forceinline__ void gather_boris(double x, double y, double z, double& ux, double& uy,
                                              double& uz, double& xn, double& yn, double& zn,
                                              const PushConsts& K, int m, FR fr, double* Wxp, double* Wyp,
-                                             double* Wzp, int& ip, int& jp, int& kp) {
+                                             double* Wzp, int& ip, int& jp, int& kp, double* ebq,
+                                             double& gout) {
     constexpr int H = M / 2;
     const double org = (double)(1 - m);
     const double xc = x - org, yc = y - org, zc = z - org;
@@ -97,10 +98,26 @@ __device__ __forceinline__ void gather_boris(double x, double y, double z, doubl
     uy = py3 + e1;
     uz = pz3 + e2;
     const double gam = sqrt(1.0 + (ux * ux + uy * uy + uz * uz));
+    ebq[0] = e0; ebq[1] = e1; ebq[2] = e2; ebq[3] = b0; ebq[4] = b1; ebq[5] = b2;
+    gout = gam;
     // particle.cpp:313-315
     xn = x + ux / gam * K.dt / K.dx;
     yn = y + uy / gam * K.dt / K.dy;
     zn = z + uz / gam * K.dt / K.dz;
+}
+
+// Spin precession of slot p after its push (particle.cpp:318-368); the
+// species' mass is the electron mass (checked on the host).  Returns 1
+// where g_factor has no value (the reference exits there).
+__device__ __forceinline__ int spin_step(const P64& P, int64_t p, const double* ebq, double ux, double uy,
+                                         double uz, double gam, const PushConsts& K) {
+    double sx = P.sx[p], sy = P.sy[p], sz = P.sz[p];
+    // particle.cpp:321-326: the fields divided back by q/m dt/2
+    const double E[3] = {ebq[0] / K.qm, ebq[1] / K.qm, ebq[2] / K.qm};
+    const double B[3] = {ebq[3] / K.qm, ebq[4] / K.qm, ebq[5] / K.qm};
+    if (!spin_precess<double>(E, B, K.qm, ux, uy, uz, gam, K.spin, sx, sy, sz)) return 1;
+    P.sx[p] = sx; P.sy[p] = sy; P.sz[p] = sz;
+    return 0;
 }
 
 // Esirkepov deposit of one particle moving (xo,yo,zo) -> (xn,yn,zn)
@@ -261,9 +278,9 @@ struct LdsSink {  // LDS accumulator box with origin (oz, oy, ox)
 // Full push of one particle against HBM fields and HBM currents (the
 // direct kernel, and the tiled kernel's path for a particle outside its
 // tile).  Returns 1 if the particle was skipped as out of bounds.
-template <int M>
+template <int M, bool SPIN>
 __device__ __forceinline__ int push_one_hbm(const P64& P, int64_t p, double wp, const Fields64& F,
-                                         const Grid& g, const PushConsts& K) {
+                                         const Grid& g, const PushConsts& K, int& spin_bad) {
     double x = P.x[p], y = P.y[p], z = P.z[p];
     if (!wrap_check(x, y, z, K)) return 1;
     const double* fp[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
@@ -271,10 +288,12 @@ __device__ __forceinline__ int push_one_hbm(const P64& P, int64_t p, double wp, 
     double ux = P.ux[p], uy = P.uy[p], uz = P.uz[p], xn, yn, zn;
     double Sx0[M + 1], Sy0[M + 1], Sz0[M + 1];
     int i1, j1, k1;
-    gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr, Sx0, Sy0, Sz0, i1, j1, k1);
+    double ebq[6], gam;
+    gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr, Sx0, Sy0, Sz0, i1, j1, k1, ebq, gam);
     P.x[p] = xn; P.y[p] = yn; P.z[p] = zn;
     P.ux[p] = ux; P.uy[p] = uy; P.uz[p] = uz;
     if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
+    if (SPIN) spin_bad += spin_step(P, p, ebq, ux, uy, uz, gam, K);
     // a non-finite momentum would put the deposit box outside the arrays
     if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) return 1;
     HbmSink d{{F.jx, F.jy, F.jz, F.rho}, g, 0, 0, 0};
@@ -318,17 +337,19 @@ __global__ __launch_bounds__(256) void k_deposit64(const double* __restrict__ xo
 // ---------------------------------------------------------------------
 // direct kernel: any particle order, global atomics
 // ---------------------------------------------------------------------
-template <int M>
+template <int M, bool SPIN>
 __global__ __launch_bounds__(256) void k_push64_direct(P64 P, Fields64 F, Grid g, PushConsts K) {
     const int64_t n = P.cnt[CNT_SLOTS];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     int64_t oob = 0;
+    int spin_bad = 0;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
         const double wp = P.w[p];
         if (!(wp == wp)) continue;
-        oob += push_one_hbm<M>(P, p, wp, F, g, K);
+        oob += push_one_hbm<M, SPIN>(P, p, wp, F, g, K, spin_bad);
     }
     if (oob) atomicAdd((unsigned long long*)&P.cnt[CNT_OOB], (unsigned long long)oob);
+    if (SPIN && spin_bad) atomicAdd((unsigned long long*)&P.cnt[CNT_SPIN], (unsigned long long)spin_bad);
 }
 
 // ---------------------------------------------------------------------
@@ -357,7 +378,7 @@ struct AccBox {
 };
 static_assert(2 * AccBox<4, 4, 4, 4>::LDS <= 160 * 1024, "two quartic workgroups per CU");
 
-template <int M, int TX, int TY, int TZ>
+template <int M, int TX, int TY, int TZ, bool SPIN>
 __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid g, PushConsts K,
                                                       Tiling tl, const int64_t* __restrict__ tile_off,
                                                       const int64_t* __restrict__ work,
@@ -396,6 +417,7 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
     LdsSink<B::PX, B::BY, B::VOL> sink{acc, T0z + g.m - 2 - B::H, T0y + g.m - 2 - B::H,
                                       T0x + g.m - 2 - B::H, 0};
     int64_t oob = 0;
+    int spin_bad = 0;
     uint32_t strays = 0;  // iterations whose particle was outside the tile
     int it = 0;
     const int len = (int)(end - beg);
@@ -415,10 +437,12 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
         double ux = P.ux[p], uy = P.uy[p], uz = P.uz[p], xn, yn, zn;
         double Sx0[M + 1], Sy0[M + 1], Sz0[M + 1];
         int i1, j1, k1;
-        gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr, Sx0, Sy0, Sz0, i1, j1, k1);
+        double ebq[6], gam;
+        gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr, Sx0, Sy0, Sz0, i1, j1, k1, ebq, gam);
         P.x[p] = xn; P.y[p] = yn; P.z[p] = zn;
         P.ux[p] = ux; P.uy[p] = uy; P.uz[p] = uz;
         if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
+        if (SPIN) spin_bad += spin_step(P, p, ebq, ux, uy, uz, gam, K);
         if (!(fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0)) { oob++; continue; }
         // in-tile particle, move < 1 cell: its whole box lies in the LDS box
         esirkepov_s0<M>(K.q * wp, Sx0, Sy0, Sz0, i1, j1, k1, xn, yn, zn, K, g.m, sink);
@@ -428,7 +452,7 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
         for (int t = threadIdx.x; t < len; t += blockDim.x, it++)
             if (strays & (1u << it)) {
                 const int64_t p = beg + t;
-                oob += push_one_hbm<M>(P, p, P.w[p], F, g, K);
+                oob += push_one_hbm<M, SPIN>(P, p, P.w[p], F, g, K, spin_bad);
             }
     }
     __syncthreads();
@@ -447,6 +471,7 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
         hbm_add(&jp[c][g.ji(gz, gy, gx)], v);
     }
     if (oob) atomicAdd((unsigned long long*)&P.cnt[CNT_OOB], (unsigned long long)oob);
+    if (SPIN && spin_bad) atomicAdd((unsigned long long*)&P.cnt[CNT_SPIN], (unsigned long long)spin_bad);
 }
 
 // ---------------------------------------------------------------------
@@ -461,6 +486,7 @@ PushConsts push_consts64(const pic_ctx* c, const Species& S) {
     K.dt = c->p.dt; K.dx = c->p.dx; K.dy = c->p.dy; K.dz = c->p.dz;
     K.cx = K.cy = K.cz = 0.0;
     for (int a = 0; a < 3; a++) { K.N[a] = c->g.n[a]; K.per[a] = c->p.periodic[a]; }
+    K.spin = c->spin;
     return K;
 }
 
@@ -470,17 +496,18 @@ static P64 p64(pic_ctx* c, Species& S) {
     const int cur = S.cur;
     return P64{S.x[cur], S.y[cur], S.z[cur], S.ux[cur], S.uy[cur], S.uz[cur], S.w[cur],
                c->keep_old ? S.xo[cur] : nullptr, c->keep_old ? S.yo[cur] : nullptr,
-               c->keep_old ? S.zo[cur] : nullptr, S.d_cnt};
+               c->keep_old ? S.zo[cur] : nullptr, S.d_cnt,
+               (double*)S.spin_v[0][cur], (double*)S.spin_v[1][cur], (double*)S.spin_v[2][cur]};
 }
 
-template <int M>
+template <int M, bool SPIN>
 static int launch_tiled64(pic_ctx* c, Species& S, const P64& P, const Fields64& Fd,
                           const PushConsts& K, int64_t max_items) {
     using TS = TileShape<0, M>;
     using B = AccBox<M, TS::X, TS::Y, TS::Z>;
-    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push64_tiled<M, TS::X, TS::Y, TS::Z>,
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push64_tiled<M, TS::X, TS::Y, TS::Z, SPIN>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
-    k_push64_tiled<M, TS::X, TS::Y, TS::Z><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
+    k_push64_tiled<M, TS::X, TS::Y, TS::Z, SPIN><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
         P, Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
@@ -515,8 +542,13 @@ int launch_push64(pic_ctx* c, int s) {
         PhaseTimer t(c, PIC_T_PUSH);
         int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 64);
         const P64 P = p64(c, S);
-        if (c->g.m == 2) k_push64_direct<2><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K);
-        else k_push64_direct<4><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K);
+        if (S.spin) {
+            if (c->g.m == 2) k_push64_direct<2, true><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K);
+            else k_push64_direct<4, true><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K);
+        } else {
+            if (c->g.m == 2) k_push64_direct<2, false><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K);
+            else k_push64_direct<4, false><<<blocks, 256, 0, c->stream>>>(P, Fd, c->g, K);
+        }
         PIC_LAUNCH_CHECK(c);
         return PIC_OK;
     }
@@ -536,8 +568,11 @@ int launch_push64(pic_ctx* c, int s) {
     S.pushes_since_sort++;
     const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / chunk + 1;
     const P64 P = p64(c, S);  // after the sort
-    if (c->g.m == 2) return launch_tiled64<2>(c, S, P, Fd, K, max_items);
-    return launch_tiled64<4>(c, S, P, Fd, K, max_items);
+    if (S.spin)
+        return c->g.m == 2 ? launch_tiled64<2, true>(c, S, P, Fd, K, max_items)
+                           : launch_tiled64<4, true>(c, S, P, Fd, K, max_items);
+    return c->g.m == 2 ? launch_tiled64<2, false>(c, S, P, Fd, K, max_items)
+                       : launch_tiled64<4, false>(c, S, P, Fd, K, max_items);
 }
 
 }  // namespace pic

@@ -49,8 +49,12 @@ __global__ __launch_bounds__(256) void k_interior(const T* __restrict__ src, dou
 struct RecView64 {
     const double *x, *y, *z, *ux, *uy, *uz, *w, *xo, *yo, *zo;
     const int64_t* id;
+    const double *sx, *sy, *sz;  // spin, may be null
     __device__ bool live(int64_t p) const { return w[p] == w[p]; }
     __device__ void fill(int64_t p, ParticleRec& r) const {
+        r.sx = sx ? sx[p] : 0.0;
+        r.sy = sy ? sy[p] : 0.0;
+        r.sz = sz ? sz[p] : 0.0;
         r.x = x[p]; r.y = y[p]; r.z = z[p];
         r.x_old = xo ? xo[p] : r.x;
         r.y_old = yo ? yo[p] : r.y;
@@ -66,8 +70,12 @@ struct RecView32 {
     const float *fx, *fy, *fz, *ux, *uy, *uz, *w;
     const int64_t* id;
     int n[3];
+    const float *sx, *sy, *sz;  // spin, may be null
     __device__ bool live(int64_t p) const { return cell[p] >= 0; }
     __device__ void fill(int64_t p, ParticleRec& r) const {
+        r.sx = sx ? (double)sx[p] : 0.0;
+        r.sy = sy ? (double)sy[p] : 0.0;
+        r.sz = sz ? (double)sz[p] : 0.0;
         const int c = cell[p];
         r.x = (double)(c % n[0]) + (double)fx[p];
         r.y = (double)((c / n[0]) % n[1]) + (double)fy[p];
@@ -142,7 +150,6 @@ __global__ __launch_bounds__(REC_BLOCK) void k_rec_write(V v, int64_t first, int
     ParticleRec r;
     v.fill(first + t, r);
     r.pad0 = 0;
-    r.sx = r.sy = r.sz = 0.0;  // spin is not carried by the hot loop
     r.gamma = sqrt(1.0 + (r.px * r.px + r.py * r.py + r.pz * r.pz));  // particle.cpp:309
     r.charge = q;
     r.mass = mass;
@@ -269,12 +276,14 @@ int pic_particle_records(pic_ctx* c, int s, int64_t first_slot, int64_t nslots, 
     const int b = S.cur;
     if (c->fp32) {
         RecView32 v{S.cell[b], S.fx[b], S.fy[b], S.fz[b], S.vx[b], S.vy[b], S.vz[b], S.fw[b], S.id[b],
-                    {c->g.n[0], c->g.n[1], c->g.n[2]}};
+                    {c->g.n[0], c->g.n[1], c->g.n[2]},
+                    (const float*)S.spin_v[0][b], (const float*)S.spin_v[1][b], (const float*)S.spin_v[2][b]};
         return pack_records(c, v, S, first_slot, nslots, records, nrec);
     }
     const bool old = c->keep_old && S.xo[b];
     RecView64 v{S.x[b], S.y[b], S.z[b], S.ux[b], S.uy[b], S.uz[b], S.w[b],
-                old ? S.xo[b] : nullptr, old ? S.yo[b] : nullptr, old ? S.zo[b] : nullptr, S.id[b]};
+                old ? S.xo[b] : nullptr, old ? S.yo[b] : nullptr, old ? S.zo[b] : nullptr, S.id[b],
+                (const double*)S.spin_v[0][b], (const double*)S.spin_v[1][b], (const double*)S.spin_v[2][b]};
     return pack_records(c, v, S, first_slot, nslots, records, nrec);
 }
 

@@ -126,6 +126,10 @@ void free_species(pic_ctx* c, Species& S) {
         dfree(S.uz[b]); dfree(S.w[b]); dfree(S.id[b]); dfree(S.cell[b]); dfree(S.fx[b]);
         dfree(S.fy[b]); dfree(S.fz[b]); dfree(S.vx[b]); dfree(S.vy[b]); dfree(S.vz[b]);
         dfree(S.fw[b]); dfree(S.xo[b]); dfree(S.yo[b]); dfree(S.zo[b]);
+        for (int a = 0; a < 3; a++) {
+            char* q = (char*)S.spin_v[a][b];
+            dfree(q);
+        }
     }
     dfree(S.d_cnt);
     dfree(S.tile_off);
@@ -140,6 +144,32 @@ int alloc_old(pic_ctx* c, Species& S) {
         if ((r = dmalloc(c, &S.xo[b], S.cap)) || (r = dmalloc(c, &S.yo[b], S.cap)) ||
             (r = dmalloc(c, &S.zo[b], S.cap)))
             return r;
+    }
+    return PIC_OK;
+}
+
+// spin arrays of a species (both sort buffers), zeroed
+int alloc_spin(pic_ctx* c, Species& S) {
+    if (S.spin || S.cap == 0) return PIC_OK;
+    const int64_t bytes = S.cap * (c->fp32 ? 4 : 8);
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 2; b++) {
+            char* q = nullptr;
+            int r = dmalloc(c, &q, bytes);
+            if (!r) r = launch_fill(c, q, bytes);
+            S.spin_v[a][b] = q;
+            if (r) return r;
+        }
+    S.spin = 1;
+    return PIC_OK;
+}
+
+// zero the current spin buffer (new particles start with spin 0)
+int clear_spin(pic_ctx* c, Species& S) {
+    if (!S.spin) return PIC_OK;
+    for (int a = 0; a < 3; a++) {
+        int r = launch_fill(c, S.spin_v[a][S.cur], S.cap * (c->fp32 ? 4 : 8));
+        if (r) return r;
     }
     return PIC_OK;
 }
@@ -163,6 +193,39 @@ int particles_set32(pic_ctx* c, int s, int64_t n, const double* x, const double*
                     const double* w, const int64_t* id);  // k_push32.hip
 
 int particles_get32(pic_ctx* c, int s, int64_t slots, std::vector<double>* a, std::vector<int64_t>& id);
+
+namespace {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill_value(T* __restrict__ p, int64_t n, T v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// live slots of the current buffer (FP64: weight not NaN; FP32: cell >= 0)
+int live_mask(pic_ctx* c, Species& S, int64_t slots, std::vector<char>& live) {
+    live.assign(slots, 0);
+    if (!slots) return PIC_OK;
+    if (c->fp32) {
+        std::vector<int32_t> cell(slots);
+        PIC_HIP(c, hipMemcpy(cell.data(), S.cell[S.cur], slots * 4, hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < slots; p++) live[p] = cell[p] >= 0;
+    } else {
+        std::vector<double> w(slots);
+        PIC_HIP(c, hipMemcpy(w.data(), S.w[S.cur], slots * 8, hipMemcpyDeviceToHost));
+        for (int64_t p = 0; p < slots; p++) live[p] = w[p] == w[p];
+    }
+    return PIC_OK;
+}
+
+int spin_species(pic_ctx* c, int s) {
+    if (!valid_species(c, s)) return PIC_ERR_ARG;
+    if (!c->sp[s].spin)
+        return set_err(c, PIC_ERR_STATE, "species %d has no spin (pic_spin_setup with its mass first)", s);
+    return PIC_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -343,6 +406,7 @@ int pic_ctx_destroy(pic_ctx* c) {
     dfree(c->work); dfree(c->nwork); dfree(c->diag_buf);
     dfree(c->jtmp);
     dfree(c->stage);
+    if (c->spin.g) hipFree((void*)c->spin.g);
     diff_free(c);
     for (auto& e : c->tev) hipEventDestroy(e);
     if (c->ev0) hipEventDestroy(c->ev0);
@@ -469,6 +533,7 @@ int pic_species_set(pic_ctx* c, int s, double charge, double mass, int immobile,
         if (!r) r = dmalloc(c, &S.id[b], S.cap);
     }
     if (!r && c->keep_old) r = alloc_old(c, S);
+    if (!r && c->spin_on && S.m == c->spin_mass) r = alloc_spin(c, S);
     // current/charge accumulator of this species (rho_particle[s])
     if (!r && !c->rho[s]) {
         const int es = c->fp32 ? 4 : 8;
@@ -539,6 +604,8 @@ int pic_particles_set(pic_ctx* c, int s, int64_t n, const double* x, const doubl
     cnt[CNT_SLOTS] = n;
     cnt[CNT_LIVE] = n;
     PIC_HIP(c, hipMemcpyAsync(S.d_cnt, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+    int r = clear_spin(c, S);
+    if (r) return r;
     PIC_HIP(c, hipStreamSynchronize(c->stream));
     S.sorted = 0;
     S.scrambled = 0;
@@ -603,6 +670,7 @@ int pic_particles_load(pic_ctx* c, int s, const pic_loader* l) {
     cnt[CNT_SLOTS] = n;
     cnt[CNT_LIVE] = n;
     if (!r) PIC_HIP(c, hipMemcpyAsync(S.d_cnt, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+    if (!r) r = clear_spin(c, S);
     PIC_HIP(c, hipStreamSynchronize(c->stream));
     dfree(d_pre);
     S.sorted = 0;
@@ -614,6 +682,123 @@ int pic_num_particles(pic_ctx* c, int s, int64_t* n) {
     if (!valid_species(c, s) || !n) return PIC_ERR_ARG;
     PIC_HIP(c, hipSetDevice(c->device));
     *n = c->sp[s].immobile ? 0 : read_cnt(c, c->sp[s], CNT_LIVE);
+    return PIC_OK;
+}
+
+// ---------------------------------------------------------------------
+// spin precession (particle.cpp:318-368)
+// ---------------------------------------------------------------------
+
+int pic_spin_setup(pic_ctx* c, const double* g_table, int n, double e_schwinger, double electron_mass) {
+    if (!c || !g_table || n < 1) return PIC_ERR_ARG;
+    if (!(e_schwinger > 0) || !(electron_mass > 0))
+        return set_err(c, PIC_ERR_ARG, "E_schwinger and the electron mass must be > 0");
+    if (c->diff) return set_err(c, PIC_ERR_STATE, "spin setup inside a differentiated run");
+    PIC_HIP(c, hipSetDevice(c->device));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    double* g = nullptr;
+    int r = dmalloc(c, &g, n);
+    if (r) return r;
+    PIC_HIP(c, hipMemcpy(g, g_table, (size_t)n * 8, hipMemcpyHostToDevice));
+    if (c->spin.g) hipFree((void*)c->spin.g);
+    c->spin.g = g;
+    c->spin.ng = n;
+    c->spin.es = e_schwinger;
+    c->spin_on = 1;
+    c->spin_mass = electron_mass;
+    for (int s = 0; s < PIC_MAX_SPECIES; s++) {
+        Species& S = c->sp[s];
+        if (S.declared && !S.immobile && S.m == electron_mass && (r = alloc_spin(c, S))) return r;
+    }
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    return PIC_OK;
+}
+
+int pic_spin_set(pic_ctx* c, int s, int64_t n, const double* sx, const double* sy, const double* sz) {
+    int r = spin_species(c, s);
+    if (r) return r;
+    if (n < 0 || (n && (!sx || !sy || !sz))) return PIC_ERR_ARG;
+    Species& S = c->sp[s];
+    PIC_HIP(c, hipSetDevice(c->device));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t slots = read_cnt(c, S, CNT_SLOTS);
+    std::vector<char> live;
+    if ((r = live_mask(c, S, slots, live))) return r;
+    int64_t nl = 0;
+    for (char v : live) nl += v;
+    if (n != nl) return set_err(c, PIC_ERR_ARG, "%lld spins for %lld particles", (long long)n, (long long)nl);
+    const double* src[3] = {sx, sy, sz};
+    for (int a = 0; a < 3; a++) {
+        std::vector<double> d(slots, 0.0);
+        for (int64_t p = 0, q = 0; p < slots; p++)
+            if (live[p]) d[p] = src[a][q++];
+        if (!slots) continue;
+        if (c->fp32) {
+            std::vector<float> f(d.begin(), d.end());
+            PIC_HIP(c, hipMemcpy(S.spin_v[a][S.cur], f.data(), slots * 4, hipMemcpyHostToDevice));
+        } else {
+            PIC_HIP(c, hipMemcpy(S.spin_v[a][S.cur], d.data(), slots * 8, hipMemcpyHostToDevice));
+        }
+    }
+    return PIC_OK;
+}
+
+int pic_spin_get(pic_ctx* c, int s, int64_t* n, double* sx, double* sy, double* sz) {
+    int r = spin_species(c, s);
+    if (r) return r;
+    if (!n) return PIC_ERR_ARG;
+    Species& S = c->sp[s];
+    PIC_HIP(c, hipSetDevice(c->device));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    const int64_t slots = read_cnt(c, S, CNT_SLOTS);
+    std::vector<char> live;
+    if ((r = live_mask(c, S, slots, live))) return r;
+    double* dst[3] = {sx, sy, sz};
+    int64_t nl = 0;
+    for (int a = 0; a < 3; a++) {
+        std::vector<double> d(slots);
+        if (slots && c->fp32) {
+            std::vector<float> f(slots);
+            PIC_HIP(c, hipMemcpy(f.data(), S.spin_v[a][S.cur], slots * 4, hipMemcpyDeviceToHost));
+            d.assign(f.begin(), f.end());
+        } else if (slots) {
+            PIC_HIP(c, hipMemcpy(d.data(), S.spin_v[a][S.cur], slots * 8, hipMemcpyDeviceToHost));
+        }
+        nl = 0;
+        for (int64_t p = 0; p < slots; p++) {
+            if (!live[p]) continue;
+            if (nl >= *n) return set_err(c, PIC_ERR_CAPACITY, "output arrays too small");
+            if (dst[a]) dst[a][nl] = d[p];
+            nl++;
+        }
+    }
+    *n = nl;
+    return PIC_OK;
+}
+
+int pic_spin_fill(pic_ctx* c, int s, double sx, double sy, double sz) {
+    int r = spin_species(c, s);
+    if (r) return r;
+    Species& S = c->sp[s];
+    PIC_HIP(c, hipSetDevice(c->device));
+    const double v[3] = {sx, sy, sz};
+    const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 4096);
+    for (int a = 0; a < 3; a++) {
+        if (c->fp32)
+            k_fill_value<float><<<blocks, 256, 0, c->stream>>>((float*)S.spin_v[a][S.cur], S.cap, (float)v[a]);
+        else
+            k_fill_value<double><<<blocks, 256, 0, c->stream>>>((double*)S.spin_v[a][S.cur], S.cap, v[a]);
+        PIC_LAUNCH_CHECK(c);
+    }
+    return PIC_OK;
+}
+
+int pic_spin_errors(pic_ctx* c, int s, int64_t* n) {
+    int r = spin_species(c, s);
+    if (r) return r;
+    if (!n) return PIC_ERR_ARG;
+    PIC_HIP(c, hipSetDevice(c->device));
+    *n = read_cnt(c, c->sp[s], CNT_SPIN);
     return PIC_OK;
 }
 

@@ -11,6 +11,7 @@
 
 #include "../../include/pic3d_amd.h"
 #include "fastdiv.h"
+#include "spin.h"
 
 namespace pic {
 
@@ -66,6 +67,10 @@ struct Species {
     int64_t* d_cnt = nullptr;
     int64_t* tile_off = nullptr;  // ntiles + 1 tile offsets after the last sort (+1: the
                                   // end of the removed slots a keep-dead sort appends)
+    // spin (sx, sy, sz; [1] the sort's scratch), FP64 or FP32 like the
+    // particles, for a species with the precession of particle.cpp:318-368
+    int spin = 0;
+    void* spin_v[3][2] = {};
     int sorted = 0;               // tile_off valid for the current order
     int scrambled = 0;            // the in-tile order has been transposed (stored_slot)
 };
@@ -73,8 +78,9 @@ struct Species {
 // d_cnt layout.  CNT_PEND: FP32 pushes classify boundary crossings as they
 // move particles and park the counts here until pic_particle_boundary.
 // The last entry is scratch for the sort (pre-sort slot count).
+// CNT_SPIN: precession steps skipped for a chi outside the g-factor table.
 enum { CNT_SLOTS = 0, CNT_LIVE = 1, CNT_LOST = 2, CNT_WRAP = 8, CNT_LOST_STEP = 14,
-       CNT_WRAP_STEP = 20, CNT_OOB = 26, CNT_PEND = 27, CNT_N = 40 };
+       CNT_WRAP_STEP = 20, CNT_OOB = 26, CNT_PEND = 27, CNT_SPIN = 39, CNT_N = 41 };
 
 struct DiffState;  // differentiated run (k_adj.hip)
 
@@ -115,6 +121,11 @@ struct pic_ctx {
     double env_val = 0.0;
     pic::DiffState* diff = nullptr;
     int adj_push = 0;             // adjoint push kernel: 0 auto, 1 dual numbers, 2 separable HBM
+    // spin precession (pic_spin_setup): device g-factor table, E_schwinger,
+    // and the electron mass that selects the species
+    int spin_on = 0;
+    double spin_mass = 0.0;
+    pic::SpinConsts spin;
     // per-phase timing (PIC_OPT_TIMING): event pairs around each phase
     int timing = 0;
     std::vector<hipEvent_t> tev;      // event pool

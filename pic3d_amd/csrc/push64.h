@@ -3,6 +3,7 @@
 #pragma once
 
 #include "pic_internal.h"
+#include "spin.h"
 
 namespace pic {
 
@@ -13,6 +14,7 @@ struct P64 {
     const double* w;
     double *xo, *yo, *zo;  // may be null
     int64_t* cnt;
+    double *sx, *sy, *sz;  // spin (species with precession only)
 };
 
 struct Fields64 {
@@ -24,6 +26,7 @@ struct PushConsts {
     double q, qm, dt, dx, dy, dz, cx, cy, cz;  // cx = q*dx/dt (times w per particle)
     double N[3];
     int per[3];
+    SpinConsts spin;
 };
 
 // Periodic wrap at the start of the push (particle.cpp:123-167) followed by

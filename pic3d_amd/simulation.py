@@ -126,6 +126,36 @@ class Simulation:
             _d(out["y_old"]), _d(out["z_old"])), "pic_particles_get")
         return {k: v[: n.value] for k, v in out.items()}
 
+    # spin precession (particle.cpp:318-368) ----------------------------
+    def spin_setup(self, g_table, e_schwinger: float = 4.1215e5, electron_mass: float = 1.0):
+        """Precess the spin of every species of mass electron_mass, with the
+        g.txt table (constants.cpp:114-136) and E_schwinger (constants.cpp:83)."""
+        g = np.ascontiguousarray(g_table, dtype=np.float64)
+        self._chk(self._L.pic_spin_setup(self._h, _d(g), len(g), e_schwinger, electron_mass),
+                  "pic_spin_setup")
+
+    def spin_set(self, s: int, sx, sy, sz):
+        """Spins in the order particles_get returns the particles."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (sx, sy, sz)]
+        self._chk(self._L.pic_spin_set(self._h, s, len(a[0]), _d(a[0]), _d(a[1]), _d(a[2])),
+                  "pic_spin_set")
+
+    def spin_get(self, s: int):
+        cap = max(self.num_particles(s), 1)
+        a = [np.empty(cap) for _ in range(3)]
+        n = C.c_int64(cap)
+        self._chk(self._L.pic_spin_get(self._h, s, C.byref(n), _d(a[0]), _d(a[1]), _d(a[2])),
+                  "pic_spin_get")
+        return [v[: n.value] for v in a]
+
+    def spin_fill(self, s: int, sx: float, sy: float, sz: float):
+        self._chk(self._L.pic_spin_fill(self._h, s, sx, sy, sz), "pic_spin_fill")
+
+    def spin_errors(self, s: int) -> int:
+        n = C.c_int64(0)
+        self._chk(self._L.pic_spin_errors(self._h, s, C.byref(n)), "pic_spin_errors")
+        return n.value
+
     # hot path --------------------------------------------------------
     def laser_boundary(self, step: int):
         self._chk(self._L.pic_laser_boundary(self._h, step), "pic_laser_boundary")

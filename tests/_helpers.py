@@ -12,8 +12,14 @@ from pic3d_amd._abi import FIELD_NAMES, PicLoader, PicParams
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-def golden_names():
-    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+def golden_names(spin: bool | None = None):
+    """Run fixtures (gfactor.npz holds the g_factor points only); spin:
+    only (True) or no (False) spin-precession cases."""
+    names = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    names = [n for n in names if n != "gfactor"]
+    if spin is not None:
+        names = [n for n in names if n.startswith("spin_") == spin]
+    return names
 
 
 def load_golden(name: str) -> dict:
@@ -23,9 +29,14 @@ def load_golden(name: str) -> dict:
     parts = {k: z["in_" + k] for k in ("x", "y", "z", "ux", "uy", "uz", "w", "id")}
     out = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
     outp = {k[5:]: z[k] for k in z.files if k.startswith("outp_")}
-    return {"params": p, "loader": l, "parts": parts, "cmds": [str(c) for c in z["cmds"]],
-            "fields": out, "particles": outp, "charge": float(z["charge"]),
-            "mass": float(z["mass"]), "rho_background": float(z["rho_background"])}
+    g = {"params": p, "loader": l, "parts": parts, "cmds": [str(c) for c in z["cmds"]],
+         "fields": out, "particles": outp, "charge": float(z["charge"]),
+         "mass": float(z["mass"]), "rho_background": float(z["rho_background"])}
+    if "in_sx" in z.files:  # spin precession case
+        g["spin_in"] = [z["in_sx"], z["in_sy"], z["in_sz"]]
+        g["e_schwinger"] = float(z["e_schwinger"])
+        g["g_table"] = z["g_table"]
+    return g
 
 
 def replay(sim, cmds):

@@ -16,6 +16,10 @@
 //      step, then per kind int64 n and x y z x_old y_old z_old px py pz
 //      weight gamma (n doubles each) and id (n int64) in vector order, then
 //      int64 the number of output calls seen.
+// With PIC_COMPAT_SPIN=1: IN carries sx sy sz after the ids of each kind and
+// OUT sx sy sz after the ids; g_data is read (2849 doubles) from the file
+// named by PIC_COMPAT_GTABLE, as main.cpp:52-61 loads g.txt; the cfg key
+// E_schwinger (optional) sets E_schwinger.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -51,6 +55,10 @@ double D(const char* k) {
     return std::strtod(it->second.c_str(), nullptr);
 }
 int I(const char* k) { return (int)D(k); }
+double D_or(const char* k, double dflt) {
+    auto it = cfg().find(k);
+    return it == cfg().end() ? dflt : std::strtod(it->second.c_str(), nullptr);
+}
 }  // namespace
 
 // ---- the parameter set (constants.h) ----
@@ -106,7 +114,7 @@ int nc_yst, nc_yend, nc_yst_new, nc_yend_new, nc_yst_out, nc_yend_out;
 double yst, yend;
 int nc_zst, nc_zend, nc_zst_new, nc_zend_new, nc_zst_out, nc_zend_out;
 double zst, zend;
-double E_schwinger = 4.1215e5;
+double E_schwinger = D_or("E_schwinger", 4.1215e5);
 extern const double alpha0 = 55.0 / 96.0, alpha1 = 5.0 / 24.0, alpha2 = 5.0 / 4.0,
                     alpha3 = 5.0 / 6.0, beta0 = 115.0 / 192.0, beta2 = 5.0 / 8.0;
 extern const int g_data_size = 2849;
@@ -136,6 +144,13 @@ int main(int argc, char** argv) {
     }
     initialize_domain(0);
     initialize_simulation(0, 1);
+    const char* spin_env = std::getenv("PIC_COMPAT_SPIN");
+    const bool spin_io = spin_env && std::atoi(spin_env) == 1;
+    if (const char* gt = std::getenv("PIC_COMPAT_GTABLE")) {
+        FILE* gf = std::fopen(gt, "rb");
+        if (!gf || std::fread(g_data, 8, g_data_size, gf) != (size_t)g_data_size) return 5;
+        std::fclose(gf);
+    }
     std::vector<std::vector<particle>> P(2);
     int n[2] = {0, 0};
     FILE* in = std::fopen(argv[1], "rb");
@@ -154,6 +169,11 @@ int main(int argc, char** argv) {
         }
         std::vector<int64_t> id(cnt);
         if (cnt && std::fread(id.data(), 8, cnt, in) != (size_t)cnt) return 4;
+        std::vector<double> sp[3];
+        for (int f = 0; f < 3 && spin_io; f++) {
+            sp[f].resize(cnt);
+            if (cnt && std::fread(sp[f].data(), 8, cnt, in) != (size_t)cnt) return 4;
+        }
         P[s].resize(cnt);
         for (int64_t p = 0; p < cnt; p++) {
             particle& r = P[s][p];
@@ -166,6 +186,7 @@ int main(int argc, char** argv) {
             r.weight = a[6][p];
             r.charge = q;
             r.mass = m;
+            if (spin_io) { r.sx = sp[0][p]; r.sy = sp[1][p]; r.sz = sp[2][p]; }
         }
         n[s] = (int)cnt;
     }
@@ -200,6 +221,12 @@ int main(int argc, char** argv) {
             int64_t id = P[s][p].id;
             std::fwrite(&id, 8, 1, out);
         }
+        for (int f = 0; f < 3 && spin_io; f++)
+            for (int64_t p = 0; p < cnt; p++) {
+                const particle& r = P[s][p];
+                const double v[3] = {r.sx, r.sy, r.sz};
+                std::fwrite(&v[f], 8, 1, out);
+            }
     }
     int64_t outs = g_outputs;
     std::fwrite(&outs, 8, 1, out);

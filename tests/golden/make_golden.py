@@ -6,7 +6,7 @@ unmodified; see oracle/ref_harness/Makefile) on small seeded cases and saves
 inputs and outputs as compressed .npz fixtures in this directory.  Needs
 /root/reference; run once in the build container:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [spin | case names]
 
 Each fixture holds the parameter set (raw pic_params bytes), the loader
 spec, the input particles, the command list, and every output field and
@@ -76,11 +76,96 @@ def cases():
     return out
 
 
+def g_table() -> np.ndarray:
+    """The g-factor table the reference reads from g.txt: its first
+    g_data_size = 2849 values (constants.cpp:111-136; the file holds one
+    more)."""
+    return np.loadtxt(os.path.join(ref_run.REF_ROOT, "g.txt"), dtype=np.float64)[:2849]
+
+
+def unit_spins(n: int, seed: int) -> dict:
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=(3, n))
+    v /= np.linalg.norm(v, axis=0)
+    return {"sx": v[0], "sy": v[1], "sz": v[2]}
+
+
+def spin_cases():
+    """Spin precession (particle.cpp:318-368): laser cases with random unit
+    spins and E_schwinger lowered so that chi = |E + v x B|_perp gamma / E_s
+    runs through the g.txt table's bins (the shipped 4.1215e5 keeps chi
+    below 0.005, where g is the constant 2.003219)."""
+    out = []
+    wl = W.laser_slab(nx=12, ny=8, nz=8, ppc=2, slab=(0.0, 12.0), seed=31, mem=2)
+    wl.species[0].loader.u_th = 0.3
+    wl.params.laser_amplitude = 3.0
+    out.append(("spin_m2_12x8x8", wl, ["pre", "steps:1:20"], 4.0, 41))
+    wl = W.laser_slab(nx=16, ny=12, nz=12, ppc=2, slab=(4.0, 12.0), seed=32)
+    wl.species[0].loader.u_th = 0.5
+    wl.params.laser_amplitude = 5.0
+    out.append(("spin_m4_16x12x12", wl, ["pre", "steps:1:30"], 0.5, 42))
+    return out
+
+
+def g_points() -> np.ndarray:
+    """chi values over every bin of g_factor, with the bin edges.  Points
+    from 99.9 up are left out: there the reference's index passes the end
+    of its 2849-entry table (it reads the next global variable)."""
+    edges = [0.0, 0.005, 0.01, 1.0, 10.0, 20.0, 100.0]
+    xs = list(edges[:-1])
+    for a, b in zip(edges[:-1], edges[1:]):
+        xs += list(np.linspace(a, b, 41)[1:-1])
+        xs += [np.nextafter(b, a), np.nextafter(a, b)]
+    rng = np.random.default_rng(5)
+    xs += list(rng.uniform(0.0, 100.0, 200))
+    xs += [99.8999, np.nextafter(99.9, 0.0)]
+    # keep the points whose index (particle.cpp:629) stays inside the table
+    return np.array(sorted(x for x in xs if x < 20.0 or int((x - 20.0) / 0.1) + 2050 <= 2848))
+
+
+def make_spin():
+    gt = g_table()
+    xs = g_points()
+    wl0 = W.thermal(n=8, ppc=1, seed=1)
+    gv = ref_run.g_factor(wl0.params, xs)
+    np.savez_compressed(os.path.join(HERE, "gfactor.npz"), g_table=gt, x=xs, g=gv)
+    print(f"gfactor: {len(xs)} points")
+    for name, wl, cmds, es, sseed in spin_cases():
+        p = wl.params
+        L = wl.species[0].loader
+        parts = load_particles(p, L)
+        parts.update(unit_spins(len(parts["x"]), sseed))
+        r = ref_run.run(p, [(1, wl.species[0].charge, wl.species[0].mass, parts),
+                            (0, 1.0, 1836.0, None)], cmds, spin=True, e_schwinger=es)
+        d = {
+            "params": np.frombuffer(bytes(p), dtype=np.uint8),
+            "loader": np.frombuffer(bytes(L), dtype=np.uint8),
+            "charge": wl.species[0].charge, "mass": wl.species[0].mass,
+            "rho_background": wl.rho_background,
+            "cmds": np.array(cmds), "e_schwinger": es, "g_table": gt,
+        }
+        for k in ("x", "y", "z", "ux", "uy", "uz", "w", "id", "sx", "sy", "sz"):
+            d["in_" + k] = parts[k]
+        for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz", "rho0"):
+            d["out_" + k] = r[k]
+        for k, v in r["p0"].items():
+            d["outp_" + k] = v
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **d)
+        print(f"{name}: {len(parts['x'])} -> {len(r['p0']['x'])} particles, "
+              f"{os.path.getsize(path) // 1024} KiB")
+
+
 def main():
     if not ref_run.available():
         if not ref_run.build():
             raise SystemExit("reference sources not present; cannot make golden vectors")
+    only = set(sys.argv[1:])
+    if not only or "spin" in only:
+        make_spin()
     for name, wl, cmds in cases():
+        if only and name not in only:
+            continue
         p = wl.params
         L = wl.species[0].loader
         parts = load_particles(p, L)

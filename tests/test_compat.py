@@ -59,7 +59,8 @@ CALLER_PROVIDES = {
     "phase", "yfocus", "zfocus", "nc_xst", "nc_xend", "nc_xst_new", "nc_xst_out", "nc_xend_out",
     "xst", "xend", "nc_yst", "nc_yend", "nc_yst_new", "nc_yst_out", "nc_yend_out", "yst", "yend",
     "nc_zst", "nc_zend", "nc_zst_new", "nc_zst_out", "nc_zend_out", "zst", "zend", "num_kind",
-    "g_data", "alpha0", "alpha1", "alpha2", "alpha3", "beta0", "beta2",
+    "g_data", "g_data_size", "E_schwinger", "m_e", "alpha0", "alpha1", "alpha2", "alpha3",
+    "beta0", "beta2",
     # io_utils.h (the caller's io_utils.cpp)
     "output_data_hdf5", "output_particles_hdf5",
 }

@@ -23,15 +23,18 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "compat", "bin", "compat_main")
-STEP_FIXTURES = [n for n in golden_names() if n.startswith(("thermal", "laser"))]
+STEP_FIXTURES = [n for n in golden_names() if n.startswith(("thermal", "laser", "spin"))]
 PRE_FIXTURES = [n for n in golden_names() if n.startswith("preloop")]
 
 
 def run_driver(g, steps, output_interval, driver=DRIVER, cwd=None):
     p = g["params"]
+    spin = "spin_in" in g
     kv = {"nx": p.nx, "ny": p.ny, "nz": p.nz, "mem": p.mem, "px": p.periodic[0],
           "py": p.periodic[1], "pz": p.periodic[2], "steps": steps,
           "output_interval": output_interval, "laser_polarization": p.laser_polarization}
+    if spin:
+        kv["E_schwinger"] = float(g["e_schwinger"]).hex()
     for k in ("dx", "dy", "dz", "dt", "pi", "omega", "l0", "lx", "laser_amplitude",
               "laser_pulse_duration", "laser_start_time", "waist", "xfocus", "yfocus", "zfocus",
               "phase"):
@@ -52,8 +55,16 @@ def run_driver(g, steps, output_interval, driver=DRIVER, cwd=None):
                     np.ascontiguousarray(pp[k], np.float64).tofile(f)
             if n:
                 np.ascontiguousarray(pp["id"], np.int64).tofile(f)
-    r = subprocess.run([driver, fin, fout], env=dict(os.environ, PIC_COMPAT_CFG=cfg),
-                       capture_output=True, text=True, timeout=300, cwd=cwd)
+            if n and spin:
+                for v in g["spin_in"]:
+                    np.ascontiguousarray(v, np.float64).tofile(f)
+    env = dict(os.environ, PIC_COMPAT_CFG=cfg)
+    if spin:
+        gt = os.path.join(td, "g.bin")
+        np.ascontiguousarray(g["g_table"], np.float64).tofile(gt)
+        env.update(PIC_COMPAT_SPIN="1", PIC_COMPAT_GTABLE=gt)
+    r = subprocess.run([driver, fin, fout], env=env, capture_output=True, text=True, timeout=300,
+                       cwd=cwd)
     assert r.returncode == 0, r.stderr[-2000:]
     raw = open(fout, "rb").read()
     m = p.mem
@@ -70,7 +81,10 @@ def run_driver(g, steps, output_interval, driver=DRIVER, cwd=None):
     for k in ("x", "y", "z", "x_old", "y_old", "z_old", "ux", "uy", "uz", "w", "gamma"):
         pd[k] = np.frombuffer(raw, np.float64, cnt, off); off += 8 * cnt
     pd["id"] = np.frombuffer(raw, np.int64, cnt, off); off += 8 * cnt
-    cnt1 = int(np.frombuffer(raw, np.int64, 1, off)[0]); off += 8 + 8 * 12 * cnt1
+    if spin:
+        for k in ("sx", "sy", "sz"):
+            pd[k] = np.frombuffer(raw, np.float64, cnt, off); off += 8 * cnt
+    cnt1 = int(np.frombuffer(raw, np.int64, 1, off)[0]); off += 8 + 8 * (12 + 3 * spin) * cnt1
     outputs = int(np.frombuffer(raw, np.int64, 1, off)[0])
     return out, pd, outputs, r.stdout
 
@@ -92,6 +106,10 @@ def test_compat_time_loop(name):
         assert np.max(np.abs(a - b)) <= 1e-10 * max(np.max(np.abs(b)), 1.0), k
     u2 = pd["ux"] ** 2 + pd["uy"] ** 2 + pd["uz"] ** 2
     assert np.array_equal(pd["gamma"], np.sqrt(1.0 + u2))
+    if "spin_in" in g:  # the spin precession of particle.cpp:318-368, through main.cpp's sequence
+        for k in ("sx", "sy", "sz"):
+            a, b = pd[k][o], g["particles"][k][r]
+            assert np.max(np.abs(a - b)) <= 1e-10, (k, np.max(np.abs(a - b)))
 
 
 @pytest.mark.parametrize("name", PRE_FIXTURES)

@@ -21,7 +21,7 @@ pytestmark = [pytest.mark.gpu,
                                  reason="libpic3d_io is built only where the image's HDF5 is")]
 
 
-def make_run(precision, keep_old=False):
+def make_run(precision, keep_old=False, spin=False):
     wl = W.laser_slab(nx=24, ny=8, nz=8, ppc=4, slab=(0.0, 24.0), seed=41, mem=2, precision=precision)
     wl.species[0].loader.u_th = 0.3
     p = wl.params
@@ -32,7 +32,12 @@ def make_run(precision, keep_old=False):
     if keep_old:
         sim.set_option(PIC_OPT_KEEP_OLD, 1)
     sim.set_option(PIC_OPT_SORT_INTERVAL, 1000)  # keep removed slots in place
+    if spin:
+        from _helpers import GOLDEN
+        sim.spin_setup(np.load(f"{GOLDEN}/gfactor.npz")["g_table"], 2.0, 1.0)
     sim.particles_set(0, parts)
+    if spin:
+        sim.spin_fill(0, 0.6, 0.0, 0.8)
     sim.preloop()
     sim.step(1, 12)
     return sim, p, len(parts["x"])
@@ -79,6 +84,27 @@ def test_output_files(tmp_path, precision, keep_old):
     assert np.allclose(rec["gamma"], np.sqrt(1.0 + u2), rtol=1e-15, atol=0)
     assert np.all(rec["charge"] == -1.0) and np.all(rec["mass"] == 1.0)
     assert np.all(rec["sx"] == 0) and np.all(rec["rank"] == 0)
+    sim.close()
+
+
+@pytest.mark.parametrize("precision", [PIC_FP64, PIC_FP32])
+def test_output_particles_spin(tmp_path, precision):
+    """With the spin precession on, the particle file carries each live
+    particle's spin (sx, sy, sz of include/particle.h:9-29)."""
+    import _h5
+    print(f"io spin precision {precision}", flush=True)
+    sim, p, n0 = make_run(precision, spin=True)
+    ppath, written = PIO.output_particles(sim, 12, str(tmp_path / "output"), chunk_slots=97)
+    ref = sim.particles_get(0)
+    sx, sy, sz = sim.spin_get(0)
+    assert written == len(sx) == len(ref["id"])
+    with _h5.File(ppath) as f:
+        rec = f.read("/particles", PIO.PARTICLE_DTYPE)
+    assert np.array_equal(rec["id"], ref["id"].astype(np.int32))
+    for k, v in (("sx", sx), ("sy", sy), ("sz", sz)):
+        assert np.array_equal(rec[k], v), k
+    assert not np.all(rec["sx"] == 0.6)  # the spins turned
+    assert np.max(np.abs(np.sqrt(sx ** 2 + sy ** 2 + sz ** 2) - 1.0)) < 1e-5
     sim.close()
 
 
