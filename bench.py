@@ -119,8 +119,15 @@ def run_gpu(args, precision: str) -> dict:
     sim = Simulation(p, push_kernel=args.kernel, sort_interval=args.sort_interval)
     if precision == "fp32":
         sim.set_option(PIC_OPT_FP32_ACCUM, args.fp32_accum)
+    if getattr(args, "spin", False):
+        # the electron spin precession on (particle.cpp:318-368) at the
+        # reference's E_schwinger, where chi stays below the table (g is
+        # 2.003219): the table's values do not matter for the timing
+        sim.spin_setup(np.full(2849, 2.0023193), 4.1215e5, 1.0)
     from pic3d_amd.simulation import setup
     setup(sim, wl, device_loader=True)
+    if getattr(args, "spin", False):
+        sim.spin_fill(0, 0.0, 0.0, 1.0)
     n_start = sim.num_particles(0)
     sim.preloop()
     if args.warmup:
@@ -263,6 +270,9 @@ def main():
     ap.add_argument("--ramp-steps", type=int, default=5,
                     help="timed steps of the FP32 ramp-target throughput case reported beside "
                          "the headline (0: skip it)")
+    ap.add_argument("--spin-steps", type=int, default=10,
+                    help="timed steps of the FP64 headline case with the spin precession on "
+                         "(0: skip it)")
     ap.add_argument("--adjoint-steps", type=int, default=400,
                     help="steps of the differentiated run (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -310,6 +320,14 @@ def main():
         line["ramp_target"] = {k: r[k] for k in ("pushes_per_s", "ms_per_step", "push_kernel_pushes_per_s",
                                                  "yee_cell_updates_per_s", "phase_ms_per_step",
                                                  "particles", "cells", "roofline", "workload")}
+    if args.spin_steps > 0 and args.workload == "laser_slab" and "fp64" in res:
+        sa = argparse.Namespace(**vars(args))
+        sa.steps, sa.warmup, sa.spin = args.spin_steps, 2, True
+        r = run_gpu(sa, "fp64")
+        line["spin_fp64"] = {"ms_per_step": r["ms_per_step"],
+                             "push_ms": r["phase_ms_per_step"]["push"],
+                             "push_over_no_spin": r["phase_ms_per_step"]["push"] /
+                             res["fp64"]["phase_ms_per_step"]["push"]}
     if args.adjoint_steps > 0:
         line["adjoint"] = run_adjoint(args)
     if not args.no_cpu_baseline:
