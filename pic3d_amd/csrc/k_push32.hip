@@ -384,6 +384,7 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
     ACC* acc = reinterpret_cast<ACC*>(lds_d);             // 4 * AVOL accumulators
     float* fld = reinterpret_cast<float*>(acc + 4 * B::AVOL);  // 6 * FVOL field tile
     __shared__ unsigned cnt12[12];
+    __shared__ unsigned rg_hist[256 / 64][8];  // per wave: bank-spreading counters
     const int64_t wi = blockIdx.x;
     if (wi >= *nwork) return;
     const int64_t item = work[wi];
@@ -406,6 +407,8 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
     }
     for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) acc[e] = ACC(0);
     if (threadIdx.x < 12) cnt12[threadIdx.x] = 0;
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    if (lane < 8) rg_hist[wv][lane] = 0u;
     __syncthreads();
     const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
     int64_t oob = 0;
@@ -413,19 +416,50 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
     uint32_t strays = 0;  // iterations whose particle was outside the tile
     int it = 0;
     const int len = (int)(end - beg);
-    for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
-        const int64_t p = beg + t;
-        const int cell = P.cell[p];
-        if (cell < 0) continue;
-        int cx, cy, cz;
-        K.cs.split(cell, cx, cy, cz);
-        if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
-            (unsigned)(cz - T0z) >= (unsigned)TZ) {
-            strays |= 1u << it;  // moved out of the tile since the last sort
-            continue;
+    // Each wave takes 64 consecutive slots (coalesced loads), then deals its
+    // particles out again over its lanes so that lanes whose deposit boxes
+    // start in the same LDS bank are served in different groups of 16
+    // (bank_spread_lane); the results go back to the particles' own slots.
+    for (int t0 = threadIdx.x & ~63; t0 < len; t0 += blockDim.x, it++) {
+        const int t = t0 + lane;
+        const int cell = t < len ? P.cell[beg + t] : -1;
+        int cx = 0, cy = 0, cz = 0;
+        bool ok = cell >= 0;
+        if (ok) {
+            K.cs.split(cell, cx, cy, cz);
+            if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
+                (unsigned)(cz - T0z) >= (unsigned)TZ) {
+                strays |= 1u << it;  // moved out of the tile since the last sort
+                ok = false;
+            }
         }
-        float f[3] = {P.fx[p], P.fy[p], P.fz[p]};
-        float u[3] = {P.ux[p], P.uy[p], P.uz[p]};
+        float f[3] = {0.0f, 0.0f, 0.0f}, u[3] = {0.0f, 0.0f, 0.0f}, w = 0.0f, sp[3] = {0.0f, 0.0f, 0.0f};
+        int r = 32;
+        if (ok) {
+            const int64_t p = beg + t;
+            f[0] = P.fx[p]; f[1] = P.fy[p]; f[2] = P.fz[p];
+            u[0] = P.ux[p]; u[1] = P.uy[p]; u[2] = P.uz[p];
+            w = P.w[p];
+            if (SPIN) { sp[0] = P.sx[p]; sp[1] = P.sy[p]; sp[2] = P.sz[p]; }
+            // bank of the deposit base: the cell's box offset moved to the
+            // nearest node of the old position, in 4-byte words
+            const int ofs = ((cz + m - 1 - aoz) * B::AY + (cy + m - 1 - aoy)) * B::AX + (cx + m - 1 - aox) +
+                            (f[0] >= 0.5f) + (f[1] >= 0.5f) * B::AX + (f[2] >= 0.5f) * (B::AX * B::AY);
+            r = (ofs * (int)(sizeof(ACC) / 4)) & 31;
+        }
+        const int to = bank_spread_lane(r, rg_hist[wv]);
+        const int okv = lane_send(to, ok ? 1 : 0);
+        const int tt = lane_send(to, t);
+        cx = lane_send(to, cx); cy = lane_send(to, cy); cz = lane_send(to, cz);
+#pragma unroll
+        for (int a = 0; a < 3; a++) { f[a] = lane_send(to, f[a]); u[a] = lane_send(to, u[a]); }
+        w = lane_send(to, w);
+        if (SPIN) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) sp[a] = lane_send(to, sp[a]);
+        }
+        if (!okv) continue;
+        const int64_t p = beg + tt;
         // field reads relative to the LDS tile; deposits relative to the
         // accumulator box (an in-tile particle's boxes lie inside both)
         const int fcell = ((cz + m - 1 - foz) * B::FY + (cy + m - 1 - foy)) * B::FX + (cx + m - 1 - fox);
@@ -436,9 +470,7 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
         LdsSink32<B::AX, B::AY, B::AVOL, ACC> sink{acc, K.inv_ufix,
                                               ((cz + m - 1 - aoz) * B::AY + (cy + m - 1 - aoy)) * B::AX + (cx + m - 1 - aox), 0};
         int nc;
-        float sp[3];
-        if (SPIN) { sp[0] = P.sx[p]; sp[1] = P.sy[p]; sp[2] = P.sz[p]; }
-        if (!push32_core<M, SPIN>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12, sp, spin_bad)) {
+        if (!push32_core<M, SPIN>(cx, cy, cz, m, f, u, w, K, fr, sink, nc, cnt12, sp, spin_bad)) {
             oob++;
             continue;
         }

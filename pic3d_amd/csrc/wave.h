@@ -30,6 +30,45 @@ __device__ __forceinline__ void wave_runs(int64_t key, bool valid, int& head_lan
     run_len = next - lane;
 }
 
+// Bank-spreading lane assignment.  Lanes whose LDS deposit bases fall in
+// the same bank conflict when they are served in the same group of 16
+// lanes (32 banks of 4 bytes; an LDS instruction serves a wave's 64 lanes
+// 16 at a time).  Given each lane's bank residue r in [0, 32) (r = 32: the
+// lane has no particle), returns the lane that should hold this lane's
+// particle: lanes are ordered by residue and dealt round-robin to the four
+// groups of 16, so a residue held by up to four lanes never repeats inside
+// a group.  hist: this wave's 8 words of LDS (32 byte-wide counters), zero
+// on entry and left zero.
+__device__ __forceinline__ int bank_spread_lane(int r, unsigned* hist) {
+    const int lane = lane_id();
+    int rank = 0;
+    if (r < 32) {
+        const unsigned sh = 8u * (unsigned)(r & 3);
+        rank = (int)((atomicAdd(&hist[r >> 2], 1u << sh) >> sh) & 0xffu);
+    }
+    // counts per residue (lane l < 32 holds residue l's), exclusive scan
+    const unsigned cw = hist[(lane & 31) >> 2];
+    const int c = lane < 32 ? (int)((cw >> (8u * (unsigned)(lane & 3))) & 0xffu) : 0;
+    int inc = c;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+        const int y = __shfl_up(inc, d);
+        if ((lane & 31) >= d) inc += y;
+    }
+    const int nvalid = __shfl(inc, 31);
+    const int base = __shfl(inc - c, r & 31);
+    if (lane < 8) hist[lane] = 0u;
+    const uint64_t none = __ballot(r >= 32);
+    const int pos = r < 32 ? base + rank : nvalid + __popcll(none & ((1ull << lane) - 1ull));
+    return (pos & 3) * 16 + (pos >> 2);
+}
+
+// value v of this lane sent to lane 'to' (a permutation of the wave)
+__device__ __forceinline__ int lane_send(int to, int v) { return __builtin_amdgcn_ds_permute(to << 2, v); }
+__device__ __forceinline__ float lane_send(int to, float v) {
+    return __int_as_float(__builtin_amdgcn_ds_permute(to << 2, __float_as_int(v)));
+}
+
 // Contiguous slot range [lo, hi) of this block for a sort pass over n
 // slots, in steps of blockDim.x * R.  Blocks resident at one time then
 // work on distant parts of the (mostly tile-ordered) array, so their
