@@ -197,6 +197,28 @@ int pic_particles_get(pic_ctx* ctx, int s, int64_t* n, double* x, double* y, dou
                       double* x_old, double* y_old, double* z_old);
 /* Run the seeded loader on the device (no host staging). */
 int pic_particles_load(pic_ctx* ctx, int s, const pic_loader* l);
+
+/* The reference's own initial loading: initialize_particles
+ * (particle.cpp:50-111) with the shape of its density function
+ * plasma_density (constants.cpp:87-103).  Particles sit at the centres of
+ * a lattice of (int)(n * cbrt(cell_per_particles)) points per axis, in
+ * k, j, i order; a point (x, y, z) (global cell units) gets weight
+ * density / cell_per_particles when lo[a]*l0 < x_a * d_a <= hi[a]*l0 on
+ * every axis (the reference: 5 and 15) and is skipped otherwise; at most
+ * n_x n_y n_z cell_per_particles particles.  Every particle starts with
+ * momentum u and (species with spin) spin s; ids are id_offset + index.
+ * Runs on the device; FP64 positions are the reference's bit for bit. */
+typedef struct pic_lattice {
+    int32_t cell_per_particles;
+    int32_t reserved;
+    double density;
+    double lo[3], hi[3];   /* density box per axis, in units of l0          */
+    double u[3];           /* initial_px, initial_py, initial_pz            */
+    double spin[3];        /* initial_sx, initial_sy, initial_sz            */
+    int64_t id_offset;     /* my_rank * num_particles in the reference      */
+} pic_lattice;
+int64_t pic_lattice_count(const pic_params* p, const pic_lattice* l);
+int pic_particles_lattice(pic_ctx* ctx, int s, const pic_lattice* l);
 int pic_num_particles(pic_ctx* ctx, int s, int64_t* n);
 
 /* ---- spin precession (particle.cpp:318-368) ---- */

@@ -626,6 +626,47 @@ static void deposit_one(const orc_ctx* c, double q, double wgt, double xo, doubl
 /* Returns 0 if pushed, 1 if the particle was outside the domain (counted,
  * left untouched: the reference would read outside its arrays). */
 /* ------------------------------------------------------------------ */
+/* initialize_particles (particle.cpp:50-111) with the box density of  */
+/* plasma_density (constants.cpp:87-103), on one domain (xst = 0,      */
+/* xend = n, simulation.cpp:31-34).  Writes at most cap particles;     */
+/* returns the number written.                                         */
+/* ------------------------------------------------------------------ */
+static double box_density(const pic_params* p, const pic_lattice* L, double x, double y, double z) {
+    const double xp = x * p->dx, yp = y * p->dy, zp = z * p->dz;
+    if (xp > L->lo[0] * p->l0 && xp <= L->hi[0] * p->l0 && yp > L->lo[1] * p->l0 &&
+        yp <= L->hi[1] * p->l0 && zp > L->lo[2] * p->l0 && zp <= L->hi[2] * p->l0)
+        return L->density;
+    return 0.0;
+}
+
+int64_t orc_lattice(const pic_params* p, const pic_lattice* L, int64_t cap, double* x, double* y,
+                    double* z, double* ux, double* uy, double* uz, double* w, int64_t* id) {
+    const int cpp = L->cell_per_particles;
+    const int64_t num = (int64_t)p->nx * p->ny * p->nz * cpp;
+    const int num_x = p->nx * cbrt(cpp), num_y = p->ny * cbrt(cpp), num_z = p->nz * cbrt(cpp);
+    const double norm_x = ((double)p->nx - 0.0) / (num_x);
+    const double norm_y = ((double)p->ny - 0.0) / (num_y);
+    const double norm_z = ((double)p->nz - 0.0) / (num_z);
+    int64_t q = 0;
+    for (int k = 0; k < num_z; k++)
+        for (int j = 0; j < num_y; j++)
+            for (int i = 0; i < num_x; i++) {
+                if (q >= num || q >= cap) break;
+                const double px = 0.0 + (i)*norm_x + norm_x / 2.0;
+                const double py = 0.0 + (j)*norm_y + norm_y / 2.0;
+                const double pz = 0.0 + (k)*norm_z + norm_z / 2.0;
+                const double wt = box_density(p, L, px, py, pz) / cpp;
+                if (wt <= 0.0) continue;
+                x[q] = px; y[q] = py; z[q] = pz;
+                ux[q] = L->u[0]; uy[q] = L->u[1]; uz[q] = L->u[2];
+                w[q] = wt;
+                id[q] = L->id_offset + q;
+                q++;
+            }
+    return q;
+}
+
+/* ------------------------------------------------------------------ */
 /* spin precession, particle.cpp:318-368, with g_factor (603-637)      */
 /* ------------------------------------------------------------------ */
 

@@ -58,6 +58,10 @@ void orc_diag(const orc_ctx* c, pic_diag_out* out);
 int64_t orc_out_of_bounds(const orc_ctx* c);
 
 /* Individual shape functions (particle.cpp:8-48), for closed-form tests. */
+/* initialize_particles (particle.cpp:50-111) with plasma_density's box */
+int64_t orc_lattice(const pic_params* p, const pic_lattice* L, int64_t cap, double* x, double* y,
+                    double* z, double* ux, double* uy, double* uz, double* w, int64_t* id);
+
 /* spin precession, particle.cpp:318-368 (species of mass m_e) */
 int orc_spin_setup(orc_ctx* c, const double* g, int n, double e_schwinger, double m_e);
 void orc_spin_set(orc_ctx* c, int s, const double* sx, const double* sy, const double* sz);

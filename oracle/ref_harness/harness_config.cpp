@@ -78,8 +78,8 @@ extern const int local_zcells = I("nz");
 extern const int global_nx = local_xcells;
 extern const int global_ny = local_ycells;
 extern const int global_nz = local_zcells;
-extern const int cell_per_particles = 1;
-extern const double density = 1.0;
+extern const int cell_per_particles = (int)D_or("cell_per_particles", 1);
+extern const double density = D_or("density", 1.0);
 extern const double dx = D("dx");
 extern const double dy = D("dy");
 extern const double dz = D("dz");
@@ -123,7 +123,16 @@ double E_schwinger = D_or("E_schwinger", 4.1215e5);
 extern const double alpha0 = 55.0 / 96.0, alpha1 = 5.0 / 24.0, alpha2 = 5.0 / 4.0,
                     alpha3 = 5.0 / 6.0, beta0 = 115.0 / 192.0, beta2 = 5.0 / 8.0;
 
-double plasma_density(double, double, double) { return 0.0; }
+// the shipped density (constants.cpp:87-103): 'density' inside the box
+// (lo, hi]^3 in units of l0 (5 and 15 there), 0 outside; box from the cfg
+// keys pd_lo / pd_hi
+double plasma_density(double x, double y, double z) {
+    static const double lo = D_or("pd_lo", 5.0), hi = D_or("pd_hi", 15.0);
+    const double xp = x * dx, yp = y * dy, zp = z * dz;
+    if (xp > lo * l0 && xp <= hi * l0 && yp > lo * l0 && yp <= hi * l0 && zp > lo * l0 && zp <= hi * l0)
+        return density;
+    return 0.0;
+}
 double plasma_density1(double, double, double) { return 0.0; }
 
 extern const int g_data_size = 2849;

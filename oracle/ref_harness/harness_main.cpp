@@ -19,6 +19,8 @@
 //        xcur:C              exchange_boundary_current_fixed(jx|jy|jz|rho0|rho1 [C])
 //        dep:K               currents_particle(p, K) for every particle of kind K
 //        gfac:PATH           g_factor(x) for the doubles x in PATH, into PATH.out
+//        init:PX:PY:PZ:SX:SY:SZ  initialize_particles(kind 0, plasma_density, ...)
+//                            in place of the kind-0 input particles
 //
 // With PIC_REF_SPIN=1 in the environment the input carries sx sy sz (n
 // doubles each) after the ids of each species, and the output sx sy sz
@@ -228,6 +230,11 @@ int main(int argc, char** argv) {
             for (int p = 0; p < n[s]; p++) currents_particle(P[s][p], s);
         } else if (cmd.rfind("gfac:", 0) == 0) {
             // done above
+        } else if (cmd.rfind("init:", 0) == 0) {
+            double v[6] = {0, 0, 0, 0, 0, 0};
+            std::sscanf(cmd.c_str(), "init:%lf:%lf:%lf:%lf:%lf:%lf", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]);
+            const double q = P[0].empty() ? -1.0 : P[0][0].charge, m = P[0].empty() ? 1.0 : P[0][0].mass;
+            initialize_particles(P[0], n[0], plasma_density, q, m, v[0], v[1], v[2], v[3], v[4], v[5], g_rank);
         } else {
             std::fprintf(stderr, "harness: unknown command %s\n", cmd.c_str());
             return 5;

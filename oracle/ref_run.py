@@ -47,7 +47,7 @@ def _cfg_text(p: PicParams) -> str:
 
 
 def run(p: PicParams, species: list, cmds: list, workdir: str | None = None, spin: bool = False,
-        e_schwinger: float | None = None) -> dict:
+        e_schwinger: float | None = None, extra_cfg: dict | None = None) -> dict:
     """species: two entries of (mobile, q, m, parts-dict or None).
     Returns fields (ex..bz, jx, jy, jz, rho0, rho1) and particles per species.
     spin: the parts carry sx, sy, sz, and the output particles sx, sy, sz
@@ -61,6 +61,8 @@ def run(p: PicParams, species: list, cmds: list, workdir: str | None = None, spi
         f.write(_cfg_text(p))
         if e_schwinger is not None:
             f.write(f"E_schwinger {float(e_schwinger).hex()}\n")
+        for k, v in (extra_cfg or {}).items():  # cell_per_particles, density, pd_lo, pd_hi
+            f.write(f"{k} {float(v).hex()}\n")
     with open(fin, "wb") as f:
         for mobile, q, m, parts in species:
             n = 0 if parts is None else len(parts["x"])

@@ -81,6 +81,16 @@ class PicLoader(C.Structure):
     ]
 
 
+class PicLattice(C.Structure):
+    """pic_lattice: initialize_particles (particle.cpp:50-111) with the box
+    density of plasma_density (constants.cpp:87-103)."""
+    _fields_ = [
+        ("cell_per_particles", C.c_int32), ("reserved", C.c_int32),
+        ("density", C.c_double), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+        ("u", C.c_double * 3), ("spin", C.c_double * 3), ("id_offset", C.c_int64),
+    ]
+
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpic3d_amd.so")
 _LIB = None
@@ -106,6 +116,8 @@ SIGNATURES = {
     "pic_particles_set": (C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7 + [_ip]),
     "pic_particles_get": (C.c_int, [C.c_void_p, C.c_int, _ip] + [_dp] * 7 + [_ip] + [_dp] * 3),
     "pic_particles_load": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PicLoader)]),
+    "pic_lattice_count": (C.c_int64, [C.POINTER(PicParams), C.POINTER(PicLattice)]),
+    "pic_particles_lattice": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(PicLattice)]),
     "pic_num_particles": (C.c_int, [C.c_void_p, C.c_int, _ip]),
     "pic_spin_setup": (C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double]),
     "pic_spin_set": (C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 3),

@@ -753,15 +753,6 @@ struct LoaderArgs32 {
     int n[3];
 };
 
-__host__ __device__ inline void to_cell_offset(double v, int n, int& c, float& f) {
-    c = (int)floor(v);
-    double r = v - (double)c;
-    f = (float)r;
-    if (f >= 1.0f) { f = 0.0f; c += 1; }
-    if (c >= n) c = n - 1, f = 1.0f - 5.96e-8f;  // only x == n exactly
-    if (c < 0) c = 0, f = 0.0f;
-}
-
 __global__ __launch_bounds__(256) void k_loader32(Soa32 A, const int64_t* __restrict__ prefix,
                                                    int64_t n, LoaderArgs32 L) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;

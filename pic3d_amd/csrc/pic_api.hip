@@ -678,6 +678,31 @@ int pic_particles_load(pic_ctx* c, int s, const pic_loader* l) {
     return r;
 }
 
+int pic_particles_lattice(pic_ctx* c, int s, const pic_lattice* l) {
+    if (!valid_species(c, s) || !l || l->cell_per_particles < 1) return PIC_ERR_ARG;
+    Species& S = c->sp[s];
+    if (S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is immobile", s);
+    PIC_HIP(c, hipSetDevice(c->device));
+    int r = clear_spin(c, S);
+    int64_t n = 0;
+    if (!r) r = launch_lattice(c, s, l, &n);
+    if (r) return r;
+    const int b = S.cur;
+    if (!c->fp32 && c->keep_old && S.xo[b] && n) {
+        PIC_HIP(c, hipMemcpyAsync(S.xo[b], S.x[b], n * 8, hipMemcpyDeviceToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.yo[b], S.y[b], n * 8, hipMemcpyDeviceToDevice, c->stream));
+        PIC_HIP(c, hipMemcpyAsync(S.zo[b], S.z[b], n * 8, hipMemcpyDeviceToDevice, c->stream));
+    }
+    int64_t cnt[CNT_N] = {0};
+    cnt[CNT_SLOTS] = n;
+    cnt[CNT_LIVE] = n;
+    PIC_HIP(c, hipMemcpyAsync(S.d_cnt, cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+    PIC_HIP(c, hipStreamSynchronize(c->stream));
+    S.sorted = 0;
+    S.scrambled = 0;
+    return PIC_OK;
+}
+
 int pic_num_particles(pic_ctx* c, int s, int64_t* n) {
     if (!valid_species(c, s) || !n) return PIC_ERR_ARG;
     PIC_HIP(c, hipSetDevice(c->device));

@@ -181,6 +181,7 @@ int alloc_sort_buffers(pic_ctx* c);
 int build_work_on(pic_ctx* c, const int64_t* tile_off, int chunk, int64_t* work, int64_t* nwork,
                   hipStream_t stream);
 int launch_loader(pic_ctx* c, int s, const pic_loader* l, int64_t n, const int64_t* d_prefix);
+int launch_lattice(pic_ctx* c, int s, const pic_lattice* l, int64_t* n_out);  // k_lattice.hip
 int launch_diag(pic_ctx* c, pic_diag_out* out);
 int launch_fill(pic_ctx* c, void* p, int64_t bytes);
 
@@ -213,6 +214,17 @@ __host__ __device__ inline int stored_slot(int r, int len) {
     if (r >= l64) return r;
     const int rows = l64 >> 6;
     return (r % rows) * 64 + r / rows;
+}
+
+// FP32 layout of a global position v on an axis of n cells: the cell and
+// the in-cell offset in [0, 1)
+__host__ __device__ inline void to_cell_offset(double v, int n, int& c, float& f) {
+    c = (int)floor(v);
+    double r = v - (double)c;
+    f = (float)r;
+    if (f >= 1.0f) { f = 0.0f; c += 1; }
+    if (c >= n) c = n - 1, f = 1.0f - 5.96e-8f;  // only x == n exactly
+    if (c < 0) c = 0, f = 0.0f;
 }
 
 // Slot of the particle at sorted position q of tile 'tile' (the sort's

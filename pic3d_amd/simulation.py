@@ -106,6 +106,20 @@ class Simulation:
                                             _d(a["ux"]), _d(a["uy"]), _d(a["uz"]), _d(a["w"]),
                                             _i(ids)), "pic_particles_set")
 
+    def particles_lattice(self, s: int, cell_per_particles: int, density: float,
+                          lo=(5.0, 5.0, 5.0), hi=(15.0, 15.0, 15.0), u=(0.0, 0.0, 0.0),
+                          spin=(0.0, 0.0, 0.0), id_offset: int = 0) -> int:
+        """initialize_particles (particle.cpp:50-111) with plasma_density's box
+        (constants.cpp:87-103; lo/hi in units of l0) on the device; returns the
+        number of particles."""
+        from ._abi import PicLattice
+        L = PicLattice()
+        L.cell_per_particles, L.density, L.id_offset = cell_per_particles, density, id_offset
+        for a in range(3):
+            L.lo[a], L.hi[a], L.u[a], L.spin[a] = lo[a], hi[a], u[a], spin[a]
+        self._chk(self._L.pic_particles_lattice(self._h, s, C.byref(L)), "pic_particles_lattice")
+        return self.num_particles(s)
+
     def particles_load(self, s: int, loader: PicLoader):
         self._chk(self._L.pic_particles_load(self._h, s, C.byref(loader)), "pic_particles_load")
 

@@ -16,7 +16,7 @@ def golden_names(spin: bool | None = None):
     """Run fixtures (gfactor.npz holds the g_factor points only); spin:
     only (True) or no (False) spin-precession cases."""
     names = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
-    names = [n for n in names if n != "gfactor"]
+    names = [n for n in names if n != "gfactor" and not n.startswith("lattice_")]
     if spin is not None:
         names = [n for n in names if n.startswith("spin_") == spin]
     return names
@@ -93,3 +93,17 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
 
 def fields_of(sim, comps=range(10)):
     return {FIELD_NAMES[c]: sim.field_get(c) for c in comps}
+
+
+def lattice_names():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "lattice_*.npz")))
+
+
+def load_lattice(name: str) -> dict:
+    """A reference initialize_particles run: params, the pic_lattice spec and
+    the particles in the reference's vector order (with spin and gamma)."""
+    from pic3d_amd._abi import PicLattice
+    z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
+    return {"params": PicParams.from_buffer_copy(z["params"].tobytes()),
+            "lattice": PicLattice.from_buffer_copy(z["lattice"].tobytes()),
+            "particles": {k[5:]: z[k] for k in z.files if k.startswith("outp_")}}

@@ -9,6 +9,11 @@
 // run_main_time_loop, free_memory).
 //
 //   compat_main IN OUT
+//   compat_main --init OUT PX PY PZ SX SY SZ
+//      initialize_particles (kind 0, electrons) with the box density of
+//      constants.cpp:87-103 (cfg keys density, cell_per_particles, pd_lo,
+//      pd_hi); writes int64 n, then x y z px py pz weight sx sy sz gamma
+//      (n doubles each) and id (n int64).  Host only: no GPU call.
 // IN : per kind int32 mobile, double q, double m, int64 n, then x y z ux uy
 //      uz w (n doubles each) and id (n int64); kinds with mobile = 0 are
 //      given no particles.
@@ -85,8 +90,8 @@ extern const int local_zcells = I("nz");
 extern const int global_nx = local_xcells;
 extern const int global_ny = local_ycells;
 extern const int global_nz = local_zcells;
-extern const int cell_per_particles = 1;
-extern const double density = 1.0;
+extern const int cell_per_particles = (int)D_or("cell_per_particles", 1);
+extern const double density = D_or("density", 1.0);
 extern const double dx = D("dx");
 extern const double dy = D("dy");
 extern const double dz = D("dz");
@@ -137,7 +142,46 @@ static void write3(FILE* f, double*** a, int lo, int hiz, int hiy, int hix) {
         for (int j = lo; j <= hiy; j++) std::fwrite(&a[k][j][lo], sizeof(double), hix - lo + 1, f);
 }
 
+// the shipped density shape (constants.cpp:87-103) with the box from the cfg
+static double box_density(double x, double y, double z) {
+    static const double lo = D_or("pd_lo", 5.0), hi = D_or("pd_hi", 15.0);
+    const double xp = x * dx, yp = y * dy, zp = z * dz;
+    if (xp > lo * l0 && xp <= hi * l0 && yp > lo * l0 && yp <= hi * l0 && zp > lo * l0 && zp <= hi * l0)
+        return density;
+    return 0.0;
+}
+
+static int init_only(int argc, char** argv) {
+    if (argc < 9) return 2;
+    double v[6];
+    for (int i = 0; i < 6; i++) v[i] = std::strtod(argv[3 + i], nullptr);
+    // one domain: the bounds initialize_domain sets (simulation.cpp:31-34)
+    xst = 0.0; xend = local_xcells;
+    yst = 0.0; yend = local_ycells;
+    zst = 0.0; zend = local_zcells;
+    std::vector<particle> P;
+    int n = 0;
+    initialize_particles(P, n, box_density, q_e, m_e, v[0], v[1], v[2], v[3], v[4], v[5], 0);
+    FILE* out = std::fopen(argv[2], "wb");
+    if (!out) return 6;
+    const int64_t cnt = n;
+    std::fwrite(&cnt, 8, 1, out);
+    for (int f = 0; f < 11; f++)
+        for (int i = 0; i < n; i++) {
+            const particle& r = P[i];
+            const double a[11] = {r.x, r.y, r.z, r.px, r.py, r.pz, r.weight, r.sx, r.sy, r.sz, r.gamma};
+            std::fwrite(&a[f], 8, 1, out);
+        }
+    for (int i = 0; i < n; i++) {
+        const int64_t id = P[i].id;
+        std::fwrite(&id, 8, 1, out);
+    }
+    std::fclose(out);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 2 && std::strcmp(argv[1], "--init") == 0) return init_only(argc, argv);
     if (argc < 3) {
         std::fprintf(stderr, "usage: %s IN OUT\n", argv[0]);
         return 2;

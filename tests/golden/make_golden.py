@@ -156,6 +156,41 @@ def make_spin():
               f"{os.path.getsize(path) // 1024} KiB")
 
 
+def lattice_cases():
+    """initialize_particles (particle.cpp:50-111) with plasma_density's box
+    (constants.cpp:87-103): (name, grid n, dx in l0, cell_per_particles,
+    density, box lo, hi in l0, initial u, initial spin)."""
+    return [
+        # the reference's own call (main.cpp:71): 0.5 n_c in (5, 15] l0, at rest
+        ("lattice_shipped_32", 32, 0.5, 1, 0.5, 5.0, 15.0, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)),
+        ("lattice_cpp8_12", 12, None, 8, 0.5, 0.1, 0.9, (0.1, -0.2, 0.05), (0.0, 0.0, 1.0)),
+        ("lattice_cpp2_10", 10, 0.05, 2, 1.25, 0.0, 0.37, (0.0, 0.3, 0.0), (0.6, 0.0, 0.8)),
+        ("lattice_cpp27_6", 6, 0.1, 27, 2.0, 0.05, 0.5, (0.0, 0.0, -0.4), (0.0, 1.0, 0.0)),
+    ]
+
+
+def make_lattice():
+    from pic3d_amd._abi import PicLattice
+    for name, n, dxl, cpp, dens, lo, hi, u, sp in lattice_cases():
+        p = W.thermal(n=n, ppc=1, seed=1).params
+        if dxl is not None:
+            p.dx = p.dy = p.dz = dxl * p.l0
+        L = PicLattice()
+        L.cell_per_particles, L.density = cpp, dens
+        for a in range(3):
+            L.lo[a], L.hi[a], L.u[a], L.spin[a] = lo, hi, u[a], sp[a]
+        cmd = "init:" + ":".join(repr(float(v)) for v in (*u, *sp))
+        r = ref_run.run(p, [(1, -1.0, 1.0, None), (0, 1.0, 1836.0, None)], [cmd], spin=True,
+                        extra_cfg={"cell_per_particles": cpp, "density": dens, "pd_lo": lo, "pd_hi": hi})
+        d = {"params": np.frombuffer(bytes(p), dtype=np.uint8),
+             "lattice": np.frombuffer(bytes(L), dtype=np.uint8)}
+        for k, v in r["p0"].items():
+            d["outp_" + k] = v
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **d)
+        print(f"{name}: {len(r['p0']['x'])} particles, {os.path.getsize(path) // 1024} KiB")
+
+
 def main():
     if not ref_run.available():
         if not ref_run.build():
@@ -163,6 +198,8 @@ def main():
     only = set(sys.argv[1:])
     if not only or "spin" in only:
         make_spin()
+    if not only or "lattice" in only:
+        make_lattice()
     for name, wl, cmds in cases():
         if only and name not in only:
             continue
