@@ -262,7 +262,11 @@ def main():
     ap.add_argument("--workload", default="laser_slab",
                     choices=["laser_slab", "ramp_target", "thermal"])
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 direct, 2 tiled")
-    ap.add_argument("--sort-interval", type=int, default=1)
+    ap.add_argument("--sort-interval", type=int, default=None,
+                    help="re-sort the tiled species every N pushes (default: the library's, 1)")
+    ap.add_argument("--ramp-sort-interval", type=int, default=2,
+                    help="sort interval of the ramp-target line (every second push: 93 vs "
+                         "103 ms per step measured, the sort being a fifth of the step there)")
     ap.add_argument("--fp32-accum", type=int, default=1,
                     help="FP32 deposit accumulators: 1 int32 fixed point (default), 0 FP64 in LDS")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -316,10 +320,12 @@ def main():
     if args.ramp_steps > 0 and args.workload == "laser_slab":
         ra = argparse.Namespace(**vars(args))
         ra.workload, ra.steps, ra.warmup = "ramp_target", args.ramp_steps, 2
+        ra.sort_interval = args.ramp_sort_interval
         r = run_gpu(ra, "fp32")
         line["ramp_target"] = {k: r[k] for k in ("pushes_per_s", "ms_per_step", "push_kernel_pushes_per_s",
                                                  "yee_cell_updates_per_s", "phase_ms_per_step",
                                                  "particles", "cells", "roofline", "workload")}
+        line["ramp_target"]["sort_interval"] = ra.sort_interval
     if args.spin_steps > 0 and args.workload == "laser_slab" and "fp64" in res:
         sa = argparse.Namespace(**vars(args))
         sa.steps, sa.warmup, sa.spin = args.spin_steps, 2, True

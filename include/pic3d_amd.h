@@ -152,7 +152,7 @@ int pic_sync(pic_ctx* ctx);
 enum pic_option {
     PIC_OPT_PUSH_KERNEL = 1,   /* 0 auto, 1 direct (global atomics), 2 tiled (LDS) */
     PIC_OPT_KEEP_OLD = 2,      /* 1: keep x_old/y_old/z_old per particle (FP64)    */
-    PIC_OPT_SORT_INTERVAL = 3, /* re-sort particles by tile every N pushes (>= 1)  */
+    PIC_OPT_SORT_INTERVAL = 3, /* re-sort particles by tile every N pushes (>= 1; default 1) */
     PIC_OPT_TIMING = 4,        /* 1: time every phase with HIP events              */
     PIC_OPT_FP32_ACCUM = 5,    /* FP32 only: 1 (default) int32 fixed-point deposit
                                   accumulators in LDS (each add rounded to 2^-19 of

@@ -50,7 +50,7 @@ def field_shape(p: PicParams, comp: int):
 
 class Simulation:
     def __init__(self, params: PicParams, device: int = 0, push_kernel: int = 0,
-                 keep_old: bool = False, sort_interval: int = 1):
+                 keep_old: bool = False, sort_interval: int | None = None):
         self.p = params.copy()
         self._L = lib()
         h = C.c_void_p()
@@ -62,7 +62,8 @@ class Simulation:
         self.set_option(PIC_OPT_PUSH_KERNEL, push_kernel)
         if keep_old:
             self.set_option(PIC_OPT_KEEP_OLD, 1)
-        self.set_option(PIC_OPT_SORT_INTERVAL, sort_interval)
+        if sort_interval is not None:  # None: the library default (1)
+            self.set_option(PIC_OPT_SORT_INTERVAL, sort_interval)
 
     # ---------------------------------------------------------------
     def _chk(self, rc: int, what: str):
