@@ -127,6 +127,7 @@ template <int M, typename T> __device__ __forceinline__ int shape_at_t(T x, T* W
 // q dt / 2m scaling)
 template <typename T>
 __device__ __forceinline__ void boris_t(const T* u, const T* E, const T* B, double qm, T* out) {
+    using ::sqrt;
     const T e0 = E[0] * qm, e1 = E[1] * qm, e2 = E[2] * qm;
     const T b0 = B[0] * qm, b1 = B[1] * qm, b2 = B[2] * qm;
     const T px1 = u[0] + e0, py1 = u[1] + e1, pz1 = u[2] + e2;
@@ -140,6 +141,53 @@ __device__ __forceinline__ void boris_t(const T* u, const T* E, const T* B, doub
     out[0] = px1 + (py2 * sz - pz2 * sy) + e0;
     out[1] = py1 + (pz2 * sx - px2 * sz) + e1;
     out[2] = pz1 + (px2 * sy - py2 * sx) + e2;
+}
+
+// Transpose of boris_t's Jacobian applied to lo (reverse mode): given the
+// adjoint lo of the output momentum, the adjoints of u (lu), E and B (lF,
+// E then B).  The forward intermediates are recomputed here, so the
+// caller keeps nothing of the forward live across its other work.
+template <typename T>
+__device__ __forceinline__ void boris_vjp(const T* u, const T* E, const T* B, T qm, const T* lo, T* lu, T* lF) {
+    using ::sqrt;
+    const T e[3] = {E[0] * qm, E[1] * qm, E[2] * qm};
+    const T b[3] = {B[0] * qm, B[1] * qm, B[2] * qm};
+    const T p1[3] = {u[0] + e[0], u[1] + e[1], u[2] + e[2]};
+    const T g1 = sqrt(T(1) + (p1[0] * p1[0] + p1[1] * p1[1] + p1[2] * p1[2]));
+    const T t[3] = {b[0] / g1, b[1] / g1, b[2] / g1};
+    const T tt = T(2) / (T(1) + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    const T sv[3] = {t[0] * tt, t[1] * tt, t[2] * tt};
+    const T p2[3] = {p1[0] + (p1[1] * t[2] - p1[2] * t[1]), p1[1] + (p1[2] * t[0] - p1[0] * t[2]),
+                     p1[2] + (p1[0] * t[1] - p1[1] * t[0])};
+    auto cross = [](const T* a, const T* c, T* r) {
+        r[0] = a[1] * c[2] - a[2] * c[1];
+        r[1] = a[2] * c[0] - a[0] * c[2];
+        r[2] = a[0] * c[1] - a[1] * c[0];
+    };
+    // out = p1 + p2 x s + e
+    T le[3] = {lo[0], lo[1], lo[2]}, lp1[3] = {lo[0], lo[1], lo[2]}, lp2[3], ls[3];
+    cross(sv, lo, lp2);  // d/dp2 of (p2 x s).lo = s x lo
+    cross(lo, p2, ls);   // d/ds = lo x p2
+    // p2 = p1 + p1 x t
+    T tl[3], lt[3];
+    cross(t, lp2, tl);
+    cross(lp2, p1, lt);
+#pragma unroll
+    for (int a = 0; a < 3; a++) lp1[a] += lp2[a] + tl[a];
+    // s = t tt, tt = 2 / (1 + t.t): d tt / dt = -tt^2 t
+    const T ltt = t[0] * ls[0] + t[1] * ls[1] + t[2] * ls[2];
+#pragma unroll
+    for (int a = 0; a < 3; a++) lt[a] += tt * ls[a] - tt * tt * ltt * t[a];
+    // t = b / g1, g1 = sqrt(1 + p1.p1)
+    const T lg1 = -(t[0] * lt[0] + t[1] * lt[1] + t[2] * lt[2]) / g1;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lp1[a] += lg1 * p1[a] / g1;
+        le[a] += lp1[a];
+        lu[a] = lp1[a];
+        lF[a] = qm * le[a];
+        lF[3 + a] = qm * (lt[a] / g1);
+    }
 }
 
 // Esirkepov deposit of particle.cpp:372-500 (J only: rho feeds nothing)
@@ -554,14 +602,9 @@ __device__ __forceinline__ T adj_particle(const int* base, const T* frac, const 
         Fv[c] = v;
         Fg[c][0] = gx; Fg[c][1] = gy; Fg[c][2] = gz;
     }
-    // Boris with d/d(u0, E, B)
-    using D9 = Dual<9, T>;
-    D9 u9[3] = {dvar<9, T>(u0v[0], 0), dvar<9, T>(u0v[1], 1), dvar<9, T>(u0v[2], 2)};
-    D9 E9[3] = {dvar<9, T>(Fv[0], 3), dvar<9, T>(Fv[1], 4), dvar<9, T>(Fv[2], 5)};
-    D9 B9[3] = {dvar<9, T>(Fv[3], 6), dvar<9, T>(Fv[4], 7), dvar<9, T>(Fv[5], 8)};
-    D9 u1[3];
-    boris_t<D9>(u9, E9, B9, (double)K.qm, u1);
-    const T v1[3] = {u1[0].v, u1[1].v, u1[2].v};
+    // Boris (its transpose below, by boris_vjp)
+    T v1[3];
+    boris_t<T>(u0v, Fv, Fv + 3, (double)K.qm, v1);
     const T gam = ::sqrt(T(1) + (v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]));
     T f1[3];
 #pragma unroll
@@ -593,12 +636,7 @@ __device__ __forceinline__ T adj_particle(const int* base, const T* frac, const 
     for (int a = 0; a < 3; a++) lx0[a] += lx[a];
     // Boris transpose
     T lF[6];
-#pragma unroll
-    for (int b = 0; b < 9; b++) {
-        const T s = lu1[0] * u1[0].d[b] + lu1[1] * u1[1].d[b] + lu1[2] * u1[2].d[b];
-        if (b < 3) lu[b] = s;
-        else lF[b - 3] = s;
-    }
+    boris_vjp<T>(u0v, Fv, Fv + 3, (T)K.qm, lu1, lu, lF);
     // gather transpose
 #pragma unroll
     for (int c = 0; c < 6; c++) {
@@ -657,6 +695,9 @@ struct AdjPush32 {
 // adjoint through adj_particle, written back to the slot.
 struct AdjView64 {
     using T = double;
+    // waves per SIMD the tiled adjoint asks of the register allocator: the
+    // FP64 kernel would spill heavily below its 256+ registers
+    static constexpr int MIN_WAVES = 1;
     AdjPush A;
     PushConsts P;
     AdjK<double> K;
@@ -691,6 +732,7 @@ struct AdjView64 {
 
 struct AdjView32 {
     using T = float;
+    static constexpr int MIN_WAVES = 1;
     AdjPush32 A;
     AdjK<float> K;
     int n[3];
@@ -754,20 +796,28 @@ struct AdjBox {
     static constexpr size_t LDS = (size_t)6 * FVOL * sizeof(LACC) + (size_t)(3 * AVOL + 6 * FVOL) * sizeof(T);
 };
 
-template <int M, int TX, int TY, int TZ, typename V, typename LACC>
-__global__ __launch_bounds__(256) void k_adj_push_tiled(V v, Grid g, Tiling tl,
+//
+// SPLIT > 1: the tile is cut along x into SPLIT parts, each work item runs
+// as SPLIT workgroups, and a workgroup takes the particles of its part only
+// (particles outside the whole tile go to the stray list from part 0).
+// The LDS boxes shrink with the part, so more workgroups fit a CU.
+template <int M, int TX, int TY, int TZ, typename V, typename LACC, int SPLIT = 1>
+__global__ __launch_bounds__(256, V::MIN_WAVES) void k_adj_push_tiled(V v, Grid g, Tiling tl,
                                                         const int64_t* __restrict__ tile_off,
                                                         const int64_t* __restrict__ work,
                                                         const int64_t* __restrict__ nwork, int chunk,
                                                         int64_t* __restrict__ stray_list,
                                                         int64_t* __restrict__ stray_n) {
     using T = typename V::T;
-    using B = AdjBox<M, TX, TY, TZ, T, LACC>;
+    constexpr int PX = TX / SPLIT;  // x cells of this workgroup's part
+    static_assert(PX * SPLIT == TX, "the tile splits evenly");
+    using B = AdjBox<M, PX, TY, TZ, T, LACC>;
     extern __shared__ double lds_adj[];
     LACC* lfb = reinterpret_cast<LACC*>(lds_adj);    // 6 * FVOL: lambda_F accumulator
     T* lj = reinterpret_cast<T*>(lfb + 6 * B::FVOL);  // 3 * AVOL: lambda_J of the deposit box
     T* fld = lj + 3 * B::AVOL;                        // 6 * FVOL: the fields of the gather box
-    const int64_t wi = blockIdx.x;
+    const int64_t wi = blockIdx.x / SPLIT;
+    const int part = (int)(blockIdx.x % SPLIT);
     if (wi >= *nwork) return;
     const int64_t item = work[wi];
     const int tile = (int)(item >> 20);
@@ -775,7 +825,7 @@ __global__ __launch_bounds__(256) void k_adj_push_tiled(V v, Grid g, Tiling tl,
     const int64_t beg = tile_off[tile] + cidx * chunk;
     const int64_t end = min(tile_off[tile + 1], beg + chunk);
     const int txi = tile % tl.nt[0], tyi = (tile / tl.nt[0]) % tl.nt[1], tzi = tile / (tl.nt[0] * tl.nt[1]);
-    const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
+    const int Tx = txi * TX, T0x = Tx + part * PX, T0y = tyi * TY, T0z = tzi * TZ;
     const int m = g.m;
     // deposit box origin (J buffer indices) and gather box origin (E/B)
     const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
@@ -803,11 +853,12 @@ __global__ __launch_bounds__(256) void k_adj_push_tiled(V v, Grid g, Tiling tl,
         int cell[3], base[3];
         T frac[3];
         if (!v.locate(p, cell, base, frac)) continue;
-        if ((unsigned)(cell[0] - T0x) >= (unsigned)TX || (unsigned)(cell[1] - T0y) >= (unsigned)TY ||
+        if ((unsigned)(cell[0] - Tx) >= (unsigned)TX || (unsigned)(cell[1] - T0y) >= (unsigned)TY ||
             (unsigned)(cell[2] - T0z) >= (unsigned)TZ) {
-            strays |= 1u << it;
+            if (part == 0) strays |= 1u << it;
             continue;
         }
+        if ((unsigned)(cell[0] - T0x) >= (unsigned)PX) continue;  // another part's particle
         v.template adjoint<M>(
             p, base, frac,
             [&](int c, int k, int j, int i) {
@@ -1482,19 +1533,25 @@ AdjK<float> adj_consts32(const pic_ctx* c, const Species& S) {
 
 int tile_kind(const pic_ctx* c) { return c->fp32 ? (c->fix32 ? 2 : 1) : 0; }
 
+// Parts per tile (split along x) so that two workgroups' LDS fit a CU
+// (the whole-tile boxes of the FP32 quartic kernel take 98 KB); the FP64
+// kernel runs one wave per SIMD (registers) and keeps whole tiles.
+template <int M, int KIND> struct AdjSplit { static constexpr int S = 1; };
+
 template <int M, int KIND, typename V, typename LACC = double>
 int launch_adj_tiled(pic_ctx* c, DiffState& D, const V& v) {
     using TS = TileShape<KIND, M>;
-    using B = AdjBox<M, TS::X, TS::Y, TS::Z, typename V::T, LACC>;
-    Species& S = c->sp[D.s];
+    constexpr int S = AdjSplit<M, KIND>::S;
+    using B = AdjBox<M, TS::X / S, TS::Y, TS::Z, typename V::T, LACC>;
+    Species& Sp = c->sp[D.s];
     int r = build_work_on(c, D.atoff, PIC_PUSH_CHUNK, D.awork, D.anwork, D.as);
     if (r) return r;
-    PIC_HIP(c, hipFuncSetAttribute((const void*)k_adj_push_tiled<M, TS::X, TS::Y, TS::Z, V, LACC>,
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_adj_push_tiled<M, TS::X, TS::Y, TS::Z, V, LACC, S>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
-    const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
+    const int64_t max_items = (int64_t)c->tl.ntiles + Sp.cap / PIC_PUSH_CHUNK + 1;
     int64_t* cnt = D.stray + std::max<int64_t>(D.slots, 1);
     if ((r = zero_adj(c, D, cnt, 16))) return r;
-    k_adj_push_tiled<M, TS::X, TS::Y, TS::Z, V, LACC><<<(unsigned)max_items, 256, B::LDS, D.as>>>(
+    k_adj_push_tiled<M, TS::X, TS::Y, TS::Z, V, LACC, S><<<(unsigned)(max_items * S), 256, B::LDS, D.as>>>(
         v, c->g, c->tl, D.atoff, D.awork, D.anwork, PIC_PUSH_CHUNK, D.stray, cnt);
     PIC_LAUNCH_CHECK(c);
     k_adj_push_hbm<M, V><<<1024, 256, 0, D.as>>>(v, c->g, D.stray, cnt);
