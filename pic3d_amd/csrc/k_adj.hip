@@ -695,8 +695,8 @@ struct AdjPush32 {
 // adjoint through adj_particle, written back to the slot.
 struct AdjView64 {
     using T = double;
-    // waves per SIMD the tiled adjoint asks of the register allocator: the
-    // FP64 kernel would spill heavily below its 256+ registers
+    // waves per SIMD the tiled adjoint asks of the register allocator (two
+    // would spill heavily)
     static constexpr int MIN_WAVES = 1;
     AdjPush A;
     PushConsts P;
@@ -1533,9 +1533,10 @@ AdjK<float> adj_consts32(const pic_ctx* c, const Species& S) {
 
 int tile_kind(const pic_ctx* c) { return c->fp32 ? (c->fix32 ? 2 : 1) : 0; }
 
-// Parts per tile (split along x) so that two workgroups' LDS fit a CU
-// (the whole-tile boxes of the FP32 quartic kernel take 98 KB); the FP64
-// kernel runs one wave per SIMD (registers) and keeps whole tiles.
+// Parts per tile (split along x).  Whole tiles: the kernels hold 256
+// registers (one wave per SIMD), and a split that let two workgroups'
+// LDS fit a CU needed the registers capped at two waves per SIMD, which
+// spilled and measured slower.
 template <int M, int KIND> struct AdjSplit { static constexpr int S = 1; };
 
 template <int M, int KIND, typename V, typename LACC = double>
