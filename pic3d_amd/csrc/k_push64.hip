@@ -18,6 +18,7 @@
 #include "atomics.h"
 #include "push64.h"
 #include "shapes.h"
+#include "wave.h"
 
 namespace pic {
 
@@ -409,6 +410,8 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
         fld[e] = fp[c][g.fi(gz, gy, gx)];
     }
     for (int e = threadIdx.x; e < 4 * B::VOL; e += blockDim.x) acc[e] = 0.0;
+    __shared__ unsigned rg_hist[256 / 64][8];  // per wave: bank-spreading counters
+    if (lane_id() < 8) rg_hist[threadIdx.x >> 6][lane_id()] = 0u;
     __syncthreads();
     // gather from the LDS field tile (an in-tile particle's stencil lies in it)
     auto fr = [&](int c, int k, int j, int i) -> double {
@@ -421,20 +424,50 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
     uint32_t strays = 0;  // iterations whose particle was outside the tile
     int it = 0;
     const int len = (int)(end - beg);
-    for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
-        const int64_t p = beg + t;
-        const double wp = P.w[p];
-        if (!(wp == wp)) continue;
-        double x = P.x[p], y = P.y[p], z = P.z[p];
-        if (!wrap_check(x, y, z, K)) { oob++; continue; }
-        // cell after the wrap (x == N belongs to the last cell)
-        const int cx = min((int)x, g.n[0] - 1), cy = min((int)y, g.n[1] - 1), cz = min((int)z, g.n[2] - 1);
-        if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
-            (unsigned)(cz - T0z) >= (unsigned)TZ) {
-            strays |= 1u << it;  // left its tile since the last sort: HBM pass below
-            continue;
+    // Each wave takes 64 consecutive slots (coalesced loads), then deals its
+    // particles out again over its lanes by the LDS bank of their deposit
+    // base (bank_spread_lane); results go back to the particles' own slots.
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    for (int t0 = threadIdx.x & ~63; t0 < len; t0 += blockDim.x, it++) {
+        const int t = t0 + lane;
+        double wp = 0.0, x = 0.0, y = 0.0, z = 0.0, ux = 0.0, uy = 0.0, uz = 0.0;
+        bool ok = false;
+        int r = 32;
+        if (t < len) {
+            const int64_t p = beg + t;
+            wp = P.w[p];
+            x = P.x[p]; y = P.y[p]; z = P.z[p];
+            ok = wp == wp;
+            if (ok && !wrap_check(x, y, z, K)) { oob++; ok = false; }
+            if (ok) {
+                // cell after the wrap (x == N belongs to the last cell)
+                const int cx = min((int)x, g.n[0] - 1), cy = min((int)y, g.n[1] - 1), cz = min((int)z, g.n[2] - 1);
+                if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
+                    (unsigned)(cz - T0z) >= (unsigned)TZ) {
+                    strays |= 1u << it;  // left its tile since the last sort: HBM pass below
+                    ok = false;
+                }
+            }
+            if (ok) {
+                ux = P.ux[p]; uy = P.uy[p]; uz = P.uz[p];
+                // deposit box base: the nearest node of the old position, in 8-byte words
+                const double org = (double)(1 - g.m);
+                const int ib = (int)(x - org + 0.5) - B::H, jb = (int)(y - org + 0.5) - B::H,
+                          kb = (int)(z - org + 0.5) - B::H;
+                const int ofs = ((kb - (T0z + g.m - 2 - B::H)) * B::BY + (jb - (T0y + g.m - 2 - B::H))) * B::PX +
+                                (ib - (T0x + g.m - 2 - B::H));
+                r = (2 * ofs) & 31;
+            }
         }
-        double ux = P.ux[p], uy = P.uy[p], uz = P.uz[p], xn, yn, zn;
+        const int to = bank_spread_lane(r, rg_hist[wv]);
+        const int okv = lane_send(to, ok ? 1 : 0);
+        const int tt = lane_send(to, t);
+        x = lane_send_d(to, x); y = lane_send_d(to, y); z = lane_send_d(to, z);
+        ux = lane_send_d(to, ux); uy = lane_send_d(to, uy); uz = lane_send_d(to, uz);
+        wp = lane_send_d(to, wp);
+        if (!okv) continue;
+        const int64_t p = beg + tt;
+        double xn, yn, zn;
         double Sx0[M + 1], Sy0[M + 1], Sz0[M + 1];
         int i1, j1, k1;
         double ebq[6], gam;

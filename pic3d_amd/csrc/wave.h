@@ -69,6 +69,13 @@ __device__ __forceinline__ float lane_send(int to, float v) {
     return __int_as_float(__builtin_amdgcn_ds_permute(to << 2, __float_as_int(v)));
 }
 
+__device__ __forceinline__ double lane_send_d(int to, double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_permute(to << 2, (int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_ds_permute(to << 2, (int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Contiguous slot range [lo, hi) of this block for a sort pass over n
 // slots, in steps of blockDim.x * R.  Blocks resident at one time then
 // work on distant parts of the (mostly tile-ordered) array, so their
