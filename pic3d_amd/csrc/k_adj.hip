@@ -32,6 +32,7 @@
 #include "pic_internal.h"
 #include "push64.h"
 #include "shapes.h"
+#include "wave.h"
 
 namespace pic {
 
@@ -837,6 +838,8 @@ __global__ __launch_bounds__(256, V::MIN_WAVES) void k_adj_push_tiled(V v, Grid 
         const int gx = min(aox + lx, g.je[0] - 2), gy = min(aoy + ly, g.je[1] - 2), gz = min(aoz + lz, g.je[2] - 2);
         lj[e] = v.A.lJ[c][g.ji(gz, gy, gx)];
     }
+    __shared__ unsigned rg_hist[256 / 64][8];  // per wave: bank-spreading counters
+    if (lane_id() < 8) rg_hist[threadIdx.x >> 6][lane_id()] = 0u;
     for (int e = threadIdx.x; e < 6 * B::FVOL; e += blockDim.x) {
         lfb[e] = LACC(0);
         const int c = e / B::FVOL, r = e % B::FVOL;
@@ -848,17 +851,36 @@ __global__ __launch_bounds__(256, V::MIN_WAVES) void k_adj_push_tiled(V v, Grid 
     uint32_t strays = 0;
     int it = 0;
     const int len = (int)(end - beg);
-    for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
-        const int64_t p = beg + t;
-        int cell[3], base[3];
-        T frac[3];
-        if (!v.locate(p, cell, base, frac)) continue;
-        if ((unsigned)(cell[0] - Tx) >= (unsigned)TX || (unsigned)(cell[1] - T0y) >= (unsigned)TY ||
-            (unsigned)(cell[2] - T0z) >= (unsigned)TZ) {
+    // Each wave takes 64 consecutive slots, then deals its particles out
+    // over its lanes by the LDS bank of their lambda_F base
+    // (bank_spread_lane), as the forward push does.
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    for (int t0 = threadIdx.x & ~63; t0 < len; t0 += blockDim.x, it++) {
+        const int t = t0 + lane;
+        int cell[3] = {0, 0, 0}, base[3] = {0, 0, 0};
+        T frac[3] = {T(0), T(0), T(0)};
+        bool ok = t < len && v.locate(beg + t, cell, base, frac);
+        if (ok && ((unsigned)(cell[0] - Tx) >= (unsigned)TX || (unsigned)(cell[1] - T0y) >= (unsigned)TY ||
+                   (unsigned)(cell[2] - T0z) >= (unsigned)TZ)) {
             if (part == 0) strays |= 1u << it;
-            continue;
+            ok = false;
         }
-        if ((unsigned)(cell[0] - T0x) >= (unsigned)PX) continue;  // another part's particle
+        if (ok && (unsigned)(cell[0] - T0x) >= (unsigned)PX) ok = false;  // another part's particle
+        int r = 32;
+        if (ok) {
+            int nn[3];
+#pragma unroll
+            for (int a = 0; a < 3; a++) nn[a] = base[a] + (int)floor(frac[a] + T(0.5));
+            const int idx = ((nn[2] - foz) * B::FY + (nn[1] - foy)) * B::FX + (nn[0] - fox);
+            r = (idx * (int)(sizeof(LACC) / 4)) & 31;
+        }
+        const int to = bank_spread_lane(r, rg_hist[wv]);
+        const int okv = lane_send(to, ok ? 1 : 0);
+        const int tt = lane_send(to, t);
+#pragma unroll
+        for (int a = 0; a < 3; a++) { base[a] = lane_send(to, base[a]); frac[a] = lane_send(to, frac[a]); }
+        if (!okv) continue;
+        const int64_t p = beg + tt;
         v.template adjoint<M>(
             p, base, frac,
             [&](int c, int k, int j, int i) {

@@ -76,6 +76,8 @@ __device__ __forceinline__ double lane_send_d(int to, double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+__device__ __forceinline__ double lane_send(int to, double v) { return lane_send_d(to, v); }
+
 // Contiguous slot range [lo, hi) of this block for a sort pass over n
 // slots, in steps of blockDim.x * R.  Blocks resident at one time then
 // work on distant parts of the (mostly tile-ordered) array, so their
