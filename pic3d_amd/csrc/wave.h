@@ -30,15 +30,16 @@ __device__ __forceinline__ void wave_runs(int64_t key, bool valid, int& head_lan
     run_len = next - lane;
 }
 
-// Bank-spreading lane assignment.  Lanes whose LDS deposit bases fall in
-// the same bank conflict when they are served in the same group of 16
-// lanes (32 banks of 4 bytes; an LDS instruction serves a wave's 64 lanes
-// 16 at a time).  Given each lane's bank residue r in [0, 32) (r = 32: the
-// lane has no particle), returns the lane that should hold this lane's
-// particle: lanes are ordered by residue and dealt round-robin to the four
-// groups of 16, so a residue held by up to four lanes never repeats inside
-// a group.  hist: this wave's 8 words of LDS (32 byte-wide counters), zero
-// on entry and left zero.
+// Bank-spreading lane assignment.  Lanes whose LDS bases fall in the same
+// bank conflict when they are served together (32 banks of 4 bytes; an
+// atomic or 8-byte access serves a wave's 64 lanes 16 at a time, a 4-byte
+// read 32 at a time).  Given each lane's bank residue r in [0, 32) (r = 32:
+// the lane has no particle), returns the lane that should hold this lane's
+// particle: lanes are ordered by residue and dealt to the groups of 16 in
+// the order 0, 2, 1, 3, so a residue held by up to four lanes never
+// repeats inside a group of 16, and one held by two never inside a half.
+// hist: this wave's 8 words of LDS (32 byte-wide counters), zero on entry
+// and left zero.
 __device__ __forceinline__ int bank_spread_lane(int r, unsigned* hist) {
     const int lane = lane_id();
     int rank = 0;
@@ -60,7 +61,8 @@ __device__ __forceinline__ int bank_spread_lane(int r, unsigned* hist) {
     if (lane < 8) hist[lane] = 0u;
     const uint64_t none = __ballot(r >= 32);
     const int pos = r < 32 ? base + rank : nvalid + __popcll(none & ((1ull << lane) - 1ull));
-    return (pos & 3) * 16 + (pos >> 2);
+    const int q = ((pos & 1) << 1) | ((pos >> 1) & 1);
+    return q * 16 + (pos >> 2);
 }
 
 // value v of this lane sent to lane 'to' (a permutation of the wave)
