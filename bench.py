@@ -16,7 +16,10 @@ particle boundary, current fold, E/B half steps).  W untimed warm-up steps,
 then K timed steps bracketed by stream synchronisation.  Prints one JSON
 line: value = particle pushes per second over whole steps; Yee
 cell-updates/s and the push kernel's roofline come from HIP events recorded
-around every phase inside the timed region.
+around every phase inside the timed region.  The roofline object also
+carries the push kernel's HBM traffic, VALU and LDS utilisation from the
+committed rocprofv3 summaries under profiles/, and the line carries the
+bandwidth of a plain 1 GiB device copy measured in a child process.
 """
 from __future__ import annotations
 
@@ -65,6 +68,30 @@ def load_traffic(name: str):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":")[1])
     return None
+
+
+VECTOR_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X vector (non-matrix) peaks
+
+
+def load_util(name: str):
+    """VALU / LDS utilisation of the push kernel from the committed
+    rocprofv3 --pmc summary (rocprofv3's derived VALUBusy, LdsUtil,
+    LDSBankConflict; executed FP instructions per push), or None when that
+    profile is absent.  These counters cannot be read from inside the timed
+    run, so they come from a profiled run of the same command."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    out = {"source": f"profiles/{name}"}
+    for line in open(path):
+        line = line.strip()
+        if line.startswith("mean: VALUBusy"):
+            for part in line[len("mean: "):].split(","):
+                k, v = part.split()[:2]
+                out[k] = round(float(v) / 100.0, 4)
+        elif line.startswith("flop per dispatch"):
+            out["flop_per_push"] = float(line.split("=")[2].split()[0])
+    return out
 
 
 def cpu_baseline(wl_name: str, precision: str, threads: int) -> dict:
@@ -152,6 +179,11 @@ def run_gpu(args, precision: str) -> dict:
     bpp = push_bytes(precision, ppc)
     achieved = bpp * 0.5 * (n0 + n1) / avg_push_s / 1e9
     traffic = load_traffic(f"push_{precision}_{args.workload}_pmc.txt")
+    util = load_util(f"push_{precision}_{args.workload}_util.txt")
+    if util and "flop_per_push" in util:
+        # executed flops per push (profiled) over this run's measured launch time
+        util["valu_tflops"] = util["flop_per_push"] * 0.5 * (n0 + n1) / avg_push_s / 1e12
+        util["valu_frac_of_vector_peak"] = util["valu_tflops"] / VECTOR_PEAK_TFLOPS[precision]
     out = {
         "precision": precision,
         "pushes_per_s": pushes / wall,
@@ -164,7 +196,8 @@ def run_gpu(args, precision: str) -> dict:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "push (gather+Boris+Esirkepov)",
                      "bytes_per_push": bpp,
-                     "avg_launch_ms": avg_push_s * 1e3},
+                     "avg_launch_ms": avg_push_s * 1e3,
+                     "valu_lds": util},
         "yee_roofline": {"bytes_per_cell_update": cell_bytes(precision),
                          "achieved_GBs": (cell_bytes(precision) * cells * args.steps /
                                           (fields_ms / 1e3) / 1e9) if fields_ms > 0 else None},
@@ -253,6 +286,39 @@ def copy_probe(nbytes: int):
     print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}), flush=True)
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 5) -> float:
+    """Measured HBM bandwidth of a plain elementwise copy (reads nbytes,
+    writes nbytes), GB/s, best of reps launches timed with events on torch's
+    stream: the attainable figure beside the 8 TB/s published peak."""
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    best = float("inf")
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        torch.mul(src, 1.0, out=dst)
+        b.record()
+        b.synchronize()
+        best = min(best, a.elapsed_time(b))
+    return 2.0 * n * 4 / (best / 1e3) / 1e9
+
+
+def measured_copy_gbs():
+    """copy_bandwidth() in a child process: torch, which links its own HIP
+    runtime, found no device in a process where the library had already
+    opened it."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-bandwidth"],
+                       capture_output=True, text=True, timeout=300)
+    for ln in r.stdout.splitlines():
+        if ln.startswith("{"):
+            return json.loads(ln).get("copy_GBs")
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,7 +349,12 @@ def main():
                     help="only run a plain device copy of BYTES (read BYTES, write BYTES) "
                          "three times: the known-size kernel that scales rocprofv3 "
                          "FETCH_SIZE / WRITE_SIZE readings to bytes")
+    ap.add_argument("--copy-bandwidth", action="store_true",
+                    help="only time a 1 GiB device copy and print its bandwidth")
     args = ap.parse_args()
+    if args.copy_bandwidth:
+        print(json.dumps({"copy_GBs": copy_bandwidth()}), flush=True)
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -315,6 +386,7 @@ def main():
         "by_precision": {pr: {k: v for k, v in r.items() if k not in ("workload",)}
                          for pr, r in res.items()},
     }
+    line["hbm_copy_GBs"] = measured_copy_gbs()
     if "fp64" in res and "fp32" in res:
         line["fp32_over_fp64"] = res["fp32"]["pushes_per_s"] / res["fp64"]["pushes_per_s"]
     if args.ramp_steps > 0 and args.workload == "laser_slab":
