@@ -193,7 +193,9 @@ __device__ __forceinline__ void boris_vjp(const T* u, const T* E, const T* B, T 
 
 // Esirkepov deposit of particle.cpp:372-500 (J only: rho feeds nothing)
 // as a function of the six positions; 'visit(c, k, j, i, value)' gets the
-// contribution to component c at buffer node (k, j, i).
+// contribution to component c at buffer node (k, j, i).  The points the
+// push kernels leave out (the last of the particle's box along the
+// cumulative axis, whose value is a rounding residue) are left out here.
 template <int M, typename T, typename V>
 __device__ __forceinline__ void esirkepov_t(double q, T xo, T yo, T zo, T xn, T yn, T zn,
                                             const PushConsts& K, int m, V&& visit) {
@@ -224,8 +226,9 @@ __device__ __forceinline__ void esirkepov_t(double q, T xo, T yo, T zo, T xn, T 
             const T sy0 = Y0[j], dsy = Y1[j] - sy0, sz0 = Z0[k], dsz = Z1[k] - sz0;
             const T wyz = sy0 * sz0 + 0.5 * dsy * sz0 + 0.5 * sy0 * dsz + third * dsy * dsz;
             T acc(0.0);
-            for (int i = 0; i < L; i++) {
+            for (int i = 0; i < L - 1; i++) {
                 acc += X1[i] - X0[i];
+                if (i == L - 2 && i1 == i2) break;
                 visit(0, bz + k, by + j, bx + i, -(cx * acc * wyz));
             }
         }
@@ -234,8 +237,9 @@ __device__ __forceinline__ void esirkepov_t(double q, T xo, T yo, T zo, T xn, T 
             const T sx0 = X0[i], dsx = X1[i] - sx0, sz0 = Z0[k], dsz = Z1[k] - sz0;
             const T wxz = sx0 * sz0 + 0.5 * dsx * sz0 + 0.5 * sx0 * dsz + third * dsx * dsz;
             T acc(0.0);
-            for (int j = 0; j < L; j++) {
+            for (int j = 0; j < L - 1; j++) {
                 acc += Y1[j] - Y0[j];
+                if (j == L - 2 && j1 == j2) break;
                 visit(1, bz + k, by + j, bx + i, -(cy * acc * wxz));
             }
         }
@@ -244,8 +248,9 @@ __device__ __forceinline__ void esirkepov_t(double q, T xo, T yo, T zo, T xn, T 
             const T sx0 = X0[i], dsx = X1[i] - sx0, sy0 = Y0[j], dsy = Y1[j] - sy0;
             const T wxy = sx0 * sy0 + 0.5 * dsx * sy0 + 0.5 * sx0 * dsy + third * dsx * dsy;
             T acc(0.0);
-            for (int k = 0; k < L; k++) {
+            for (int k = 0; k < L - 1; k++) {
                 acc += Z1[k] - Z0[k];
+                if (k == L - 2 && k1 == k2) break;
                 visit(2, bz + k, by + j, bx + i, -(cz * acc * wxy));
             }
         }
@@ -432,12 +437,16 @@ template <int M, typename T> __device__ __forceinline__ void place_t(const T* S,
 // Adjoint of one Esirkepov current component along axis a (cumulative
 // sum along a; weights over the other two axes b (middle loop) and c
 // (outer loop)).  lj(oc, ob, oa) reads lambda of this component at the box
-// point.  Adds to g[0..5] (d/dx0,y0,z0,x1,y1,z1) and returns G.
+// point.  Adds to g[0..5] (d/dx0,y0,z0,x1,y1,z1) and returns G.  As in the
+// forward deposit, the last point of the particle's box along a (box point
+// L-1 if it changes node along a, 'moved', else L-2) is not deposited, so
+// it takes no lambda.
 template <int L, typename T, typename LJ>
 __device__ __forceinline__ T esirkepov_adj_axis(int a, const T* A0, const T* A1, const T* dA0,
                                                 const T* dA1, const T* B0, const T* B1, const T* dB0,
                                                 const T* dB1, const T* C0, const T* C1, const T* dC0,
-                                                const T* dC1, T coef, LJ lj, T* g, int ab, int ac) {
+                                                const T* dC1, T coef, LJ lj, T* g, int ab, int ac,
+                                                bool moved) {
     T S[L], dS0[L], dS1[L];
     T s = T(0), s0 = T(0), s1 = T(0);
 #pragma unroll
@@ -466,8 +475,8 @@ __device__ __forceinline__ T esirkepov_adj_axis(int a, const T* A0, const T* A1,
             const T W = b0 * c0 + half * db * c0 + half * b0 * dc + third * db * dc;
             T q = T(0);
 #pragma unroll
-            for (int oa = 0; oa < L; oa++) {
-                const T lam = lj(oc, ob, oa);
+            for (int oa = 0; oa < L - 1; oa++) {
+                const T lam = (oa < L - 2 || moved) ? lj(oc, ob, oa) : T(0);
                 P[oa] += lam * W;
                 q += lam * S[oa];
             }
@@ -531,15 +540,15 @@ __device__ __forceinline__ T deposit_adj(const int* base, const T* f0, const T* 
     // Jx: cumulative in x, weights over y (middle) and z (outer)
     G += esirkepov_adj_axis<L, T>(0, X0, X1, dX0, dX1, Y0, Y1, dY0, dY1, Z0, Z1, dZ0, dZ1, K.jc[0],
                                   [&](int oz, int oy, int ox) { return lj(0, bz + oz, by + oy, bx + ox); },
-                                  g, 1, 2);
+                                  g, 1, 2, i1 != i2);
     // Jy: cumulative in y, weights over x (middle) and z (outer)
     G += esirkepov_adj_axis<L, T>(1, Y0, Y1, dY0, dY1, X0, X1, dX0, dX1, Z0, Z1, dZ0, dZ1, K.jc[1],
                                   [&](int oz, int ox, int oy) { return lj(1, bz + oz, by + oy, bx + ox); },
-                                  g, 0, 2);
+                                  g, 0, 2, j1 != j2);
     // Jz: cumulative in z, weights over x (middle) and y (outer)
     G += esirkepov_adj_axis<L, T>(2, Z0, Z1, dZ0, dZ1, X0, X1, dX0, dX1, Y0, Y1, dY0, dY1, K.jc[2],
                                   [&](int oy, int ox, int oz) { return lj(2, bz + oz, by + oy, bx + ox); },
-                                  g, 0, 1);
+                                  g, 0, 1, k1 != k2);
     return G;
 }
 

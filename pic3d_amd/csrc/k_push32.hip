@@ -144,9 +144,10 @@ __device__ __forceinline__ void boris32(float* u, const float* E, const float* B
 // offsets f0 in [0,1), new offsets f1 = f0 + d with |d| < 1.  The sink D
 // receives D.base(bz, by, bx) (box origin relative to the cell) and then
 // D(comp, k, j, i, v) with box-relative (k, j, i).  The box is always
-// L = M+2 points per axis; the last point of each cumulative sum (which
-// carries sum(S1) - sum(S0) = 0) is not deposited, and the extra planes
-// along the other axes carry exact zeros.
+// L = M+2 points per axis.  The last point of the particle's box along the
+// cumulative axis (box point L-1 if the particle changes node along that
+// axis, else L-2) carries sum(S1) - sum(S0), zero in exact arithmetic, and
+// is not deposited; the extra planes along the other axes carry exact zeros.
 // S0x, S0y, S0z: the weights at f0 (as the gather evaluates them)
 template <int M, typename D>
 __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const float* f1, const float* S0x,
@@ -156,12 +157,14 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
     const float* S0[3] = {S0x, S0y, S0z};
     float S1[3][M + 1];
     int b[3], d0[3], d1[3];
+    bool moved[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const int i1 = f0[a] >= 0.5f ? 1 : 0;
         const int i2 = (int)floorf(f1[a] + 0.5f);  // -1 .. 2
         shape32<M>(f1[a] - (float)i2, S1[a]);
         b[a] = min(i1, i2) - H;
+        moved[a] = i1 != i2;
         d0[a] = i1 - H - b[a];
         d1[a] = i2 - H - b[a];
     }
@@ -196,7 +199,7 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
 #pragma unroll
             for (int i = 0; i < L - 1; i++) {
                 acc += X1[i] - X0[i];
-                dst(0, k, j, i, acc * wyz);
+                if (i < L - 2 || moved[0]) dst(0, k, j, i, acc * wyz);
             }
         }
         shift_down32<L>(za); shift_down32<L>(zb);
@@ -216,7 +219,7 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
 #pragma unroll
             for (int j = 0; j < L - 1; j++) {
                 acc += Y1[j] - Y0[j];
-                dst(1, k, j, i, acc * wxz);
+                if (j < L - 2 || moved[1]) dst(1, k, j, i, acc * wxz);
             }
         }
         shift_down32<L>(za); shift_down32<L>(zb);
@@ -237,7 +240,7 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
 #pragma unroll
             for (int k = 0; k < L - 1; k++) {
                 acc += Z1[k] - Z0[k];
-                dst(2, k, j, i, acc * wxy);
+                if (k < L - 2 || moved[2]) dst(2, k, j, i, acc * wxy);
             }
         }
         shift_down32<L>(ya); shift_down32<L>(yb);

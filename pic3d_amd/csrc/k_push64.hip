@@ -132,11 +132,15 @@ __device__ __forceinline__ int spin_step(const P64& P, int64_t p, const double* 
 // so the loops unroll into registers with immediate LDS offsets.  Callers
 // guarantee |x_new - x_old| < 1 cell per axis, hence |i2 - i1| <= 1.  Values
 // and the order of each cumulative sum are the reference's, so every term
-// the reference deposits is deposited bit-identically; the extra points are
+// below is the reference's bit for bit.  Points of the L-box skipped:
 // (a) planes outside the reference's box along the non-cumulative axes,
 // whose weights are exactly zero (adding +-0 changes no value), and (b) the
-// last point of a cumulative sum when the particle stays on its node, which
-// is predicated off.
+// last point of the reference's box along the cumulative axis (box point
+// L-1 if the particle changes node along that axis, else L-2).  There the
+// cumulative sum is sum(S1) - sum(S0), zero in exact arithmetic since both
+// shapes sum to one: the reference adds a rounding residue of about 1e-16
+// of the particle's contribution at that point, which is left out here (one
+// LDS atomic per row saved).
 template <int M>
 __device__ __forceinline__ void place(const double* S, int d, double* A) {
 #pragma unroll
@@ -181,9 +185,9 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
             if (wyz == 0.0) continue;  // a plane outside both supports: only +-0 to add
             double acc = 0.0;
 #pragma unroll
-            for (int i = 0; i < L; i++) {
+            for (int i = 0; i < L - 1; i++) {
                 acc += X1[i] - X0[i];
-                if (i < L - 1 || crx) dst(0, k, j, i, -(cx * acc * wyz));
+                if (i < L - 2 || crx) dst(0, k, j, i, -(cx * acc * wyz));
             }
         }
     }
@@ -200,9 +204,9 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
             if (wxz == 0.0) continue;
             double acc = 0.0;
 #pragma unroll
-            for (int j = 0; j < L; j++) {
+            for (int j = 0; j < L - 1; j++) {
                 acc += Y1[j] - Y0[j];
-                if (j < L - 1 || cry) dst(1, k, j, i, -(cy * acc * wxz));
+                if (j < L - 2 || cry) dst(1, k, j, i, -(cy * acc * wxz));
             }
         }
     }
@@ -220,9 +224,9 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
             if (wxy == 0.0) continue;
             double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < L; k++) {
+            for (int k = 0; k < L - 1; k++) {
                 acc += Z1[k] - Z0[k];
-                if (k < L - 1 || crz) dst(2, k, j, i, -(cz * acc * wxy));
+                if (k < L - 2 || crz) dst(2, k, j, i, -(cz * acc * wxy));
             }
         }
     }
