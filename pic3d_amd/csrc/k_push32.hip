@@ -16,6 +16,7 @@
 // and boundary crossings are classified by the push itself (the deposit
 // still uses the unwrapped position, so the ghost fold carries the current).
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "pic_internal.h"
@@ -376,8 +377,18 @@ struct Fields32 {
     float *jx, *jy, *jz, *rho;
 };
 
+// Threads per workgroup of the tiled push.  The LDS boxes allow two
+// workgroups per CU, so 256 threads leave each SIMD two waves.  The
+// quadratic int32-accumulator kernel without spin takes 512 threads at the
+// 128 registers it uses anyway (four waves per SIMD): 29 % faster at the
+// 1.07e9-electron ramp target.  The quartic one measured 21 % slower at 384
+// threads (three waves per SIMD at its 158 registers) and keeps 256.
+template <int M, typename ACC, bool SPIN> struct Push32Threads {
+    static constexpr int N = (M == 2 && !SPIN && std::is_same<ACC, int>::value) ? 512 : 256;
+};
+
 template <int M, int TX, int TY, int TZ, typename ACC, bool SPIN>
-__global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid g, Consts32 K,
+__global__ __launch_bounds__((Push32Threads<M, ACC, SPIN>::N), 2) void k_push32_tiled(Soa32 P, Fields32 F, Grid g, Consts32 K,
                                                       Tiling tl, const int64_t* __restrict__ tile_off,
                                                       const int64_t* __restrict__ work,
                                                       const int64_t* __restrict__ nwork, int chunk,
@@ -387,7 +398,7 @@ __global__ __launch_bounds__(256) void k_push32_tiled(Soa32 P, Fields32 F, Grid 
     ACC* acc = reinterpret_cast<ACC*>(lds_d);             // 4 * AVOL accumulators
     float* fld = reinterpret_cast<float*>(acc + 4 * B::AVOL);  // 6 * FVOL field tile
     __shared__ unsigned cnt12[12];
-    __shared__ unsigned rg_hist[256 / 64][8];  // per wave: bank-spreading counters
+    __shared__ unsigned rg_hist[Push32Threads<M, ACC, SPIN>::N / 64][8];  // per wave: bank-spreading counters
     const int64_t wi = blockIdx.x;
     if (wi >= *nwork) return;
     const int64_t item = work[wi];
@@ -862,7 +873,8 @@ static int launch_tiled32_s(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
     const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
     PIC_HIP(c, hipFuncSetAttribute((const void*)k_push32_tiled<M, TX, TY, TZ, ACC, SPIN>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
-    k_push32_tiled<M, TX, TY, TZ, ACC, SPIN><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
+    k_push32_tiled<M, TX, TY, TZ, ACC, SPIN><<<(unsigned)max_items, Push32Threads<M, ACC, SPIN>::N, B::LDS,
+                                               c->stream>>>(
         soa32(S, S.cur), Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, S.d_cnt);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
