@@ -331,8 +331,8 @@ def main():
     ap.add_argument("--sort-interval", type=int, default=None,
                     help="re-sort the tiled species every N pushes (default: the library's, 1)")
     ap.add_argument("--ramp-sort-interval", type=int, default=2,
-                    help="sort interval of the ramp-target line (every second push: 93 vs "
-                         "103 ms per step measured, the sort being a fifth of the step there)")
+                    help="sort interval of the ramp-target line (every second push: 68.5 vs "
+                         "80.0 ms per step measured, one sort taking 18-22 ms there)")
     ap.add_argument("--fp32-accum", type=int, default=1,
                     help="FP32 deposit accumulators: 1 int32 fixed point (default), 0 FP64 in LDS")
     ap.add_argument("--no-cpu-baseline", action="store_true")
