@@ -301,6 +301,18 @@ int pic_filter_currents(pic_ctx* ctx);
 int pic_deposit(pic_ctx* ctx, int s, int64_t n, const double* x_old, const double* y_old,
                 const double* z_old, const double* x, const double* y, const double* z,
                 const double* w, int64_t* skipped);
+/* No counterpart in the reference.  pic_deposit for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same deposit, with the
+ * caller's arrays staged in a fixed per-GPU device array instead of a
+ * buffer allocated per call.  It calls no hipMalloc/hipFree.  At most
+ * PIC_NOALLOC_MAX_PARTICLES particles per call (more: PIC_ERR_CAPACITY).
+ * Every call on one GPU shares that GPU's staging array: calls that
+ * overlap on one GPU race.  Synchronises. */
+#define PIC_NOALLOC_MAX_PARTICLES 65536
+int pic_deposit_noalloc(pic_ctx* ctx, int s, int64_t n, const double* x_old,
+                        const double* y_old, const double* z_old, const double* x,
+                        const double* y, const double* z, const double* w, int64_t* skipped);
 
 /* ---- differentiation form (no counterpart in the reference) ----
  *

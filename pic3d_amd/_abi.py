@@ -145,6 +145,8 @@ SIGNATURES = {
     "pic_diff_backward": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
     "pic_diff_end": (C.c_int, [C.c_void_p]),
     "pic_deposit": (C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7 + [C.POINTER(C.c_int64)]),
+    "pic_deposit_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int64] + [_dp] * 7
+                            + [C.POINTER(C.c_int64)]),
     "pic_field_get": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "pic_field_set": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "pic_field_size": (C.c_int64, [C.POINTER(PicParams), C.c_int]),

@@ -200,14 +200,16 @@ class Simulation:
     def filter_currents(self):
         self._chk(self._L.pic_filter_currents(self._h), "pic_filter_currents")
 
-    def deposit(self, s: int, parts: dict) -> int:
-        """currents_particle for every particle of parts (x_old.., x.., w)."""
+    def deposit(self, s: int, parts: dict, noalloc: bool = False) -> int:
+        """currents_particle for every particle of parts (x_old.., x.., w);
+        noalloc: through pic_deposit_noalloc (no device allocation)."""
         a = {k: np.ascontiguousarray(parts[k], dtype=np.float64)
              for k in ("x_old", "y_old", "z_old", "x", "y", "z", "w")}
         skipped = C.c_int64()
-        self._chk(self._L.pic_deposit(self._h, s, len(a["x"]), _d(a["x_old"]), _d(a["y_old"]),
-                                      _d(a["z_old"]), _d(a["x"]), _d(a["y"]), _d(a["z"]),
-                                      _d(a["w"]), C.byref(skipped)), "pic_deposit")
+        name = "pic_deposit_noalloc" if noalloc else "pic_deposit"
+        self._chk(getattr(self._L, name)(self._h, s, len(a["x"]), _d(a["x_old"]), _d(a["y_old"]),
+                                         _d(a["z_old"]), _d(a["x"]), _d(a["y"]), _d(a["z"]),
+                                         _d(a["w"]), C.byref(skipped)), name)
         return skipped.value
 
     def push(self, s: int):
