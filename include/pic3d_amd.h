@@ -86,18 +86,49 @@ typedef struct pic_counts {
     int64_t remaining;
 } pic_counts;
 
-/* FP64 diagnostics (north_star: energies, spectrum, Gauss residual). */
+/* FP64 diagnostics: energies, charge, spectrum and the Gauss-law residual
+ * (no counterpart in the reference, which has no diagnostics).
+ *
+ * Gauss law.  In the reference's step order (simulation.cpp:166-182) E's
+ * two half steps straddle the push, the first with the previous push's J
+ * and the second with the new one, and Esirkepov's deposit satisfies
+ * rho^{n+1} - rho^n = -dt div J^{n+1/2} (particle.cpp:372-515).  So the
+ * time-centred residual at every interior node
+ *     R = div E - (rho_prev + rho) / 2 - rho_background
+ * stays constant from step to step (to rounding) on periodic axes, where
+ *   - div E uses backward differences on the E stagger,
+ *   - rho is the sum of the mobile species' charge arrays now (rho^{n+1}),
+ *   - rho_prev is the same sum as it stood when the currents were last
+ *     cleared (rho^n; rho itself when no clear happened since the
+ *     particles, fields, species or background were last set).
+ * R changes at open faces: their ghost E is never updated and the charge
+ * of particles that leave there is dropped.  It also changes at periodic
+ * faces between steps 1 and 2, because the pre-loop push deposits without
+ * a fold (main.cpp:75-79): step 1's rho_prev is that unfolded charge.
+ * The first pic_diag call made after a current clear (since the state was
+ * last set) records R as the reference R_ref; later calls report R - R_ref,
+ * the drift.  The inner region leaves out the nodes within mem + 2 cells
+ * of an open face. */
 #define PIC_SPECTRUM_BINS 64
 typedef struct pic_diag_out {
     double field_energy;       /* 0.5 * sum_interior (E^2 + B^2) * dV          */
     double e_energy, b_energy; /* the two halves of field_energy              */
     double kinetic[PIC_MAX_SPECIES]; /* sum_p m * w * (gamma - 1) * dV          */
     double charge[PIC_MAX_SPECIES];  /* sum of rho over interior nodes         */
-    double gauss_l2;           /* || div E - rho_total - rho_background ||_2  */
-    double gauss_max;          /* max |div E - rho_total - rho_background|    */
+    double gauss_l2;           /* ||R||_2 over the interior nodes              */
+    double gauss_max;          /* max |R|                                      */
     double spectrum_emax;      /* upper edge of the spectrum, in m c^2 units  */
     double spectrum[PIC_SPECTRUM_BINS]; /* species-0 weight histogram of gamma-1 */
     int64_t out_of_bounds;     /* particles outside the domain at a push       */
+    double gauss_drift_l2;     /* ||R - R_ref||_2 over the interior nodes      */
+    double gauss_drift_max;    /* max |R - R_ref|                              */
+    double gauss_inner_drift_l2;  /* the same over the inner region            */
+    double gauss_inner_drift_max;
+    double gauss_ref_l2;       /* ||R_ref||_2                                  */
+    int32_t gauss_state;       /* 0: rho_prev unknown (R uses rho for it), no
+                                  drift; 1: this call recorded R_ref (drift 0);
+                                  2: drift against R_ref                       */
+    int32_t reserved;
 } pic_diag_out;
 
 /* Seeded initial-state generator.  Identical streams on host and device:
@@ -158,10 +189,19 @@ enum pic_option {
                                   accumulators in LDS (each add rounded to 2^-19 of
                                   one particle's largest contribution); 0 FP64
                                   accumulators in LDS (smaller tiles, slower) */
-    PIC_OPT_ADJ_PUSH = 6       /* differentiated runs: 0 (default) the LDS-tiled
+    PIC_OPT_ADJ_PUSH = 6,      /* differentiated runs: 0 (default) the LDS-tiled
                                   adjoint push (HBM kernel after a direct forward);
                                   1 dual-number kernel (FP64; FP32 takes 2);
                                   2 separable HBM kernel */
+    PIC_OPT_XMAX_BOUNDARY = 7  /* x-max Silver-Mueller write (x open):
+                                  0 (default) the reference's: column
+                                  local_xcells+1 from column local_xcells
+                                  (boundary.cpp:92-96), three columns inside
+                                  the domain at mem 4, and unstable once a
+                                  wave reaches it; 1 the mirror of the x-min
+                                  write: the last dual column n+mem-1 from
+                                  E, Bx and J at the primal column n+mem-1
+                                  and By, Bz at dual column n+mem-2 */
 };
 int pic_set_option(pic_ctx* ctx, int option, int64_t value);
 

@@ -44,6 +44,7 @@ _SIG = {
     "orc_field_get": (None, [C.c_void_p, C.c_int, _dp]),
     "orc_field_set": (None, [C.c_void_p, C.c_int, _dp]),
     "orc_diag": (None, [C.c_void_p, C.POINTER(PicDiag)]),
+    "orc_set_xmax_face": (None, [C.c_void_p, C.c_int]),
     "orc_out_of_bounds": (C.c_int64, [C.c_void_p]),
     "orc_lattice": (C.c_int64, [C.POINTER(PicParams), C.POINTER(PicLattice), C.c_int64]
                     + [_dp] * 7 + [_ip]),
@@ -227,6 +228,9 @@ class Oracle:
 
     def preloop(self):
         lib().orc_preloop(self._h)
+
+    def set_xmax_face(self, on: bool):
+        lib().orc_set_xmax_face(self._h, int(on))
 
     def radiating_boundary(self, step, is_left=1, is_right=0):
         lib().orc_radiating_boundary(self._h, step, is_left, is_right)

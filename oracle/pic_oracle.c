@@ -51,6 +51,11 @@ struct orc_ctx {
     double* tbuf; /* per-thread J + rho scratch, nthreads * 4 * jsize */
     double rho_bg;
     int64_t oob;
+    /* Gauss diagnostics: mobile charge at the last clear, recorded residual */
+    double* rho_prev;
+    double* gauss_ref;
+    int rho_prev_valid, gauss_ref_valid;
+    int xmax_face; /* 1: the x-max write at the last dual column n+m-1 */
     /* spin precession, particle.cpp:318-368 */
     int spin_on;
     double spin_mass, e_schwinger;
@@ -110,6 +115,13 @@ static inline void shape(int m, double x, double* W) {
 /* context                                                            */
 /* ------------------------------------------------------------------ */
 
+/* The state was set from outside the time loop: the Gauss residual's
+ * rho_prev and reference no longer apply. */
+static void gauss_invalidate(orc_ctx* c) {
+    c->rho_prev_valid = 0;
+    c->gauss_ref_valid = 0;
+}
+
 orc_ctx* orc_create(const pic_params* p, int nthreads) {
     if (p->mem != 2 && p->mem != 4) return NULL;
     orc_ctx* c = (orc_ctx*)calloc(1, sizeof(orc_ctx));
@@ -129,6 +141,8 @@ orc_ctx* orc_create(const pic_params* p, int nthreads) {
     for (int f = 0; f < 3; f++) c->J[f] = (double*)calloc((size_t)c->jsize, sizeof(double));
     for (int s = 0; s < PIC_MAX_SPECIES; s++)
         c->rho[s] = (double*)calloc((size_t)c->jsize, sizeof(double));
+    c->rho_prev = (double*)calloc((size_t)c->jsize, sizeof(double));
+    c->gauss_ref = (double*)calloc((size_t)c->jsize, sizeof(double));
     c->nthreads = nthreads < 1 ? 1 : nthreads;
     return c;
 }
@@ -154,11 +168,14 @@ void orc_destroy(orc_ctx* c) {
         species_free(&c->sp[s]);
     }
     free(c->tbuf);
+    free(c->rho_prev);
+    free(c->gauss_ref);
     free(c);
 }
 
 int orc_species_set(orc_ctx* c, int s, double charge, double mass, int immobile) {
     if (s < 0 || s >= PIC_MAX_SPECIES) return PIC_ERR_ARG;
+    gauss_invalidate(c);
     c->sp[s].q = charge;
     c->sp[s].m = mass;
     c->sp[s].immobile = immobile;
@@ -166,7 +183,10 @@ int orc_species_set(orc_ctx* c, int s, double charge, double mass, int immobile)
     return PIC_OK;
 }
 
-void orc_set_background(orc_ctx* c, double rho_bg) { c->rho_bg = rho_bg; }
+void orc_set_background(orc_ctx* c, double rho_bg) {
+    c->rho_bg = rho_bg;
+    gauss_invalidate(c);
+}
 
 static void species_alloc(orc_species* s, int64_t n) {
     species_free(s);
@@ -187,6 +207,7 @@ int orc_particles_set(orc_ctx* c, int s, int64_t n, const double* x, const doubl
                       const double* z, const double* ux, const double* uy, const double* uz,
                       const double* w, const int64_t* id) {
     if (s < 0 || s >= PIC_MAX_SPECIES || n < 0) return PIC_ERR_ARG;
+    gauss_invalidate(c);
     orc_species* S = &c->sp[s];
     species_alloc(S, n);
     size_t b = (size_t)n * sizeof(double);
@@ -236,6 +257,7 @@ void orc_field_get(const orc_ctx* c, int comp, double* out) {
 
 void orc_field_set(orc_ctx* c, int comp, const double* in) {
     int64_t n;
+    gauss_invalidate(c);
     double* p = comp_ptr(c, comp, &n);
     if (p) memcpy(p, in, (size_t)n * sizeof(double));
 }
@@ -363,6 +385,8 @@ double orc_gpulse(const orc_ctx* c, double time, double x, double y, double z) {
            cos(ph + P->phase);
 }
 
+void orc_set_xmax_face(orc_ctx* c, int on) { c->xmax_face = on != 0; }
+
 void orc_laser_boundary(orc_ctx* c, int step) { orc_radiating_boundary(c, step, 1, c->p.laser_right); }
 
 void orc_radiating_boundary(orc_ctx* c, int step, int is_left, int is_right) {
@@ -399,15 +423,21 @@ void orc_radiating_boundary(orc_ctx* c, int step, int is_left, int is_right) {
                 alpha * (-4 * Pl + 2 * ez[FI(c, k, j, m)] +
                          dt / dy * (bx[FI(c, k, j, m)] - bx[FI(c, k, j - 1, m)]) -
                          beta * by[FI(c, k, j, m)] + dt * jz[JI(c, k, j, m)]);
-            /* x-max: column local_xcells+1 (an interior column when m > 1) */
-            bz[FI(c, k, j, N + 1)] =
-                alpha * (-4 * Sr + 2 * ey[FI(c, k, j, N)] -
-                         dt / dz * (bx[FI(c, k, j, N)] - bx[FI(c, k - 1, j, N)]) -
-                         beta * bz[FI(c, k, j, N)] + dt * jy[JI(c, k, j, N)]);
-            by[FI(c, k, j, N + 1)] =
-                alpha * (4 * Pr - 2 * ez[FI(c, k, j, N)] +
-                         dt / dy * (bx[FI(c, k, j, N)] - bx[FI(c, k, j - 1, N)]) -
-                         beta * by[FI(c, k, j, N)] - dt * jz[JI(c, k, j, N)]);
+            /* x-max: column local_xcells+1, reading column local_xcells
+             * (boundary.cpp:92-96; an interior column when m > 2).  With
+             * xmax_face, the mirror of the x-min write: the last dual column
+             * w = n+m-1 from E, Bx and J at the primal column w next to it
+             * and By, Bz one dual column in. */
+            const int w = c->xmax_face ? N + m - 1 : N + 1;
+            const int re = c->xmax_face ? w : w - 1, rb = w - 1;
+            bz[FI(c, k, j, w)] =
+                alpha * (-4 * Sr + 2 * ey[FI(c, k, j, re)] -
+                         dt / dz * (bx[FI(c, k, j, re)] - bx[FI(c, k - 1, j, re)]) -
+                         beta * bz[FI(c, k, j, rb)] + dt * jy[JI(c, k, j, re)]);
+            by[FI(c, k, j, w)] =
+                alpha * (4 * Pr - 2 * ez[FI(c, k, j, re)] +
+                         dt / dy * (bx[FI(c, k, j, re)] - bx[FI(c, k, j - 1, re)]) -
+                         beta * by[FI(c, k, j, rb)] - dt * jz[JI(c, k, j, re)]);
         }
 }
 
@@ -416,6 +446,14 @@ void orc_radiating_boundary(orc_ctx* c, int step, int is_left, int is_right) {
 /* ------------------------------------------------------------------ */
 
 void orc_clear_currents(orc_ctx* c) {
+    /* keep the mobile charge (species order) for the Gauss diagnostics */
+    for (int64_t q = 0; q < c->jsize; q++) {
+        double sum = 0.0;
+        for (int s = 0; s < PIC_MAX_SPECIES; s++)
+            if (c->sp[s].declared && !c->sp[s].immobile) sum += c->rho[s][q];
+        c->rho_prev[q] = sum;
+    }
+    c->rho_prev_valid = 1;
     for (int f = 0; f < 3; f++) memset(c->J[f], 0, (size_t)c->jsize * sizeof(double));
     for (int s = 0; s < PIC_MAX_SPECIES; s++) memset(c->rho[s], 0, (size_t)c->jsize * sizeof(double));
 }
@@ -993,33 +1031,65 @@ void orc_step(orc_ctx* c, int first_step, int nsteps) {
 /* diagnostics (new: the reference has none)                           */
 /* ------------------------------------------------------------------ */
 
-void orc_diag(const orc_ctx* c, pic_diag_out* o) {
+void orc_diag(orc_ctx* c, pic_diag_out* o) {
     const int m = c->m;
     const double dV = c->p.dx * c->p.dy * c->p.dz;
     double emax = o->spectrum_emax > 0.0 ? o->spectrum_emax : 1.0;
     memset(o, 0, sizeof(*o));
     o->spectrum_emax = emax;
+    /* Gauss residual R = div E - (rho_prev + rho)/2 - rho_bg (time-centred:
+     * E's half steps straddle the push, simulation.cpp:166-182); mode 0: no
+     * rho_prev yet, 1: record R as the drift reference, 2: drift R - R_ref */
+    const int mode = !c->rho_prev_valid ? 0 : (c->gauss_ref_valid ? 2 : 1);
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        const int margin = c->p.periodic[a] ? 0 : m + 2; /* inner region */
+        lo[a] = m + margin;
+        hi[a] = m + c->n[a] - 1 - margin;
+    }
     double ee = 0.0, be = 0.0, g2 = 0.0, gmax = 0.0;
+    double d2 = 0.0, dmax = 0.0, i2 = 0.0, imax = 0.0, ref2 = 0.0;
     double chg[PIC_MAX_SPECIES] = {0};
     for (int k = m; k < m + c->n[2]; k++)
         for (int j = m; j < m + c->n[1]; j++)
             for (int i = m; i < m + c->n[0]; i++) {
-                int64_t f = FI(c, k, j, i);
+                int64_t f = FI(c, k, j, i), q = JI(c, k, j, i);
                 ee += c->F[EX][f] * c->F[EX][f] + c->F[EY][f] * c->F[EY][f] + c->F[EZ][f] * c->F[EZ][f];
                 be += c->F[BX][f] * c->F[BX][f] + c->F[BY][f] * c->F[BY][f] + c->F[BZ][f] * c->F[BZ][f];
                 double div = (c->F[EX][f] - c->F[EX][FI(c, k, j, i - 1)]) / c->p.dx +
                              (c->F[EY][f] - c->F[EY][FI(c, k, j - 1, i)]) / c->p.dy +
                              (c->F[EZ][f] - c->F[EZ][FI(c, k - 1, j, i)]) / c->p.dz;
-                double rho = c->rho_bg;
+                double now = 0.0;
                 for (int s = 0; s < PIC_MAX_SPECIES; s++) {
-                    double r = c->rho[s][JI(c, k, j, i)];
+                    double r = c->rho[s][q];
                     chg[s] += r;
-                    if (c->sp[s].declared && !c->sp[s].immobile) rho += r;
+                    if (c->sp[s].declared && !c->sp[s].immobile) now += r;
                 }
-                double res = div - rho;
+                double prev = mode ? c->rho_prev[q] : now;
+                double res = div - 0.5 * (prev + now) - c->rho_bg;
                 g2 += res * res;
                 if (fabs(res) > gmax) gmax = fabs(res);
+                if (mode == 1) {
+                    c->gauss_ref[q] = res;
+                    ref2 += res * res;
+                } else if (mode == 2) {
+                    double dr = res - c->gauss_ref[q];
+                    d2 += dr * dr;
+                    if (fabs(dr) > dmax) dmax = fabs(dr);
+                    if (i >= lo[0] && i <= hi[0] && j >= lo[1] && j <= hi[1] && k >= lo[2] && k <= hi[2]) {
+                        i2 += dr * dr;
+                        if (fabs(dr) > imax) imax = fabs(dr);
+                    }
+                    ref2 += c->gauss_ref[q] * c->gauss_ref[q];
+                }
             }
+    if (mode == 1) c->gauss_ref_valid = 1;
+    o->gauss_drift_l2 = sqrt(d2);
+    o->gauss_drift_max = dmax;
+    o->gauss_inner_drift_l2 = sqrt(i2);
+    o->gauss_inner_drift_max = imax;
+    o->gauss_ref_l2 = sqrt(ref2);
+    o->gauss_state = mode;
     o->e_energy = 0.5 * ee * dV;
     o->b_energy = 0.5 * be * dV;
     o->field_energy = o->e_energy + o->b_energy;

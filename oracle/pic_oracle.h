@@ -44,6 +44,10 @@ void orc_fold_currents(orc_ctx* c);             /* mpi_utils.cpp:232-417   */
 void orc_field_wrap(orc_ctx* c, int comp);      /* mpi_utils.cpp:9-60      */
 void orc_preloop(orc_ctx* c);                   /* main.cpp:75-79          */
 void orc_radiating_boundary(orc_ctx* c, int step, int is_left, int is_right); /* boundary.cpp:42-99 */
+/* 1: place the x-max Silver-Mueller write at the last dual column n+m-1
+ * (PIC_OPT_XMAX_BOUNDARY); 0 (default): column local_xcells+1 as the
+ * reference (boundary.cpp:92-96). */
+void orc_set_xmax_face(orc_ctx* c, int on);
 void orc_fields_full(orc_ctx* c);               /* field.cpp:24-54         */
 void orc_fold_one(orc_ctx* c, int comp);        /* mpi_utils.cpp:232-417, one array */
 void orc_filter_currents(orc_ctx* c);           /* particle.cpp:520-574    */
@@ -54,7 +58,9 @@ void orc_step(orc_ctx* c, int first_step, int nsteps); /* simulation.cpp:159-183
 
 void orc_field_get(const orc_ctx* c, int comp, double* out);
 void orc_field_set(orc_ctx* c, int comp, const double* in);
-void orc_diag(const orc_ctx* c, pic_diag_out* out);
+/* FP64 diagnostics (pic_diag_out).  The first call after a current clear
+ * since the state was last set records the Gauss residual's reference. */
+void orc_diag(orc_ctx* c, pic_diag_out* out);
 int64_t orc_out_of_bounds(const orc_ctx* c);
 
 /* Individual shape functions (particle.cpp:8-48), for closed-form tests. */

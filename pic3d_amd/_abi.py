@@ -21,6 +21,7 @@ PIC_PROFILE_UNIFORM, PIC_PROFILE_SLAB, PIC_PROFILE_RAMP_TARGET = 0, 1, 2
 PIC_OPT_PUSH_KERNEL, PIC_OPT_KEEP_OLD, PIC_OPT_SORT_INTERVAL, PIC_OPT_TIMING = 1, 2, 3, 4
 PIC_OPT_FP32_ACCUM = 5
 PIC_OPT_ADJ_PUSH = 6
+PIC_OPT_XMAX_BOUNDARY = 7
 TIMING_CATS = ["fields", "clear", "push", "sort", "boundary", "fold"]
 
 
@@ -60,6 +61,9 @@ class PicDiag(C.Structure):
         ("gauss_l2", C.c_double), ("gauss_max", C.c_double),
         ("spectrum_emax", C.c_double), ("spectrum", C.c_double * PIC_SPECTRUM_BINS),
         ("out_of_bounds", C.c_int64),
+        ("gauss_drift_l2", C.c_double), ("gauss_drift_max", C.c_double),
+        ("gauss_inner_drift_l2", C.c_double), ("gauss_inner_drift_max", C.c_double),
+        ("gauss_ref_l2", C.c_double), ("gauss_state", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -69,6 +73,10 @@ class PicDiag(C.Structure):
             "charge": list(self.charge), "gauss_l2": self.gauss_l2,
             "gauss_max": self.gauss_max, "spectrum_emax": self.spectrum_emax,
             "spectrum": list(self.spectrum), "out_of_bounds": int(self.out_of_bounds),
+            "gauss_drift_l2": self.gauss_drift_l2, "gauss_drift_max": self.gauss_drift_max,
+            "gauss_inner_drift_l2": self.gauss_inner_drift_l2,
+            "gauss_inner_drift_max": self.gauss_inner_drift_max,
+            "gauss_ref_l2": self.gauss_ref_l2, "gauss_state": int(self.gauss_state),
         }
 
 

@@ -1120,22 +1120,22 @@ __global__ __launch_bounds__(256) void k_adj_laser(T* __restrict__ lEy, T* __res
                 if (L.pol == 2 || L.pol == 3) ge += -4.0 * alpha * b * gr;
             }
         }
-        // right: column N+1 from column N (the sources there do not depend
-        // on the x-min envelope)
+        // right: column L.xw from columns L.xre (E, Bx, J) and L.xrb (By,
+        // Bz) (the sources there do not depend on the x-min envelope)
         {
-            const int64_t o = g.fi(k, j, N), gr = g.fi(k, j, N + 1);
+            const int64_t o = g.fi(k, j, L.xre), ob = g.fi(k, j, L.xrb), gr = g.fi(k, j, L.xw);
             const double a = lBz[gr], b = lBy[gr];
             lBz[gr] = T(0);
             lBy[gr] = T(0);
             hbm_add(&lEy[o], (T)(2.0 * alpha * a));
             hbm_add(&lBx[o], (T)(-alpha * dtdz * a + alpha * dtdy * b));
-            hbm_add(&lBx[g.fi(k - 1, j, N)], (T)(alpha * dtdz * a));
-            hbm_add(&lBx[g.fi(k, j - 1, N)], (T)(-alpha * dtdy * b));
-            hbm_add(&lBz[o], (T)(-alpha * beta * a));
-            hbm_add(&lBy[o], (T)(-alpha * beta * b));
+            hbm_add(&lBx[g.fi(k - 1, j, L.xre)], (T)(alpha * dtdz * a));
+            hbm_add(&lBx[g.fi(k, j - 1, L.xre)], (T)(-alpha * dtdy * b));
+            hbm_add(&lBz[ob], (T)(-alpha * beta * a));
+            hbm_add(&lBy[ob], (T)(-alpha * beta * b));
             hbm_add(&lEz[o], (T)(-2.0 * alpha * b));
-            hbm_add(&lJy[g.ji(k, j, N)], (T)(alpha * dt * a));
-            hbm_add(&lJz[g.ji(k, j, N)], (T)(-alpha * dt * b));
+            hbm_add(&lJy[g.ji(k, j, L.xre)], (T)(alpha * dt * a));
+            hbm_add(&lJz[g.ji(k, j, L.xre)], (T)(-alpha * dt * b));
         }
     }
     // block sum of ge
@@ -1745,6 +1745,7 @@ using namespace pic;
 extern "C" {
 
 int pic_diff_begin(pic_ctx* c, int s) {
+    if (c) gauss_invalidate(c);
     if (!c || s < 0 || s >= PIC_MAX_SPECIES) return PIC_ERR_ARG;
     Species& S = c->sp[s];
     if (!S.declared || S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is not a mobile species", s);
@@ -1848,6 +1849,7 @@ int pic_diff_initial(pic_ctx* c, double* x, double* y, double* z, double* w) {
 
 int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double* envelope,
                      const double* weights, double* objective) {
+    if (c) gauss_invalidate(c);
     if (!c || nsteps < 0 || K < 1 || !objective) return PIC_ERR_ARG;
     if (!c->diff) return set_err(c, PIC_ERR_STATE, "no differentiated run (pic_diff_begin)");
     DiffState& D = *c->diff;
@@ -1932,6 +1934,7 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
 }
 
 int pic_diff_backward(pic_ctx* c, double grad_objective, double* grad_envelope, double* grad_weights) {
+    if (c) gauss_invalidate(c);
     if (!c) return PIC_ERR_ARG;
     if (!c->diff || !c->diff->forward_done)
         return set_err(c, PIC_ERR_STATE, "pic_diff_backward needs a pic_diff_forward first");

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void k_laser(T* __restrict__ ey, T* __restrict
     const int j = g.m + blockIdx.x * blockDim.x + threadIdx.x;  // y
     const int k = g.m + blockIdx.y;                              // z
     if (j >= g.m + g.n[1]) return;
-    const int m = g.m, N = g.n[0];
+    const int m = g.m;
     const double dt = L.dt;
     const double alpha = 1 / (1 + dt / L.dx);
     const double beta = 1 - dt / L.dx;
@@ -151,17 +151,18 @@ __global__ __launch_bounds__(256) void k_laser(T* __restrict__ ey, T* __restrict
         by[g.fi(k, j, m - 1)] = (T)byv;
     }
     {
-        // x-max: column local_xcells + 1 (boundary.cpp:92-96; an interior
-        // column because of the ghost offset)
-        const int64_t o = g.fi(k, j, N);
+        // x-max: column local_xcells + 1 from column local_xcells
+        // (boundary.cpp:92-96; an interior column when m > 2), or the face
+        // form (L.xw, L.xre, L.xrb)
+        const int64_t o = g.fi(k, j, L.xre), ob = g.fi(k, j, L.xrb);
         const double bzv = alpha * (-4 * Sr + 2 * (double)ey[o] -
-                                    dtdz * ((double)bx[o] - (double)bx[g.fi(k - 1, j, N)]) -
-                                    beta * (double)bz[o] + dt * (double)jy[g.ji(k, j, N)]);
+                                    dtdz * ((double)bx[o] - (double)bx[g.fi(k - 1, j, L.xre)]) -
+                                    beta * (double)bz[ob] + dt * (double)jy[g.ji(k, j, L.xre)]);
         const double byv = alpha * (4 * Pr - 2 * (double)ez[o] +
-                                    dtdy * ((double)bx[o] - (double)bx[g.fi(k, j - 1, N)]) -
-                                    beta * (double)by[o] - dt * (double)jz[g.ji(k, j, N)]);
-        bz[g.fi(k, j, N + 1)] = (T)bzv;
-        by[g.fi(k, j, N + 1)] = (T)byv;
+                                    dtdy * ((double)bx[o] - (double)bx[g.fi(k, j - 1, L.xre)]) -
+                                    beta * (double)by[ob] - dt * (double)jz[g.ji(k, j, L.xre)]);
+        bz[g.fi(k, j, L.xw)] = (T)bzv;
+        by[g.fi(k, j, L.xw)] = (T)byv;
     }
 }
 
@@ -230,17 +231,44 @@ int launch_fill(pic_ctx* c, void* p, int64_t bytes) {
     return PIC_OK;
 }
 
+// Charge clear that keeps the previous charge for the Gauss diagnostics:
+// prev = the sum of the mobile species' rho (species order), then every
+// rho array is zeroed.  Grid-stride over the whole J layout.
+struct RhoPtrs { void* a[PIC_MAX_SPECIES]; };
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_clear_rho(RhoPtrs R, int mobile, double* __restrict__ prev,
+                                                   int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+        double sum = 0.0;
+#pragma unroll
+        for (int s = 0; s < PIC_MAX_SPECIES; s++) {
+            T* a = reinterpret_cast<T*>(R.a[s]);
+            if (!a) continue;
+            if (mobile & (1 << s)) sum += (double)a[t];
+            a[t] = (T)0;
+        }
+        prev[t] = sum;
+    }
+}
+
+// clear_currents (particle.cpp:576-592): J and every rho array to zero,
+// keeping the mobile charge in rho_prev first.
 int launch_clear(pic_ctx* c) {
     const int64_t bytes = c->g.jsize * (c->fp32 ? 4 : 8);
     for (int f = 0; f < 3; f++) {
         int r = launch_fill(c, c->J[f], bytes);
         if (r) return r;
     }
-    for (int s = 0; s < PIC_MAX_SPECIES; s++)
-        if (c->rho[s]) {
-            int r = launch_fill(c, c->rho[s], bytes);
-            if (r) return r;
-        }
+    RhoPtrs R{};
+    for (int s = 0; s < PIC_MAX_SPECIES; s++) R.a[s] = c->rho[s];
+    const int64_t n = c->g.jsize;
+    const int blocks = (int)std::min<int64_t>(blocks_for(n, 256), 8192);
+    if (c->fp32) k_clear_rho<float><<<blocks, 256, 0, c->stream>>>(R, mobile_mask(c), c->rho_prev, n);
+    else k_clear_rho<double><<<blocks, 256, 0, c->stream>>>(R, mobile_mask(c), c->rho_prev, n);
+    PIC_LAUNCH_CHECK(c);
+    c->rho_prev_valid = 1;
     return PIC_OK;
 }
 
@@ -299,6 +327,9 @@ LaserArgs laser_args(const pic_ctx* c, int step, int left, int right) {
     L.dt = P.dt; L.dx = P.dx; L.dy = P.dy; L.dz = P.dz; L.time = step * P.dt;
     L.pol = P.laser_polarization; L.left = left; L.right = right;
     L.env_on = c->env_on; L.env_val = c->env_val;
+    const int N = c->g.n[0], m = c->g.m;
+    if (c->xmax_face) { L.xw = N + m - 1; L.xre = N + m - 1; L.xrb = N + m - 2; }
+    else { L.xw = N + 1; L.xre = N; L.xrb = N; }
     return L;
 }
 

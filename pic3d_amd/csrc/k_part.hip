@@ -426,61 +426,84 @@ int launch_loader(pic_ctx* c, int s, const pic_loader* l, int64_t n, const int64
 }
 
 // ---------------------------------------------------------------------
-// diagnostics (FP64 reductions)
-//   buf[0] sum E^2, [1] sum B^2, [2] sum res^2, [3] max |res| (bits),
-//   [4..7] charge per species, [8..11] kinetic per species,
-//   [12..12+bins) species-0 spectrum
+// diagnostics (FP64 reductions into c->diag_buf, layout DG_*)
 // ---------------------------------------------------------------------
+struct GaussArgs {
+    const double* prev;  // rho at the last clear (null: use rho now)
+    double* ref;         // recorded residual (null when mode == 0)
+    int mode;            // 0 no drift, 1 record the reference, 2 drift against it
+    int lo[3], hi[3];    // inner region, buffer indices (inclusive)
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag_fields(const T* __restrict__ ex, const T* __restrict__ ey,
                                                      const T* __restrict__ ez, const T* __restrict__ bx,
                                                      const T* __restrict__ by, const T* __restrict__ bz,
                                                      const T* r0, const T* r1, const T* r2, const T* r3,
-                                                     int mobile_mask, double rho_bg, Grid g, double dx,
-                                                     double dy, double dz, double* __restrict__ buf) {
+                                                     int mobile_mask, double rho_bg, GaussArgs G, Grid g,
+                                                     double dx, double dy, double dz, double* __restrict__ buf) {
+    constexpr int NV = 13;  // E2 B2 R2 Rmax chg0..3 D2 Dmax I2 Imax Ref2
     const int i = g.m + blockIdx.x * blockDim.x + threadIdx.x;
     const int j = g.m + blockIdx.y, k = g.m + blockIdx.z;
-    double ee = 0, be = 0, r2s = 0, rmax = 0, ch[4] = {0, 0, 0, 0};
+    double v[NV] = {};
     if (i < g.m + g.n[0]) {
         const int64_t o = g.fi(k, j, i);
         const double a = ex[o], b = ey[o], c = ez[o], d = bx[o], e = by[o], f = bz[o];
-        ee = a * a + b * b + c * c;
-        be = d * d + e * e + f * f;
+        v[0] = a * a + b * b + c * c;
+        v[1] = d * d + e * e + f * f;
         const double div = (a - (double)ex[o - 1]) / dx + (b - (double)ey[o - g.fe[0]]) / dy +
                            (c - (double)ez[o - (int64_t)g.fe[0] * g.fe[1]]) / dz;
         const int64_t q = g.ji(k, j, i);
         const T* rr[4] = {r0, r1, r2, r3};
-        double rho = rho_bg;
+        double now = 0.0;
 #pragma unroll
         for (int s = 0; s < 4; s++)
             if (rr[s]) {
-                ch[s] = rr[s][q];
-                if (mobile_mask & (1 << s)) rho += ch[s];
+                const double x = rr[s][q];
+                v[4 + s] = x;
+                if (mobile_mask & (1 << s)) now += x;
             }
-        const double res = div - rho;
-        r2s = res * res;
-        rmax = fabs(res);
+        const double prev = G.prev ? G.prev[q] : now;
+        const double res = div - 0.5 * (prev + now) - rho_bg;
+        v[2] = res * res;
+        v[3] = fabs(res);
+        if (G.mode == 1) {
+            G.ref[q] = res;
+            v[12] = res * res;
+        } else if (G.mode == 2) {
+            const double rf = G.ref[q];
+            const double dr = res - rf;
+            v[8] = dr * dr;
+            v[9] = fabs(dr);
+            if (i >= G.lo[0] && i <= G.hi[0] && j >= G.lo[1] && j <= G.hi[1] && k >= G.lo[2] &&
+                k <= G.hi[2]) {
+                v[10] = dr * dr;
+                v[11] = fabs(dr);
+            }
+            v[12] = rf * rf;
+        }
     }
-    // block reduction
-    __shared__ double red[8][256 / 64];
-    double vals[8] = {ee, be, r2s, rmax, ch[0], ch[1], ch[2], ch[3]};
+    // block reduction (sums, or maxima of non-negative values)
+    __shared__ double red[NV][256 / 64];
+    auto is_max = [](int t) { return t == 3 || t == 9 || t == 11; };
 #pragma unroll
-    for (int v = 0; v < 8; v++) {
-        double x = vals[v];
+    for (int t = 0; t < NV; t++) {
+        double x = v[t];
         for (int off = 32; off > 0; off >>= 1) {
             const double y = __shfl_down(x, off);
-            x = (v == 3) ? fmax(x, y) : x + y;
+            x = is_max(t) ? fmax(x, y) : x + y;
         }
-        if (lane_id() == 0) red[v][threadIdx.x / 64] = x;
+        if (lane_id() == 0) red[t][threadIdx.x / 64] = x;
     }
     __syncthreads();
-    if (threadIdx.x < 8) {
-        const int v = threadIdx.x;
-        double x = red[v][0];
-        for (int w = 1; w < (int)(blockDim.x / 64); w++) x = (v == 3) ? fmax(x, red[v][w]) : x + red[v][w];
-        if (v == 3) atomicMax((unsigned long long*)&buf[3], (unsigned long long)__double_as_longlong(x));
-        else if (v < 3) hbm_add(&buf[v], x);
-        else hbm_add(&buf[4 + (v - 4)], x);
+    if (threadIdx.x < NV) {
+        const int t = threadIdx.x;
+        double x = red[t][0];
+        for (int w = 1; w < (int)(blockDim.x / 64); w++) x = is_max(t) ? fmax(x, red[t][w]) : x + red[t][w];
+        const int dst[NV] = {DG_E2, DG_B2, DG_R2, DG_RMAX, DG_CHG, DG_CHG + 1, DG_CHG + 2,
+                                        DG_CHG + 3, DG_D2, DG_DMAX, DG_I2, DG_IMAX, DG_REF2};
+        if (is_max(t)) atomicMax((unsigned long long*)&buf[dst[t]], (unsigned long long)__double_as_longlong(x));
+        else hbm_add(&buf[dst[t]], x);
     }
 }
 
@@ -510,22 +533,28 @@ __global__ __launch_bounds__(256) void k_diag_p64(const double* __restrict__ ux,
     for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off);
     if (lane_id() == 0) atomicAdd(&ke_s, ke);
     __syncthreads();
-    if (threadIdx.x == 0) hbm_add(&buf[8 + s], ke_s);
+    if (threadIdx.x == 0) hbm_add(&buf[DG_KIN + s], ke_s);
     if (s == 0)
         for (int b = threadIdx.x; b < PIC_SPECTRUM_BINS; b += blockDim.x)
-            if (hist[b] != 0) hbm_add(&buf[12 + b], hist[b]);
+            if (hist[b] != 0) hbm_add(&buf[DG_SPEC + b], hist[b]);
 }
 
 int diag_p32(pic_ctx* c, int s, double emax);  // k_push32.hip
 
 int launch_diag(pic_ctx* c, pic_diag_out* out) {
-    const int nbuf = 12 + PIC_SPECTRUM_BINS;
-    int r = launch_fill(c, c->diag_buf, nbuf * 8);
+    int r = launch_fill(c, c->diag_buf, DG_N * 8);
     if (r) return r;
     const double emax = out->spectrum_emax > 0 ? out->spectrum_emax : 1.0;
-    int mobile = 0;
-    for (int s = 0; s < PIC_MAX_SPECIES; s++)
-        if (c->sp[s].declared && !c->sp[s].immobile) mobile |= 1 << s;
+    const int mobile = mobile_mask(c);
+    GaussArgs G{};
+    G.prev = c->rho_prev_valid ? c->rho_prev : nullptr;
+    G.mode = !c->rho_prev_valid ? 0 : (c->gauss_ref_valid ? 2 : 1);
+    G.ref = G.mode ? c->gauss_ref : nullptr;
+    const int margin = c->g.m + 2;  // cells left out next to an open face
+    for (int a = 0; a < 3; a++) {
+        G.lo[a] = c->g.m + (c->p.periodic[a] ? 0 : margin);
+        G.hi[a] = c->g.m + c->g.n[a] - 1 - (c->p.periodic[a] ? 0 : margin);
+    }
     dim3 grid(blocks_for(c->g.n[0], 256), c->g.n[1], c->g.n[2]);
     if (c->fp32) {
         const float* rr[4];
@@ -533,14 +562,14 @@ int launch_diag(pic_ctx* c, pic_diag_out* out) {
         k_diag_fields<float><<<grid, 256, 0, c->stream>>>(
             (const float*)c->F[0], (const float*)c->F[1], (const float*)c->F[2], (const float*)c->F[3],
             (const float*)c->F[4], (const float*)c->F[5], rr[0], rr[1], rr[2], rr[3], mobile, c->rho_bg,
-            c->g, c->p.dx, c->p.dy, c->p.dz, c->diag_buf);
+            G, c->g, c->p.dx, c->p.dy, c->p.dz, c->diag_buf);
     } else {
         const double* rr[4];
         for (int s = 0; s < 4; s++) rr[s] = c->sp[s].declared ? (const double*)c->rho[s] : nullptr;
         k_diag_fields<double><<<grid, 256, 0, c->stream>>>(
             (const double*)c->F[0], (const double*)c->F[1], (const double*)c->F[2], (const double*)c->F[3],
             (const double*)c->F[4], (const double*)c->F[5], rr[0], rr[1], rr[2], rr[3], mobile, c->rho_bg,
-            c->g, c->p.dx, c->p.dy, c->p.dz, c->diag_buf);
+            G, c->g, c->p.dx, c->p.dy, c->p.dz, c->diag_buf);
     }
     PIC_LAUNCH_CHECK(c);
     for (int s = 0; s < PIC_MAX_SPECIES; s++) {
@@ -556,26 +585,36 @@ int launch_diag(pic_ctx* c, pic_diag_out* out) {
                                                   S.d_cnt, S.m, s, emax, c->diag_buf);
         PIC_LAUNCH_CHECK(c);
     }
-    double h[12 + PIC_SPECTRUM_BINS];
+    double h[DG_N];
     PIC_HIP(c, hipMemcpyAsync(h, c->diag_buf, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     PIC_HIP(c, hipStreamSynchronize(c->stream));
+    if (G.mode == 1) c->gauss_ref_valid = 1;
     const double dV = c->p.dx * c->p.dy * c->p.dz;
     out->spectrum_emax = emax;
-    out->e_energy = 0.5 * h[0] * dV;
-    out->b_energy = 0.5 * h[1] * dV;
+    out->e_energy = 0.5 * h[DG_E2] * dV;
+    out->b_energy = 0.5 * h[DG_B2] * dV;
     out->field_energy = out->e_energy + out->b_energy;
-    out->gauss_l2 = sqrt(h[2]);
-    out->gauss_max = h[3];  // max of non-negative doubles, reduced on their bit patterns
+    // maxima of non-negative doubles are reduced on their bit patterns
+    out->gauss_l2 = sqrt(h[DG_R2]);
+    out->gauss_max = h[DG_RMAX];
+    out->gauss_drift_l2 = sqrt(h[DG_D2]);
+    out->gauss_drift_max = h[DG_DMAX];
+    out->gauss_inner_drift_l2 = sqrt(h[DG_I2]);
+    out->gauss_inner_drift_max = h[DG_IMAX];
+    out->gauss_ref_l2 = sqrt(h[DG_REF2]);
+    out->gauss_state = G.mode;
+    out->reserved = 0;
     for (int s = 0; s < PIC_MAX_SPECIES; s++) {
-        out->charge[s] = h[4 + s];
-        out->kinetic[s] = h[8 + s] * dV;
+        out->charge[s] = h[DG_CHG + s];
+        out->kinetic[s] = h[DG_KIN + s] * dV;
     }
-    for (int b = 0; b < PIC_SPECTRUM_BINS; b++) out->spectrum[b] = h[12 + b];
+    for (int b = 0; b < PIC_SPECTRUM_BINS; b++) out->spectrum[b] = h[DG_SPEC + b];
     int64_t oob = 0;
     for (int s = 0; s < PIC_MAX_SPECIES; s++)
         if (c->sp[s].declared && c->sp[s].d_cnt) {
             int64_t v = 0;
-            PIC_HIP(c, hipMemcpy(&v, c->sp[s].d_cnt + CNT_OOB, 8, hipMemcpyDeviceToHost));
+            PIC_HIP(c, hipMemcpyAsync(&v, c->sp[s].d_cnt + CNT_OOB, 8, hipMemcpyDeviceToHost, c->stream));
+            PIC_HIP(c, hipStreamSynchronize(c->stream));
             oob += v;
         }
     out->out_of_bounds = oob;

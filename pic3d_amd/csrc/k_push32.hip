@@ -846,10 +846,10 @@ __global__ __launch_bounds__(256) void k_diag_p32(Soa32 A, const int64_t* __rest
     for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off);
     if (lane_id() == 0) atomicAdd(&ke_s, ke);
     __syncthreads();
-    if (threadIdx.x == 0) hbm_add(&buf[8 + s], ke_s);
+    if (threadIdx.x == 0) hbm_add(&buf[DG_KIN + s], ke_s);
     if (s == 0)
         for (int b = threadIdx.x; b < PIC_SPECTRUM_BINS; b += blockDim.x)
-            if (hist[b] != 0) hbm_add(&buf[12 + b], hist[b]);
+            if (hist[b] != 0) hbm_add(&buf[DG_SPEC + b], hist[b]);
 }
 
 int diag_p32(pic_ctx* c, int s, double emax) {

@@ -16,6 +16,10 @@ struct LaserArgs {
     // generate_laser_pulse(time, 0) (the differentiated envelope)
     int env_on;
     double env_val;
+    // x-max Silver-Mueller columns: written (xw), E/Bx/J read (xre), By/Bz
+    // read (xrb).  The reference: local_xcells+1 from local_xcells
+    // (boundary.cpp:92-96); the face form: n+m-1 from n+m-1 and n+m-2.
+    int xw, xre, xrb;
 };
 
 // generate_laser_pulse, boundary.cpp:8-27 (the sin envelope, in amplitude)

@@ -376,7 +376,9 @@ int pic_ctx_create(const pic_params* p, int device, pic_ctx** out) {
     }
     if (!r) r = alloc_sort_buffers(c);
     if (!r) r = dmalloc(c, &c->nwork, 8);
-    if (!r) r = dmalloc(c, &c->diag_buf, 12 + PIC_SPECTRUM_BINS + 8);
+    if (!r) r = dmalloc(c, &c->diag_buf, DG_N);
+    if (!r) r = dmalloc(c, &c->rho_prev, g.jsize);
+    if (!r) r = dmalloc(c, &c->gauss_ref, g.jsize);
     if (r) return fail(r);
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         set_err(c, PIC_ERR_HIP, "initial fill failed");
@@ -404,6 +406,7 @@ int pic_ctx_destroy(pic_ctx* c) {
     for (int f = 0; f < 3; f++) if (c->J[f]) hipFree(c->J[f]);
     dfree(c->tile_cnt); dfree(c->tile_cur); dfree(c->scan_sums);
     dfree(c->work); dfree(c->nwork); dfree(c->diag_buf);
+    dfree(c->rho_prev); dfree(c->gauss_ref);
     dfree(c->jtmp);
     dfree(c->stage);
     if (c->spin.g) hipFree((void*)c->spin.g);
@@ -481,6 +484,10 @@ int pic_set_option(pic_ctx* c, int option, int64_t value) {
             }
             return PIC_OK;
         }
+        case PIC_OPT_XMAX_BOUNDARY:
+            if (value != 0 && value != 1) return set_err(c, PIC_ERR_ARG, "x-max boundary must be 0 or 1");
+            c->xmax_face = (int)value;
+            return PIC_OK;
         case PIC_OPT_ADJ_PUSH:
             if (value < 0 || value > 2) return set_err(c, PIC_ERR_ARG, "adjoint push kernel must be 0, 1 or 2");
             c->adj_push = (int)value;
@@ -496,6 +503,7 @@ int pic_set_option(pic_ctx* c, int option, int64_t value) {
 
 int pic_species_set(pic_ctx* c, int s, double charge, double mass, int immobile, int64_t capacity) {
     if (!c || s < 0 || s >= PIC_MAX_SPECIES) return PIC_ERR_ARG;
+    gauss_invalidate(c);
     if (!(mass > 0)) return set_err(c, PIC_ERR_ARG, "mass must be > 0");
     if (capacity < 0) return set_err(c, PIC_ERR_ARG, "negative capacity");
     PIC_HIP(c, hipSetDevice(c->device));
@@ -561,6 +569,7 @@ int pic_species_set(pic_ctx* c, int s, double charge, double mass, int immobile,
 int pic_set_background(pic_ctx* c, double rho_bg) {
     if (!c) return PIC_ERR_ARG;
     c->rho_bg = rho_bg;
+    gauss_invalidate(c);
     return PIC_OK;
 }
 
@@ -568,6 +577,7 @@ int pic_particles_set(pic_ctx* c, int s, int64_t n, const double* x, const doubl
                       const double* z, const double* ux, const double* uy, const double* uz,
                       const double* w, const int64_t* id) {
     if (!valid_species(c, s) || n < 0) return PIC_ERR_ARG;
+    gauss_invalidate(c);
     Species& S = c->sp[s];
     if (S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is immobile (no particle storage)", s);
     if (n > S.cap) return set_err(c, PIC_ERR_CAPACITY, "%lld particles exceed capacity %lld",
@@ -653,6 +663,7 @@ int pic_particles_get(pic_ctx* c, int s, int64_t* n, double* x, double* y, doubl
 
 int pic_particles_load(pic_ctx* c, int s, const pic_loader* l) {
     if (!valid_species(c, s) || !l || l->ppc <= 0) return PIC_ERR_ARG;
+    gauss_invalidate(c);
     Species& S = c->sp[s];
     if (S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is immobile", s);
     const int64_t n = pic_loader_count(&c->p, l);
@@ -680,6 +691,7 @@ int pic_particles_load(pic_ctx* c, int s, const pic_loader* l) {
 
 int pic_particles_lattice(pic_ctx* c, int s, const pic_lattice* l) {
     if (!valid_species(c, s) || !l || l->cell_per_particles < 1) return PIC_ERR_ARG;
+    gauss_invalidate(c);
     Species& S = c->sp[s];
     if (S.immobile) return set_err(c, PIC_ERR_STATE, "species %d is immobile", s);
     PIC_HIP(c, hipSetDevice(c->device));
@@ -1054,6 +1066,7 @@ int pic_field_get(pic_ctx* c, int comp, double* host) {
 
 int pic_field_set(pic_ctx* c, int comp, const double* host) {
     if (!c || !host) return PIC_ERR_ARG;
+    gauss_invalidate(c);
     int64_t n;
     void* p = comp_ptr(c, comp, &n);
     if (!p) return set_err(c, PIC_ERR_ARG, "component %d is not allocated", comp);

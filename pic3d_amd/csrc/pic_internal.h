@@ -110,8 +110,16 @@ struct pic_ctx {
     int64_t* nwork = nullptr;     // number of work items (device)
     int sort_interval = 1;        // re-sort the tiled species every N pushes
     double* diag_buf = nullptr;   // diagnostics reduction buffer
+    // Gauss-law diagnostics: the mobile charge as it stood at the last
+    // current clear (FP64, J layout), and the residual recorded as the
+    // drift reference (FP64, J layout); see pic_diag_out
+    double* rho_prev = nullptr;
+    double* gauss_ref = nullptr;
+    int rho_prev_valid = 0;       // a clear happened since the state was last set
+    int gauss_ref_valid = 0;      // gauss_ref holds a recorded residual
     int push_kernel = 0;          // 0 auto, 1 naive (global atomics), 2 tiled
     int keep_old = 0;
+    int xmax_face = 0;            // PIC_OPT_XMAX_BOUNDARY
     int fix32 = 0;                // FP32: int32 fixed-point LDS accumulators
     void* jtmp = nullptr;         // J-sized scratch of the current filter (lazy)
     void* stage = nullptr;        // output staging buffer (lazy, k_stage.hip)
@@ -260,5 +268,28 @@ template <> struct TileShape<2, 4> { static constexpr int X = 8, Y = 4, Z = 4; }
 void set_tiling(pic_ctx* c);
 
 inline int blocks_for(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// The particles, fields, species or background were set from outside the
+// time loop: the Gauss residual's rho_prev and reference no longer apply.
+inline void gauss_invalidate(pic_ctx* c) {
+    c->rho_prev_valid = 0;
+    c->gauss_ref_valid = 0;
+}
+
+// Bit mask of the declared mobile species (their charge forms rho).
+inline int mobile_mask(const pic_ctx* c) {
+    int m = 0;
+    for (int s = 0; s < PIC_MAX_SPECIES; s++)
+        if (c->sp[s].declared && !c->sp[s].immobile) m |= 1 << s;
+    return m;
+}
+
+// Diagnostics reduction buffer (doubles): [0] sum E^2, [1] sum B^2,
+// [2] sum R^2, [3] max |R| (bits), [4..7] charge per species, [8..11]
+// kinetic per species, [12..12+bins) species-0 spectrum, then sum and max
+// of the drift R - R_ref over the interior and over the inner region, and
+// sum R_ref^2.
+enum { DG_E2 = 0, DG_B2 = 1, DG_R2 = 2, DG_RMAX = 3, DG_CHG = 4, DG_KIN = 8, DG_SPEC = 12,
+       DG_D2 = DG_SPEC + PIC_SPECTRUM_BINS, DG_DMAX, DG_I2, DG_IMAX, DG_REF2, DG_N };
 
 }  // namespace pic
