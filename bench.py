@@ -12,8 +12,12 @@ synthetic seeded particles generated on the device.
 
 A step is one full pass of the reference's time loop (laser boundary,
 E/B half steps, current clear, push = gather + Boris + Esirkepov deposit,
-particle boundary, current fold, E/B half steps).  W untimed warm-up steps,
-then K timed steps bracketed by stream synchronisation.  Prints one JSON
+particle boundary, current fold, E/B half steps).  W untimed warm-up steps
+(default 400: the pulse, which enters the slab near step 128, is inside
+the plasma), then K timed steps bracketed by stream synchronisation.  The
+cold window (steps 4..3+K, before the pulse meets the plasma) is timed
+during the warm-up and reported beside it, with the time-centred Gauss
+residual's drift between step 2 and the end of the timed window.  Prints one JSON
 line: value = particle pushes per second over whole steps; Yee
 cell-updates/s and the push kernel's roofline come from HIP events recorded
 around every phase inside the timed region.  The roofline object also
@@ -94,21 +98,48 @@ def load_util(name: str):
     return out
 
 
-def cpu_baseline(wl_name: str, precision: str, threads: int) -> dict:
-    """The CPU oracle (FP64 restatement of the reference loop, OpenMP) on a
-    bounded sample of the same workload, about 10-30 s of CPU work: the full
-    grid with a thinner electron slab, pre-loop push + 4 timed steps."""
+def host_cores():
+    """(physical cores from lscpu, logical CPUs this process may run on)."""
+    import subprocess
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=30).stdout
+        pairs = {tuple(l.split(",")[:2]) for l in out.splitlines() if l and not l.startswith("#")}
+        phys = len(pairs) or None
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return phys, allowed
+
+
+def cpu_baseline(wl_name: str, precision: str, threads: int | None, warmup: int = 5,
+                 runs: int = 3) -> dict:
+    """The CPU oracle (FP64 restatement of the reference loop, OpenMP:
+    particles split over threads with private current buffers, field loops
+    collapsed) on the full workload: the pre-loop push, `warmup` untimed
+    steps, then `runs` timed steps, each timed alone; the median of their
+    rates.  Threads: the host's physical cores from lscpu, capped at the
+    CPUs this process may use (the GPU box's share), with
+    OMP_PROC_BIND=close."""
     from oracle.oracle import Oracle, load_particles
     from pic3d_amd import workloads as W
+    phys, allowed = host_cores()
+    if threads is None:
+        threads = min(phys or allowed, allowed)
     if wl_name == "laser_slab":
-        wl = W.laser_slab(slab=(64.0, 160.0))
-        sample = "laser_slab grid 256x128x128, mem 4, slab x in [64,160) 16 ppc (25.2M e-), 4 steps"
+        wl = W.laser_slab()
+        sample = ("laser_slab 256x128x128, mem 4, 16 ppc in x in [64,224) (41.9M e-): pre-loop push, "
+                  f"{warmup} warm-up steps, median of {runs} single timed steps")
     elif wl_name == "thermal":
         wl = W.thermal()
-        sample = "thermal 32^3, 8 ppc, mem 2, 2 steps"
+        sample = f"thermal 32^3, 8 ppc, mem 2: {warmup} warm-up steps, median of {runs} steps"
     else:
         wl = W.ramp_target(nx=128, ny=64, nz=64, ppc_target=228)
-        sample = "ramp_target at 128x64x64 (1/64 of the cells), 2 steps"
+        sample = f"ramp_target at 128x64x64 (1/64 of the cells): {warmup} warm-up steps, median of {runs}"
     p = wl.params.copy()
     p.precision = 0
     parts = load_particles(p, wl.species[0].loader)
@@ -117,16 +148,22 @@ def cpu_baseline(wl_name: str, precision: str, threads: int) -> dict:
         o.species_set(s, sp.charge, sp.mass, sp.immobile)
     o.set_background(wl.rho_background)
     o.particles_set(0, parts)
+    del parts
     o.preloop()
-    n0 = o.num_particles(0)
-    t = time.perf_counter()
-    nst = 4
-    o.step(1, nst)
-    dt = time.perf_counter() - t
-    n1 = o.num_particles(0)
+    o.step(1, warmup)
+    rates, secs = [], []
+    for r in range(runs):
+        n0 = o.num_particles(0)
+        t = time.perf_counter()
+        o.step(1 + warmup + r, 1)
+        dt = time.perf_counter() - t
+        rates.append(0.5 * (n0 + o.num_particles(0)) / dt)
+        secs.append(round(dt, 3))
     o.close()
-    return {"value": nst * 0.5 * (n0 + n1) / dt, "unit": "particle pushes/s", "cores": threads,
-            "kind": "port", "sample": sample, "seconds": round(dt, 3)}
+    return {"value": float(np.median(rates)), "unit": "particle pushes/s", "cores": threads,
+            "threads": threads, "physical_cores_lscpu": phys, "cpus_allowed": allowed,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "kind": "port", "sample": sample,
+            "seconds_per_step": secs}
 
 
 def run_gpu(args, precision: str) -> dict:
@@ -156,9 +193,30 @@ def run_gpu(args, precision: str) -> dict:
     if getattr(args, "spin", False):
         sim.spin_fill(0, 0.0, 0.0, 1.0)
     n_start = sim.num_particles(0)
+    emax = 20.0
     sim.preloop()
-    if args.warmup:
-        sim.step(1, args.warmup)
+    # warm-up: steps 1..W; the Gauss reference is recorded after step 2
+    # (step 1 carries the pre-loop push's unfolded charge), and steps
+    # 4..3+K, before the pulse meets the plasma, are timed as the cold window
+    done, cold = 0, None
+    if args.warmup >= 2:
+        sim.step(1, 2)
+        done = 2
+    d_ref = sim.diag(emax) if done else None
+    if args.warmup >= 3 + args.steps:
+        sim.step(3, 1)
+        sim.sync()
+        c0 = sim.num_particles(0)
+        t = time.perf_counter()
+        sim.step(4, args.steps)
+        sim.sync()
+        dtc = time.perf_counter() - t
+        c1 = sim.num_particles(0)
+        cold = {"steps": [4, 3 + args.steps], "ms_per_step": 1e3 * dtc / args.steps,
+                "pushes_per_s": args.steps * 0.5 * (c0 + c1) / dtc}
+        done = 3 + args.steps
+    if args.warmup > done:
+        sim.step(1 + done, args.warmup - done)
     sim.sync()
     n0 = sim.num_particles(0)
     sim.set_option(PIC_OPT_TIMING, 1)
@@ -169,7 +227,9 @@ def run_gpu(args, precision: str) -> dict:
     sim.sync()
     t1 = time.perf_counter()
     tm = sim.timing(reset=True)
+    sim.set_option(PIC_OPT_TIMING, 0)
     n1 = sim.num_particles(0)
+    d_end = sim.diag(emax)
     wall = t1 - t0
     pushes = args.steps * 0.5 * (n0 + n1)
     cells = p.nx * p.ny * p.nz
@@ -202,7 +262,19 @@ def run_gpu(args, precision: str) -> dict:
                          "achieved_GBs": (cell_bytes(precision) * cells * args.steps /
                                           (fields_ms / 1e3) / 1e9) if fields_ms > 0 else None},
         "workload": wl.notes,
+        "window": [1 + args.warmup, args.warmup + args.steps],
+        "field_energy": d_end["field_energy"], "kinetic_energy": d_end["kinetic"][0],
     }
+    if cold:
+        out["cold_window"] = cold
+    if d_ref is not None and d_end["gauss_state"] == 2:
+        # time-centred Gauss residual div E - (rho^n + rho^(n+1))/2 - rho_bg:
+        # its largest value at step 2 and its change since then (whole
+        # interior; inner region = away from open faces)
+        out["gauss"] = {"residual_max_at_step_2": d_ref["gauss_max"],
+                        "drift_max": d_end["gauss_drift_max"],
+                        "inner_drift_max": d_end["gauss_inner_drift_max"],
+                        "inner_drift_rel": d_end["gauss_inner_drift_max"] / (d_ref["gauss_max"] + 1.0)}
     sim.close()
     return out
 
@@ -323,7 +395,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=400,
+                    help="untimed steps before the timed window (default 400: the laser-slab "
+                         "pulse is inside the plasma; steps 4..3+K are timed as the cold window)")
     ap.add_argument("--precision", default="both", choices=["fp64", "fp32", "both"])
     ap.add_argument("--workload", default="laser_slab",
                     choices=["laser_slab", "ramp_target", "thermal"])
@@ -336,7 +410,9 @@ def main():
     ap.add_argument("--fp32-accum", type=int, default=1,
                     help="FP32 deposit accumulators: 1 int32 fixed point (default), 0 FP64 in LDS")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="CPU baseline threads (default: lscpu's physical cores, capped at the "
+                         "CPUs this process may use)")
     ap.add_argument("--ramp-steps", type=int, default=5,
                     help="timed steps of the FP32 ramp-target throughput case reported beside "
                          "the headline (0: skip it)")
@@ -383,6 +459,9 @@ def main():
         "yee_cell_updates_per_s": head["yee_cell_updates_per_s"],
         "roofline": head["roofline"],
         "phase_ms_per_step": head["phase_ms_per_step"],
+        "window": head["window"],
+        "cold_window": head.get("cold_window"),
+        "gauss": head.get("gauss"),
         "by_precision": {pr: {k: v for k, v in r.items() if k not in ("workload",)}
                          for pr, r in res.items()},
     }
@@ -400,7 +479,7 @@ def main():
         line["ramp_target"]["sort_interval"] = ra.sort_interval
     if args.spin_steps > 0 and args.workload == "laser_slab" and "fp64" in res:
         sa = argparse.Namespace(**vars(args))
-        sa.steps, sa.warmup, sa.spin = args.spin_steps, 2, True
+        sa.steps, sa.warmup, sa.spin = args.spin_steps, args.warmup, True  # the headline's window
         r = run_gpu(sa, "fp64")
         line["spin_fp64"] = {"ms_per_step": r["ms_per_step"],
                              "push_ms": r["phase_ms_per_step"]["push"],
@@ -409,6 +488,7 @@ def main():
     if args.adjoint_steps > 0:
         line["adjoint"] = run_adjoint(args)
     if not args.no_cpu_baseline:
+        os.environ.setdefault("OMP_PROC_BIND", "close")  # read when the oracle's OpenMP starts
         line["cpu_baseline"] = cpu_baseline(args.workload, hp, args.cpu_threads)
     print(json.dumps(line), flush=True)
 
