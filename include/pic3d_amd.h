@@ -366,15 +366,20 @@ int pic_deposit_noalloc(pic_ctx* ctx, int s, int64_t n, const double* x_old,
  * One mobile species (others immobile), in the context's precision: an
  * FP32 context runs the FP32 forward and FP32 adjoint kernels (fields,
  * momenta, offsets and their adjoints in fp32; the objective, the envelope
- * gradient and the weight gradients accumulated in FP64).  pic_diff_begin
- * sorts the species by tile once; the run never re-sorts, so particle
- * slots stay fixed (particles that leave their tile take the tiled push's
- * HBM path; particles leaving through open faces are marked dead in
- * place).  The forward keeps a full-state
- * checkpoint every K steps; the backward recomputes each K-step segment
- * from its checkpoint, saving the per-step push inputs, and runs the
- * adjoint kernels of that segment in reverse.  Memory: one checkpoint per
- * K steps (fields, currents, particles) plus K push-input states. */
+ * gradient and the weight gradients accumulated in FP64).  A tiled run
+ * re-sorts the species by tile every PIC_OPT_SORT_INTERVAL pushes with a
+ * sort that keeps removed slots (particles leaving through open faces are
+ * marked dead, so the slot count never changes) and records each sort's
+ * permutation; the recompute replays the recorded permutations, so it
+ * puts every particle in the slot the forward did, and the adjoint carries
+ * the particle adjoints back through them.  The forward keeps a
+ * full-state checkpoint every K steps; the backward recomputes each
+ * K-step segment from its checkpoint, saving the per-step push inputs,
+ * and runs the adjoint kernels of that segment in reverse.  Memory: one
+ * checkpoint per K steps (fields, currents, particles), K push-input
+ * states and one int32 permutation per sort.  In FP32 the deposit's
+ * fixed-point unit follows the largest weight: pic_diff_forward with new
+ * weights sizes it for them. */
 
 /* Snapshot the current state of the context as the initial state of a
  * differentiated run of species s. */

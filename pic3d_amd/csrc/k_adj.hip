@@ -1873,6 +1873,13 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
         if (!r) r = dalloc(c, (void**)&D.toffs, (size_t)D.nsorts * (c->tl.ntiles + 2) * 8);
     }
     if (!r && weights && D.slots) {  // new initial weights: into the state and the snapshot
+        // The FP32 deposit's fixed-point unit is sized from the largest
+        // weight (k_push32.hip consts32): size it for these weights, so that
+        // the forward, the recompute and the adjoint of this run all use it.
+        double wmax = 0.0;
+        for (int64_t q = 0; q < D.slots; q++)
+            if (std::isfinite(weights[q])) wmax = std::max(wmax, std::fabs(weights[q]));
+        c->sp[D.s].wmax = wmax;
         double* tmp = nullptr;
         r = dalloc(c, (void**)&tmp, D.slots * 8);
         if (!r && hipMemcpyAsync(tmp, weights, D.slots * 8, hipMemcpyHostToDevice, c->stream) != hipSuccess)
@@ -1890,7 +1897,9 @@ int pic_diff_forward(pic_ctx* c, int first_step, int nsteps, int K, const double
         if (!r && hipStreamSynchronize(c->stream) != hipSuccess) r = set_err(c, PIC_ERR_HIP, "pic_diff_forward failed");
         if (tmp) hipFree(tmp);
     }
-    c->push_kernel = D.kernel;  // slot-stable push: no re-sort during the run
+    // the push's own re-sort is off: diff_step re-sorts every D.sort_every
+    // steps itself and records each permutation for the recompute
+    c->push_kernel = D.kernel;
     c->sort_interval = 1 << 30;
     for (int st = first_step; st < first_step + nsteps && !r; st++) {
         const int i = st - first_step;

@@ -32,11 +32,35 @@ def load_golden(name: str) -> dict:
     g = {"params": p, "loader": l, "parts": parts, "cmds": [str(c) for c in z["cmds"]],
          "fields": out, "particles": outp, "charge": float(z["charge"]),
          "mass": float(z["mass"]), "rho_background": float(z["rho_background"])}
+    if "charge1" in z.files:  # a second mobile species (protons, main.cpp:71-79)
+        g["species1"] = {"charge": float(z["charge1"]), "mass": float(z["mass1"]),
+                         "parts": {k: z["in1_" + k] for k in ("x", "y", "z", "ux", "uy", "uz", "w", "id")},
+                         "particles": {k[6:]: z[k] for k in z.files if k.startswith("outp1_")}}
     if "in_sx" in z.files:  # spin precession case
         g["spin_in"] = [z["in_sx"], z["in_sy"], z["in_sz"]]
         g["e_schwinger"] = float(z["e_schwinger"])
         g["g_table"] = z["g_table"]
     return g
+
+
+def declare_golden(sim, g, capacity_slack: int = 0):
+    """Declare a fixture's species on an Oracle or a Simulation (species 0,
+    and species 1 when the fixture pushes protons too), set the background
+    and load the input particles."""
+    from oracle.oracle import Oracle
+    species = [(g["charge"], g["mass"], g["parts"])]
+    if "species1" in g:
+        s1 = g["species1"]
+        species.append((s1["charge"], s1["mass"], s1["parts"]))
+    for s, (q, m, parts) in enumerate(species):
+        if isinstance(sim, Oracle):
+            sim.species_set(s, q, m)
+        else:
+            sim.species_set(s, q, m, False, len(parts["x"]) + capacity_slack)
+    sim.set_background(g["rho_background"])
+    for s, (_, _, parts) in enumerate(species):
+        sim.particles_set(s, parts)
+    return len(species)
 
 
 def replay(sim, cmds):

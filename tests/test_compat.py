@@ -99,3 +99,31 @@ def test_undefined_symbols_are_the_callers():
             "stderr", "strlen"}
     extra = sorted(ours - CALLER_PROVIDES - libc)
     assert not extra, extra
+
+
+REF = "/root/reference"
+MPI_LIB = "/opt/conda/lib/libmpi.so"
+
+
+@pytest.mark.skipif(not (os.path.exists(os.path.join(REF, "main.cpp")) and os.path.exists(MPI_LIB)),
+                    reason="needs the reference sources and the image's MPICH")
+def test_reference_main_links_unchanged(tmp_path):
+    """The reference's own main.cpp and src/constants.cpp, compiled from
+    where they lie and unmodified, link against the drop-in libraries
+    (libpic3d_compat_io for io_utils.h, libpic3d_compat for the rest of
+    include/*.h, over libpic3d_amd) with nothing else of the reference."""
+    inc = [f"-I{REF}/include", "-I/opt/conda/include", "-DMPICH_SKIP_MPICXX"]
+    objs = []
+    for src in ("main.cpp", "src/constants.cpp"):
+        obj = str(tmp_path / (os.path.basename(src) + ".o"))
+        r = subprocess.run(["g++", "-std=c++17", "-O0", *inc, "-c", os.path.join(REF, src), "-o", obj],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:]
+        objs.append(obj)
+    lib = os.path.join(ROOT, "pic3d_amd")
+    exe = str(tmp_path / "main_on_dropin")
+    r = subprocess.run(["g++", *objs, f"-L{lib}", "-lpic3d_compat_io", "-lpic3d_io", "-lpic3d_compat",
+                        "-lpic3d_amd", f"-L{os.path.join(lib, '_hdf5')}", "-lhdf5", MPI_LIB, "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert os.path.getsize(exe) > 0

@@ -186,3 +186,40 @@ def test_fp32_adjoint_kernels_agree(mem):
             assert rel(a, b) <= 1e-5, (k, rel(a, b))
     run.close()
     sim.close()
+
+
+@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("scale", [4.0, 0.25])
+def test_fp32_new_weights_size_the_deposit_unit(kernel, scale):
+    """The FP32 deposit accumulates in int32 fixed point whose unit is sized
+    from the largest weight (a chunk of 2^12 particles has 2x headroom).  A
+    differentiated run given weights 4x (or 1/4x) its loaded ones must equal
+    the same run loaded with those weights (the bound q w d/dt of one
+    particle's contribution, particle.cpp:422-425, follows the weights the
+    run uses).  The two runs differ only in the order of the global float
+    atomics that flush the deposits: objective within 1e-6, gradients within
+    1e-4 (relative 2-norm).  A unit sized for the old weights would wrap
+    the int32 accumulators at 4x and be off by orders of magnitude."""
+    print(f"diff32 new weights x{scale} kernel {kernel}", flush=True)
+    wl = W.laser_slab(nx=24, ny=8, nz=8, ppc=2, slab=(3.0, 14.0), seed=33, mem=2)
+    wl.params.laser_amplitude = 2.0
+    wl.params.laser_pulse_duration = 6.0
+    parts = fp32_grid(load_particles(wl.params, wl.species[0].loader))
+    scaled = dict(parts)
+    scaled["w"] = parts["w"] * scale
+    sim_a, run_a = make(wl, parts, PIC_FP32, kernel, 30, 7)
+    env = run_a.reference_envelope()
+    k_a = run_a.forward(env, run_a.initial_weights() * scale)
+    ge_a, gw_a = run_a.backward()
+    sim_b, run_b = make(wl, scaled, PIC_FP32, kernel, 30, 7)
+    k_b = run_b.forward(env, None)
+    ge_b, gw_b = run_b.backward()
+    # the tiled sort's order within a tile follows its atomics: match slots
+    # by their initial positions
+    gw_a, gw_b = gw_a[slot_order(run_a)], gw_b[slot_order(run_b)]
+    err = {"K": abs(k_a - k_b) / abs(k_b), "env": rel(ge_a, ge_b), "w": rel(gw_a, gw_b)}
+    print(f"  K {k_a:.10e} vs {k_b:.10e}; {err}", flush=True)
+    assert err["K"] <= 1e-6 and err["env"] <= 1e-4 and err["w"] <= 1e-4, err
+    for r, s in ((run_a, sim_a), (run_b, sim_b)):
+        r.close()
+        s.close()

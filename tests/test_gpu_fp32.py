@@ -27,7 +27,7 @@ from pic3d_amd import PIC_FP32, Simulation, workloads as W
 from pic3d_amd._abi import FIELD_NAMES, PIC_OPT_FP32_ACCUM
 from pic3d_amd.simulation import field_shape
 
-from _helpers import by_id, golden_names, load_golden, rel_err, replay
+from _helpers import by_id, declare_golden, golden_names, load_golden, rel_err, replay
 
 pytestmark = pytest.mark.gpu
 
@@ -160,11 +160,16 @@ def test_fp32_golden(name):
     print(f"fp32 golden {name}", flush=True)
     g = load_golden(name)
     if any(c.startswith("dep:") for c in g["cmds"]):
+        # pic_deposit is an FP64 entry point; surfnodep_* runs the rest of
+        # this surface in FP32
         pytest.skip("the stand-alone deposit (pic_deposit) is an FP64 entry point")
+    if name.startswith("edges_"):
+        # particles resting exactly on the open face x == N: FP64 drops them
+        # (x >= N, mpi_utils.cpp:80-86), the FP32 layout holds them 2^-24
+        # inside; test_gpu_parity.py::test_hand_placed_push_fp32 states it
+        pytest.skip("x == N exactly on an open face is not representable in the FP32 layout")
     sim = Simulation(fp32(g["params"]))
-    sim.species_set(0, g["charge"], g["mass"], False, len(g["parts"]["x"]))
-    sim.set_background(g["rho_background"])
-    sim.particles_set(0, g["parts"])
+    declare_golden(sim, g)
     replay(sim, g["cmds"])
     for k in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz"):
         got = sim.field_get(FIELD_NAMES.index(k))

@@ -17,7 +17,7 @@ from oracle.oracle import Oracle, load_particles
 from pic3d_amd import Simulation, workloads as W
 from pic3d_amd._abi import FIELD_NAMES
 
-from _helpers import by_id, golden_names, load_golden, rel_err, replay
+from _helpers import by_id, declare_golden, golden_names, load_golden, rel_err, replay
 
 pytestmark = pytest.mark.gpu
 
@@ -53,17 +53,18 @@ def test_golden_vs_reference(name, kernel):
     print(f"golden {name} kernel {kernel}", flush=True)
     g = load_golden(name)
     dep = any(c.startswith("dep:") for c in g["cmds"])  # deposit needs x_old
-    sim = make_sim(g["params"], g["charge"], g["mass"], g["parts"], kernel=kernel, keep_old=dep)
-    sim.set_background(g["rho_background"])
+    sim = Simulation(g["params"], push_kernel=kernel, keep_old=dep)
+    nsp = declare_golden(sim, g)
     replay(sim, g["cmds"])
     for k, ref in g["fields"].items():
         got = sim.field_get(FIELD_NAMES.index(k))
         assert rel_err(got, ref) <= 1e-10, (k, rel_err(got, ref))
-    pg = by_id(sim.particles_get(0))
-    pr = by_id(g["particles"])
-    assert np.array_equal(pg["id"], pr["id"])  # same particles survived
-    for k in ("x", "y", "z", "ux", "uy", "uz"):
-        assert np.max(np.abs(pg[k] - pr[k])) <= 1e-10 * max(1.0, np.max(np.abs(pr[k]))), k
+    for s in range(nsp):
+        pg = by_id(sim.particles_get(s))
+        pr = by_id(g["particles"] if s == 0 else g["species1"]["particles"])
+        assert np.array_equal(pg["id"], pr["id"]), s  # same particles survived
+        for k in ("x", "y", "z", "ux", "uy", "uz"):
+            assert np.max(np.abs(pg[k] - pr[k])) <= 1e-10 * max(1.0, np.max(np.abs(pr[k]))), (s, k)
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -237,3 +238,88 @@ def test_thermal_32_energy_and_cells(kernel):
     assert abs(sg.sum() - so.sum()) <= 1e-12 * so.sum()
     assert np.max(np.abs(sg - so)) <= 2 * parts["w"][0] + 1e-12 * np.max(so)
     assert abs(dg["gauss_l2"] - do["gauss_l2"]) <= 1e-8 * max(do["gauss_l2"], 1e-300)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("mem", [2, 4])
+@pytest.mark.parametrize("fields", ["zero", "random"])
+def test_hand_placed_push_bitexact(mem, kernel, fields):
+    """The hand-placed particles of tests/golden/edges_* (positions exactly
+    at 0 and N on the open x and periodic y/z axes, half nodes where
+    (int)(x + 0.5) flips and just below them, tile corners and just below
+    them, moves that end exactly on a node; particle.cpp:123-167, 184-203,
+    mpi_utils.cpp:80-86): one push and particle boundary (the pre-loop of
+    main.cpp:75-76) on zero or random fields.  Positions, momenta and x_old
+    bit-exact against the oracle, surviving ids and per-face loss and wrap
+    counts exact, J and rho within 1e-12."""
+    print(f"hand-placed mem {mem} kernel {kernel} fields {fields}", flush=True)
+    g = load_golden(f"edges_m{mem}_16x12x12")
+    p = g["params"]
+    sim = Simulation(p, push_kernel=kernel, keep_old=True)
+    orc = Oracle(p, 1)
+    declare_golden(sim, g)
+    declare_golden(orc, g)
+    if fields == "random":
+        for c, a in random_fields(p, np.random.default_rng(11 + mem)).items():
+            sim.field_set(c, a)
+            orc.field_set(c, a)
+    sim.push(0); orc.push(0)
+    cg, co = sim.particle_boundary(0), orc.particle_boundary(0)
+    assert cg["lost"] == co["lost"] and cg["wrapped"] == co["wrapped"], (cg, co)
+    pg, po = by_id(sim.particles_get(0)), by_id(orc.particles_get(0))
+    assert np.array_equal(pg["id"], po["id"])
+    for k in ("x", "y", "z", "ux", "uy", "uz", "x_old", "y_old", "z_old"):
+        assert np.array_equal(pg[k], po[k]), k
+    for c in (6, 7, 8, 9):
+        a, b = sim.field_get(c), orc.field_get(c)
+        assert rel_err(a, b) <= 1e-12, (FIELD_NAMES[c], rel_err(a, b))
+    if fields == "zero":
+        # the construction holds: the moves meant to end exactly on a node did
+        on = np.zeros(len(po["x"]), bool)
+        for t in (3.0, 5.5, 7.0, 8.0, 9.5):
+            on |= (po["x"] == t) & (po["x_old"] != t)
+            on |= (po["y"] == t) & (po["y_old"] != t)
+        assert np.count_nonzero(on) >= 15, np.count_nonzero(on)
+
+
+@pytest.mark.parametrize("mem", [2, 4])
+def test_hand_placed_push_fp32(mem):
+    """The same hand-placed push in FP32 (cell + fp32 offset) against the
+    FP64 oracle, on random fields whose periodic ghosts are the images:
+    positions within 2e-6 cells (periodic axes modulo N: FP32 wraps right
+    after the move, FP64 at the next push), momenta within 2e-6 of max |u|,
+    the same survivors -- except a particle that ends exactly on the open
+    face x == N, which FP64 drops (x >= N, mpi_utils.cpp:80-86) and the
+    FP32 layout holds 2^-24 inside (to_cell_offset)."""
+    print(f"hand-placed fp32 mem {mem}", flush=True)
+    g = load_golden(f"edges_m{mem}_16x12x12")
+    p = g["params"].copy()
+    orc = Oracle(g["params"], 1)
+    declare_golden(orc, g)
+    p.precision = 1
+    sim = Simulation(p)
+    declare_golden(sim, g)
+    for c, a in random_fields(g["params"], np.random.default_rng(21 + mem)).items():
+        sim.field_set(c, a)
+        orc.field_set(c, a)
+    for c in range(6):  # periodic ghosts hold the images (y == N and y == 0 are one place)
+        sim.field_exchange(c)
+        orc.field_exchange(c)
+    orc.push(0); sim.push(0)
+    orc.particle_boundary(0); sim.particle_boundary(0)
+    pg, po = by_id(sim.particles_get(0)), by_id(orc.particles_get(0))
+    extra = np.setdiff1d(pg["id"], po["id"])
+    missing = np.setdiff1d(po["id"], pg["id"])
+    assert len(missing) == 0
+    if len(extra):
+        assert np.all(np.abs(pg["x"][np.isin(pg["id"], extra)] - p.nx) < 1e-6), extra
+    keep = np.isin(pg["id"], po["id"])
+    umax = max(float(np.max(np.abs(po[k]))) for k in ("ux", "uy", "uz"))
+    for a, k in enumerate(("x", "y", "z")):
+        d = pg[k][keep] - po[k]
+        if p.periodic[a]:  # FP32 wraps right after the move, FP64 at the next push
+            n = (p.nx, p.ny, p.nz)[a]
+            d = (d + n / 2) % n - n / 2
+        assert np.max(np.abs(d)) <= 2e-6, k
+    for k in ("ux", "uy", "uz"):
+        assert np.max(np.abs(pg[k][keep] - po[k])) <= 2e-6 * umax, k
