@@ -19,6 +19,10 @@ __device__ __forceinline__ void lds_add(float* p, float v) {
 __device__ __forceinline__ void lds_add(double* p, double v) {
     __hip_atomic_fetch_add((lds_f64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+__device__ __forceinline__ void lds_add_u64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_add((lds_u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 typedef __attribute__((address_space(3))) int lds_i32;
 __device__ __forceinline__ void lds_add_i32(int* p, int v) {
     __hip_atomic_fetch_add((lds_i32*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

@@ -27,6 +27,10 @@ struct PushConsts {
     double N[3];
     int per[3];
     SpinConsts spin;
+    // int64 fixed-point units of the tiled push's LDS deposit (jx, jy, jz,
+    // rho): 2^-48 of one particle's largest contribution, |q| w_max d/dt
+    // (|q| w_max for rho), and their inverses
+    double unit[4], inv_unit[4];
 };
 
 // Periodic wrap at the start of the push (particle.cpp:123-167) followed by
