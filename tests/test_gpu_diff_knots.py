@@ -3,17 +3,21 @@ quadratic shapes, the laser-slab pulse at lambda = 16 cells, K = total
 electron kinetic energy after 200 steps, checkpoint every 20 steps.
 Gradients with respect to 16 envelope knot amplitudes (piecewise-linear in
 time) and the 64 x-profile density values (particle weight = n(x cell) /
-ppc) come from torch autograd through the library's adjoint; selected
-components are checked against central finite differences of the FP64
-forward, to 1e-4.
+ppc) come from torch autograd through the library's adjoint.
 
-Over these 200 steps about 82,000 electrons leave through the open x
-faces, and the step at which each leaves is a discontinuous function of
-the parameters: K is smooth only between such events.  A central
-difference with h = 1e-6 (relative) straddles some of them (knot 3:
-201.81 vs 201.67 from the adjoint; h = 1e-4 gives 171); with h = 1e-7 it
-agrees with the adjoint to 2e-8.  The test uses h = 1e-7; over 100 steps,
-with no losses, every h from 1e-4 to 1e-7 agrees to 1e-8."""
+Every knot and every 8th profile value is checked against central finite
+differences of the FP64 forward, at h = 1e-6 and h = 1e-7 (relative), side
+by side.  Over these 200 steps about 82,000 electrons leave through the
+open x faces, and whether (and when) each leaves is a discontinuous
+function of the parameters: K is smooth only between such events.  For
+every difference the net number of loss events it straddles is reported:
+the difference in surviving particles between the +h and -h runs (zero can
+still hide a loss whose step shifted).  Bar: every component whose h = 1e-7
+pair shows no straddled loss agrees with the adjoint to 1e-4 (relative,
+plus the difference quotient of 1e-12 K, the forward's run-to-run rounding,
+for components near zero: profile values outside the slab move no
+particle), and such components are the large majority; the rest are
+reported."""
 import numpy as np
 import pytest
 
@@ -22,15 +26,20 @@ from pic3d_amd import Simulation, workloads as W
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
+_STATE = {}
 
-def test_knots_and_profile_gradients():
+
+def state():
+    """The run, its adjoint gradients and the finite-difference objective
+    (built once per module)."""
+    if _STATE:
+        return _STATE
     from pic3d_amd.diff import (DiffRun, KineticEnergy, envelope_from_knots, knot_matrix,
                                 reference_knots, step_times, weights_from_profile)
-    print("diff knots 64^3", flush=True)
+    from pic3d_amd.simulation import setup
     wl = W.diff_small()
     p = wl.params
     sim = Simulation(p)
-    from pic3d_amd.simulation import setup
     setup(sim, wl, device_loader=True)
     ppc = wl.species[0].loader.ppc
     run = DiffRun(sim, 0, first_step=1, nsteps=wl.steps, checkpoint_every=20)
@@ -46,36 +55,75 @@ def test_knots_and_profile_gradients():
             prof0[i] = slots["w"][sel][0] * ppc
     knots = torch.tensor(k0, requires_grad=True)
     prof = torch.tensor(prof0, requires_grad=True)
-    env = envelope_from_knots(knots, phi)
-    w = weights_from_profile(prof, cell_x, ppc, live)
-    K = KineticEnergy.apply(env, w, run)
+    K = KineticEnergy.apply(envelope_from_knots(knots, phi), weights_from_profile(prof, cell_x, ppc, live), run)
     K.backward()
-    gk, gp = knots.grad.numpy(), prof.grad.numpy()
-    k_val = float(K)
-    print(f"  K = {k_val:.10e}", flush=True)
-    assert k_val > 0 and np.any(gk != 0) and np.any(gp != 0)
 
     def objective(kn, pr):
-        e = phi @ kn
-        ww = np.where(live, pr[cell_x] / ppc, np.nan)
-        return run.forward(e, ww)
+        """K and the survivors' ids of a forward run."""
+        k = run.forward(phi @ kn, np.where(live, pr[cell_x] / ppc, np.nan))
+        parts = sim.particles_get(0)  # lost particles keep their slots with a NaN weight
+        return k, int(np.count_nonzero(np.isfinite(parts["w"])))
 
-    H = 1e-7
-    for k in (3, 9):
-        h = H * max(abs(k0[k]), 1.0)
-        kp, km = k0.copy(), k0.copy()
-        kp[k] += h
-        km[k] -= h
-        fd = (objective(kp, prof0) - objective(km, prof0)) / (2 * h)
-        print(f"  knot {k}: fd {fd:.10e} adjoint {gk[k]:.10e}", flush=True)
-        assert abs(fd - gk[k]) <= 1e-4 * abs(gk[k]) + 1e-10 * k_val, (k, fd, gk[k])
-    for i in (20, 40):
-        h = H * prof0[i]
-        pp, pm = prof0.copy(), prof0.copy()
-        pp[i] += h
-        pm[i] -= h
-        fd = (objective(k0, pp) - objective(k0, pm)) / (2 * h)
-        print(f"  profile x={i}: fd {fd:.10e} adjoint {gp[i]:.10e}", flush=True)
-        assert abs(fd - gp[i]) <= 1e-4 * abs(gp[i]) + 1e-10 * k_val, (i, fd, gp[i])
-    run.close()
-    sim.close()
+    _STATE.update(sim=sim, run=run, k0=k0, prof0=prof0, gk=knots.grad.numpy().copy(),
+                  gp=prof.grad.numpy().copy(), K=float(K.detach()), objective=objective)
+    return _STATE
+
+
+def central(which, idx, h):
+    """(fd, net straddled losses) of component idx of 'knots' or 'profile':
+    the difference in surviving particles between the +h and -h runs (a
+    lower bound on the loss events the pair straddles)."""
+    st = state()
+    out = []
+    for sgn in (1.0, -1.0):
+        kn, pr = st["k0"].copy(), st["prof0"].copy()
+        (kn if which == "knots" else pr)[idx] += sgn * h
+        out.append(st["objective"](kn, pr))
+    (kp, np_), (km, nm) = out
+    return (kp - km) / (2 * h), abs(np_ - nm)
+
+
+def step(which, i, H):
+    """h relative to the component (knots: to max(|k|, 1); a profile value
+    outside the slab, 0, moves no particle: relative to the slab's)."""
+    st = state()
+    if which == "knots":
+        return H * max(abs(st["k0"][i]), 1.0)
+    v = st["prof0"][i]
+    return H * (v if v > 0 else float(np.max(st["prof0"])))
+
+
+def check(which, comps, adj):
+    st = state()
+    rows, bad = [], []
+    for i in comps:
+        (fd6, s6), (fd7, s7) = (central(which, i, step(which, i, H)) for H in (1e-6, 1e-7))
+        # absolute: the difference quotient of the forward's rounding noise
+        # (two runs of one state agree to about 1e-13 K: atomic order)
+        ok = abs(fd7 - adj[i]) <= 1e-4 * abs(adj[i]) + 1e-12 * st["K"] / step(which, i, 1e-7)
+        print(f"  {which} {i:2d}: adjoint {adj[i]: .8e}  fd(1e-6) {fd6: .8e} [{s6} straddled]  "
+              f"fd(1e-7) {fd7: .8e} [{s7} straddled]  {'ok' if ok else 'OFF'}", flush=True)
+        rows.append((i, s7, ok))
+        if s7 == 0 and not ok:
+            bad.append(i)
+    clean = [r for r in rows if r[1] == 0]
+    print(f"  {which}: {len(clean)} of {len(rows)} h=1e-7 pairs straddle no loss; "
+          f"{sum(r[2] for r in rows)} of {len(rows)} agree to 1e-4", flush=True)
+    assert not bad, f"components without straddled losses off the adjoint: {bad}"
+    assert len(clean) >= 0.75 * len(rows), rows
+
+
+def test_knot_gradients():
+    print("diff knots 64^3: all 16 envelope knots", flush=True)
+    st = state()
+    assert st["K"] > 0 and np.any(st["gk"] != 0)
+    check("knots", range(16), st["gk"])
+
+
+def test_profile_gradients():
+    print("diff knots 64^3: every 8th x-profile value", flush=True)
+    st = state()
+    assert np.any(st["gp"] != 0)
+    check("profile", range(0, 64, 8), st["gp"])
+    st["run"].close()
+    st["sim"].close()
