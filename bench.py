@@ -289,9 +289,11 @@ def run_adjoint(args) -> dict:
     the differentiated forward and the backward (recompute + adjoint
     kernels), per step, over the same steps; the FP32 objective against the
     FP64 one.  The FP32 gradients are compared with the FP64 ones over the
-    first 200 steps: beyond that (the pulse on the 20 n_c target) the
-    gradients grow 3000-fold and two FP64 runs from states 1e-7 apart
-    already differ by order one, so only the objective is compared at 400."""
+    first 200 steps and over the full run; at the full run they are shown
+    beside the spread between two FP64 runs whose initial positions and
+    momenta differ only by rounding to fp32 (about 6e-8): once the pulse is
+    on the 20 n_c target the gradients grow 3000-fold and that spread is
+    itself of order one, the bound no precision can beat."""
     from pic3d_amd import Simulation, workloads as W
     from pic3d_amd._abi import PIC_FP32, PIC_FP64
     from pic3d_amd.diff import DiffRun
@@ -299,11 +301,38 @@ def run_adjoint(args) -> dict:
     nsteps, K = args.adjoint_steps, 40
     out = {"workload": "diff_ramp 128x64x64, mem 4, 8 ppc on average (4.17M e-), a0 5",
            "steps": nsteps, "checkpoint_every": K}
-    for name, prec in (("fp64", PIC_FP64), ("fp32", PIC_FP32)):
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    def xprofile(run, gw, nx):
+        """d objective / d x-profile: slot gradients times weights, by initial x cell"""
+        s = run.initial_slots()
+        live = np.isfinite(s["w"])
+        ix = np.clip(np.floor(s["x"][live]).astype(np.int64), 0, nx - 1)
+        return np.bincount(ix, weights=(gw * s["w"])[live], minlength=nx)
+
+    full = {}
+    for name, prec in (("fp64", PIC_FP64), ("fp32", PIC_FP32), ("fp64_rounded", PIC_FP64)):
         wl = W.diff_ramp(precision=prec)
         sim = Simulation(wl.params)
         setup(sim, wl, device_loader=True)
+        if name == "fp64_rounded":
+            # the same initial state with positions (in-cell offsets) and momenta
+            # rounded to fp32: a perturbation of about 6e-8
+            pa = sim.particles_get(0)
+            for a in ("x", "y", "z"):
+                c = np.floor(pa[a])
+                pa[a] = c + (pa[a] - c).astype(np.float32).astype(np.float64)
+            for a in ("ux", "uy", "uz"):
+                pa[a] = pa[a].astype(np.float32).astype(np.float64)
+            sim.particles_set(0, pa)
         run = DiffRun(sim, 0, first_step=1, nsteps=nsteps, checkpoint_every=K)
+        if name == "fp64_rounded":
+            obj = run.forward()
+            ge, gw = run.backward()
+            full[name] = (obj, ge, xprofile(run, gw, wl.params.nx))
+            run.close()
+            sim.close()
+            continue
         sim.sync()
         t = time.perf_counter()
         sim.step(1, nsteps)  # the plain forward over the same steps
@@ -318,8 +347,17 @@ def run_adjoint(args) -> dict:
         out[name] = {"objective": obj, "forward_ms_per_step": fwd * 1e3,
                      "diff_forward_ms_per_step": tf * 1e3, "backward_ms_per_step": tb * 1e3,
                      "backward_over_forward": tb / fwd}
+        full[name] = (obj, ge, xprofile(run, gw, wl.params.nx))
         run.close()
         sim.close()
+    # at the full length: FP32 against FP64, beside two FP64 runs 6e-8 apart
+    # (the gradient's own sensitivity to a rounding-sized change of the state)
+    cmp = lambda a, b: {"objective": abs(a[0] - b[0]) / abs(b[0]), "d_objective_d_envelope": rel(a[1], b[1]),
+                        "d_objective_d_x_profile": rel(a[2], b[2])}
+    out["fp32_vs_fp64_at_%d_steps" % nsteps] = cmp(full["fp32"], full["fp64"])
+    out["fp64_spread_at_%d_steps" % nsteps] = dict(
+        cmp(full["fp64_rounded"], full["fp64"]),
+        perturbation="positions and momenta rounded to fp32 (about 6e-8 relative)")
     out["fp32_objective_rel_err"] = abs(out["fp32"]["objective"] - out["fp64"]["objective"]) / \
         abs(out["fp64"]["objective"])
     # FP32 against FP64 gradients where they are well conditioned
@@ -339,7 +377,6 @@ def run_adjoint(args) -> dict:
         g[name] = (obj, ge, prof)
         run.close()
         sim.close()
-    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     out["fp32_vs_fp64_at_%d_steps" % nchk] = {
         "objective": abs(g["fp32"][0] - g["fp64"][0]) / abs(g["fp64"][0]),
         "d_objective_d_envelope": rel(g["fp32"][1], g["fp64"][1]),
