@@ -215,8 +215,12 @@ def run_gpu(args, precision: str) -> dict:
         cold = {"steps": [4, 3 + args.steps], "ms_per_step": 1e3 * dtc / args.steps,
                 "pushes_per_s": args.steps * 0.5 * (c0 + c1) / dtc}
         done = 3 + args.steps
-    if args.warmup > done:
-        sim.step(1 + done, args.warmup - done)
+    while args.warmup > done:  # the rest of the warm-up, with a progress line every 100 steps
+        n = min(100, args.warmup - done)
+        sim.step(1 + done, n)
+        done += n
+        sim.sync()
+        print(f"# {precision} {args.workload}: {done} warm-up steps", file=sys.stderr, flush=True)
     sim.sync()
     n0 = sim.num_particles(0)
     sim.set_option(PIC_OPT_TIMING, 1)
