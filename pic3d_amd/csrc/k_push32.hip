@@ -144,7 +144,8 @@ __device__ __forceinline__ void boris32(float* u, const float* E, const float* B
 // Esirkepov deposit (fp32) in the frame of the particle's old cell: old
 // offsets f0 in [0,1), new offsets f1 = f0 + d with |d| < 1.  The sink D
 // receives D.base(bz, by, bx) (box origin relative to the cell) and then
-// D(comp, k, j, i, v) with box-relative (k, j, i).  The box is always
+// D.put(comp, k, j, i, a, b) with box-relative (k, j, i): the contribution
+// a * b, with a carrying D.scale(comp).  The box is always
 // L = M+2 points per axis.  The last point of the particle's box along the
 // cumulative axis (box point L-1 if the particle changes node along that
 // axis, else L-2) carries sum(S1) - sum(S0), zero in exact arithmetic, and
@@ -181,7 +182,23 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
     place(S0[0], d0[0], X0); place(S1[0], d1[0], X1);
     place(S0[1], d0[1], Y0); place(S1[1], d1[1], Y1);
     place(S0[2], d0[2], Z0); place(S1[2], d1[2], Z1);
-    const float cx = -qw * K.cjx, cy = -qw * K.cjy, cz = -qw * K.cjz;
+    // cumulative sums along each axis times the current scale (with the
+    // sink's scale folded in): a contribution is CX[i] * W_yz
+    float CX[L - 1], CY[L - 1], CZ[L - 1];
+    {
+        const float sx = -qw * K.cjx * dst.scale(0), sy = -qw * K.cjy * dst.scale(1),
+                    sz = -qw * K.cjz * dst.scale(2);
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+        for (int t = 0; t < L - 1; t++) {
+            ax += X1[t] - X0[t];
+            ay += Y1[t] - Y0[t];
+            az += Z1[t] - Z0[t];
+            CX[t] = sx * ax;
+            CY[t] = sy * ay;
+            CZ[t] = sz * az;
+        }
+    }
     const float third = 1.0f / 3.0f;
     dst.base(b[2], b[1], b[0]);
     float za[L], zb[L];
@@ -194,14 +211,11 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
 #pragma unroll
         for (int j = 0; j < L; j++) {
             const float sy0 = Y0[j], dsy = Y1[j] - sy0;
-            const float wyz = cx * (sy0 * sz0 + 0.5f * dsy * sz0 + 0.5f * sy0 * dsz + third * dsy * dsz);
+            const float wyz = sy0 * sz0 + 0.5f * dsy * sz0 + 0.5f * sy0 * dsz + third * dsy * dsz;
             if (wyz == 0.0f) continue;  // a plane outside both supports: only zeros to add
-            float acc = 0.0f;
 #pragma unroll
-            for (int i = 0; i < L - 1; i++) {
-                acc += X1[i] - X0[i];
-                if (i < L - 2 || moved[0]) dst(0, k, j, i, acc * wyz);
-            }
+            for (int i = 0; i < L - 1; i++)
+                if (i < L - 2 || moved[0]) dst.put(0, k, j, i, CX[i], wyz);
         }
         shift_down32<L>(za); shift_down32<L>(zb);
     }
@@ -214,14 +228,11 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
 #pragma unroll
         for (int i = 0; i < L; i++) {
             const float sx0 = X0[i], dsx = X1[i] - sx0;
-            const float wxz = cy * (sx0 * sz0 + 0.5f * dsx * sz0 + 0.5f * sx0 * dsz + third * dsx * dsz);
+            const float wxz = sx0 * sz0 + 0.5f * dsx * sz0 + 0.5f * sx0 * dsz + third * dsx * dsz;
             if (wxz == 0.0f) continue;
-            float acc = 0.0f;
 #pragma unroll
-            for (int j = 0; j < L - 1; j++) {
-                acc += Y1[j] - Y0[j];
-                if (j < L - 2 || moved[1]) dst(1, k, j, i, acc * wxz);
-            }
+            for (int j = 0; j < L - 1; j++)
+                if (j < L - 2 || moved[1]) dst.put(1, k, j, i, CY[j], wxz);
         }
         shift_down32<L>(za); shift_down32<L>(zb);
     }
@@ -235,60 +246,71 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
 #pragma unroll
         for (int i = 0; i < L; i++) {
             const float sx0 = X0[i], dsx = X1[i] - sx0;
-            const float wxy = cz * (sx0 * sy0 + 0.5f * dsx * sy0 + 0.5f * sx0 * dsy + third * dsx * dsy);
+            const float wxy = sx0 * sy0 + 0.5f * dsx * sy0 + 0.5f * sx0 * dsy + third * dsx * dsy;
             if (wxy == 0.0f) continue;
-            float acc = 0.0f;
 #pragma unroll
-            for (int k = 0; k < L - 1; k++) {
-                acc += Z1[k] - Z0[k];
-                if (k < L - 2 || moved[2]) dst(2, k, j, i, acc * wxy);
-            }
+            for (int k = 0; k < L - 1; k++)
+                if (k < L - 2 || moved[2]) dst.put(2, k, j, i, CZ[k], wxy);
         }
         shift_down32<L>(ya); shift_down32<L>(yb);
     }
     // rho at the new position: nodes i2-H .. i2+H
     dst.base(b[2] + d1[2], b[1] + d1[1], b[0] + d1[0]);
     float zr[M + 1];
+    const float qs = qw * dst.scale(3);
 #pragma unroll
-    for (int t = 0; t <= M; t++) zr[t] = qw * S1[2][t];
+    for (int t = 0; t <= M; t++) zr[t] = qs * S1[2][t];
 #pragma unroll(M == 2 ? M + 1 : 1)
     for (int k = 0; k <= M; k++) {
 #pragma unroll
         for (int j = 0; j <= M; j++) {
             const float syz = zr[0] * S1[1][j];
 #pragma unroll
-            for (int i = 0; i <= M; i++) dst(3, k, j, i, syz * S1[0][i]);
+            for (int i = 0; i <= M; i++) dst.put(3, k, j, i, syz, S1[0][i]);
         }
         shift_down32<M + 1>(zr);
     }
 }
 
 // deposit sinks (fp32) ---------------------------------------------------
+// Sinks take each contribution as a product a * b, a carrying the sink's
+// scale(c).
 struct HbmSink32 {  // global atomics; indices relative to the cell (Cz, Cy, Cx)
     float* jp[4];
     Grid g;
     int Cz, Cy, Cx;
     int bz, by, bx;
+    __device__ __forceinline__ float scale(int) const { return 1.0f; }
     __device__ __forceinline__ void base(int z, int y, int x) { bz = Cz + z; by = Cy + y; bx = Cx + x; }
-    __device__ __forceinline__ void operator()(int c, int k, int j, int i, float v) {
-        hbm_add(&jp[c][g.ji(bz + k, by + j, bx + i)], v);
+    __device__ __forceinline__ void put(int c, int k, int j, int i, float a, float b) {
+        hbm_add(&jp[c][g.ji(bz + k, by + j, bx + i)], a * b);
     }
 };
 
 // LDS accumulator box (ACC = double: FP64 sums; ACC = int: fixed point in
-// units ufix[c]); 'cell' = the particle's cell offset in the box.
+// units ufix[c]); 'cell' = the particle's cell offset in the box.  Fixed
+// point: a carries 1/ufix[c], |a * b| <= 2^18 units, and one fused
+// a * b + 1.5 * 2^23 rounds the product to the nearest integer in the low
+// mantissa bits; subtracting that constant's bit pattern leaves the
+// two's-complement integer (valid below 2^22 units).
 template <int AX, int AY, int AVOL, typename ACC>
 struct LdsSink32 {
     ACC* acc;
     const float* inv_u;
     int cell;
     int off;
+    __device__ __forceinline__ float scale(int c) const {
+        if constexpr (sizeof(ACC) == 8) return 1.0f;
+        else return inv_u[c];
+    }
     __device__ __forceinline__ void base(int z, int y, int x) { off = cell + (z * AY + y) * AX + x; }
-    __device__ __forceinline__ void operator()(int c, int k, int j, int i, float v) {
-        if constexpr (sizeof(ACC) == 8)
-            lds_add((double*)&acc[off + c * AVOL + (k * AY + j) * AX + i], (double)v);
-        else
-            lds_add_i32((int*)&acc[off + c * AVOL + (k * AY + j) * AX + i], __float2int_rn(v * inv_u[c]));
+    __device__ __forceinline__ void put(int c, int k, int j, int i, float a, float b) {
+        if constexpr (sizeof(ACC) == 8) {
+            lds_add((double*)&acc[off + c * AVOL + (k * AY + j) * AX + i], (double)(a * b));
+        } else {
+            const float t = __builtin_fmaf(a, b, 0x1.8p23f);
+            lds_add_i32((int*)&acc[off + c * AVOL + (k * AY + j) * AX + i], __float_as_int(t) - 0x4B400000);
+        }
     }
 };
 

@@ -123,8 +123,12 @@ __device__ __forceinline__ int spin_step(const P64& P, int64_t p, const double* 
 
 // Esirkepov deposit of one particle moving (xo,yo,zo) -> (xn,yn,zn)
 // (particle.cpp:381-514).  The sink D receives D.base(bz, by, bx) once
-// (the box origin, buffer indices) and then D(comp, k, j, i, v) with
-// box-relative compile-time (k, j, i); comp 0..2 = J, 3 = rho.
+// (the box origin, buffer indices) and then D.put(comp, k, j, i, a, b) with
+// box-relative compile-time (k, j, i); comp 0..2 = J, 3 = rho.  The
+// contribution is the product a * b, where a already carries the sink's
+// scale D.scale(comp) (1 for a sink that adds values, 1/unit for one that
+// adds fixed-point integers): with scale 1 every product is the
+// reference's term bit for bit.
 //
 // The reference walks a box [min(i1,i2)-H, max(i1,i2)+H] per axis (M+1 or
 // M+2 points).  Here the box is always L = M+2 points with the old/new
@@ -170,6 +174,22 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
     place<M>(Sy0, j1 - H - by, Y0); place<M>(Sy1, j2 - H - by, Y1);
     place<M>(Sz0, k1 - H - bz, Z0); place<M>(Sz1, k2 - H - bz, Z1);
     const double cx = qw * K.dx / K.dt, cy = qw * K.dy / K.dt, cz = qw * K.dz / K.dt;
+    // the cumulative sums along each axis (the reference's order) times the
+    // current scale: the reference's term -(c * acc * w) is (-(c * acc)) * w
+    double CX[L - 1], CY[L - 1], CZ[L - 1];
+    {
+        const double sx = cx * dst.scale(0), sy = cy * dst.scale(1), sz = cz * dst.scale(2);
+        double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+        for (int t = 0; t < L - 1; t++) {
+            ax += X1[t] - X0[t];
+            ay += Y1[t] - Y0[t];
+            az += Z1[t] - Z0[t];
+            CX[t] = -(sx * ax);
+            CY[t] = -(sy * ay);
+            CZ[t] = -(sz * az);
+        }
+    }
     dst.base(bz, by, bx);
     // Jx: cumulative sum along x, weights W_yz (particle.cpp:427-450)
     double za[L], zb[L];
@@ -183,12 +203,9 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
             const double sy0 = Y0[j], dsy = Y1[j] - sy0;
             const double wyz = sy0 * sz0 + 0.5 * dsy * sz0 + 0.5 * sy0 * dsz + (1.0 / 3.0) * dsy * dsz;
             if (wyz == 0.0) continue;  // a plane outside both supports: only +-0 to add
-            double acc = 0.0;
 #pragma unroll
-            for (int i = 0; i < L - 1; i++) {
-                acc += X1[i] - X0[i];
-                if (i < L - 2 || crx) dst(0, k, j, i, -(cx * acc * wyz));
-            }
+            for (int i = 0; i < L - 1; i++)
+                if (i < L - 2 || crx) dst.put(0, k, j, i, CX[i], wyz);
         }
     }
     // Jy (particle.cpp:452-475)
@@ -202,12 +219,9 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
             const double sx0 = X0[i], dsx = X1[i] - sx0;
             const double wxz = sx0 * sz0 + 0.5 * dsx * sz0 + 0.5 * sx0 * dsz + (1.0 / 3.0) * dsx * dsz;
             if (wxz == 0.0) continue;
-            double acc = 0.0;
 #pragma unroll
-            for (int j = 0; j < L - 1; j++) {
-                acc += Y1[j] - Y0[j];
-                if (j < L - 2 || cry) dst(1, k, j, i, -(cy * acc * wxz));
-            }
+            for (int j = 0; j < L - 1; j++)
+                if (j < L - 2 || cry) dst.put(1, k, j, i, CY[j], wxz);
         }
     }
     // Jz (particle.cpp:477-500)
@@ -222,19 +236,20 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
             const double sx0 = X0[i], dsx = X1[i] - sx0;
             const double wxy = sx0 * sy0 + 0.5 * dsx * sy0 + 0.5 * sx0 * dsy + (1.0 / 3.0) * dsx * dsy;
             if (wxy == 0.0) continue;
-            double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < L - 1; k++) {
-                acc += Z1[k] - Z0[k];
-                if (k < L - 2 || crz) dst(2, k, j, i, -(cz * acc * wxy));
-            }
+            for (int k = 0; k < L - 1; k++)
+                if (k < L - 2 || crz) dst.put(2, k, j, i, CZ[k], wxy);
         }
     }
     // rho at the new position (particle.cpp:502-514): nodes i2-H .. i2+H.
     // The reference skips planes whose weight is exactly zero; adding +-0
     // there changes nothing, so every node is deposited.
+    // The term is ((qw * Sx) * Sy) * Sz.
     dst.base(k2 - H, j2 - H, i2 - H);
-    double zr[M + 1];
+    double rx[M + 1], zr[M + 1];
+    const double qs = qw * dst.scale(3);
+#pragma unroll
+    for (int i = 0; i <= M; i++) rx[i] = qs * Sx1[i];
     copy_arr<M + 1>(Sz1, zr);
 #pragma unroll 1
     for (int k = 0; k <= M; k++) {
@@ -243,7 +258,7 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
 #pragma unroll
         for (int j = 0; j <= M; j++)
 #pragma unroll
-            for (int i = 0; i <= M; i++) dst(3, k, j, i, qw * Sx1[i] * Sy1[j] * sz);
+            for (int i = 0; i <= M; i++) dst.put(3, k, j, i, rx[i] * Sy1[j], sz);
     }
 }
 
@@ -261,28 +276,32 @@ struct HbmSink {  // global FP64 atomics on the ghosted J/rho arrays
     double* jp[4];
     Grid g;
     int bz, by, bx;
+    __device__ __forceinline__ double scale(int) const { return 1.0; }
     __device__ __forceinline__ void base(int z, int y, int x) { bz = z; by = y; bx = x; }
-    __device__ __forceinline__ void operator()(int c, int k, int j, int i, double v) {
-        hbm_add(&jp[c][g.ji(bz + k, by + j, bx + i)], v);
+    __device__ __forceinline__ void put(int c, int k, int j, int i, double a, double b) {
+        hbm_add(&jp[c][g.ji(bz + k, by + j, bx + i)], a * b);
     }
 };
 
 // LDS accumulator box with origin (oz, oy, ox), int64 fixed point: each
-// contribution v becomes round(v / unit[c]) (|v| <= 2^48 units, so adding
-// 1.5 * 2^52 puts that integer in the low mantissa bits exactly), added with
-// ds_add_u64 (measured about 20 % cheaper per LDS instruction than
-// ds_add_f64).  A chunk of at most 2^12 particles sums to below 2^60 units.
+// contribution a * b (a scaled by 1/unit[c]) becomes the nearest integer:
+// |a * b| <= 2^48 units, so one fused a * b + 1.5 * 2^52 rounds it into the
+// low mantissa bits, and subtracting that constant's bit pattern leaves the
+// two's-complement integer.  Added with ds_add_u64 (measured about 20 %
+// cheaper per LDS instruction than ds_add_f64); a chunk of at most 2^12
+// particles sums to below 2^60 units.
 template <int AX, int AY, int VOL>
 struct LdsSink {
     unsigned long long* acc;
     const double* inv_unit;
     int oz, oy, ox;
     int off;
+    __device__ __forceinline__ double scale(int c) const { return inv_unit[c]; }
     __device__ __forceinline__ void base(int z, int y, int x) {
         off = ((z - oz) * AY + (y - oy)) * AX + (x - ox);
     }
-    __device__ __forceinline__ void operator()(int c, int k, int j, int i, double v) {
-        const double t = v * inv_unit[c] + 0x1.8p52;
+    __device__ __forceinline__ void put(int c, int k, int j, int i, double a, double b) {
+        const double t = __builtin_fma(a, b, 0x1.8p52);
         const unsigned long long q =
             (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull;  // bits of 1.5 * 2^52
         lds_add_u64(&acc[off + c * VOL + (k * AY + j) * AX + i], q);
