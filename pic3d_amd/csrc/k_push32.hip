@@ -199,19 +199,29 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
             CZ[t] = sz * az;
         }
     }
+    // transverse row weights s0 t0 + ds t0 / 2 + s0 dt / 2 + ds dt / 3
+    // regrouped as s0 * (t0 + dt / 2) + ds * (t0 / 2 + dt / 3), the two
+    // factors precomputed per axis point
     const float third = 1.0f / 3.0f;
+    float PX[L], QX[L], PY[L], QY[L], DY[L], DZ[L];
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        const float dx_ = X1[t] - X0[t], dy_ = Y1[t] - Y0[t], dz_ = Z1[t] - Z0[t];
+        PX[t] = X0[t] + 0.5f * dx_; QX[t] = 0.5f * X0[t] + third * dx_;
+        PY[t] = Y0[t] + 0.5f * dy_; QY[t] = 0.5f * Y0[t] + third * dy_;
+        DY[t] = dy_; DZ[t] = dz_;
+    }
     dst.base(b[2], b[1], b[0]);
     float za[L], zb[L];
 #pragma unroll
-    for (int t = 0; t < L; t++) { za[t] = Z0[t]; zb[t] = Z1[t] - Z0[t]; }
+    for (int t = 0; t < L; t++) { za[t] = Z0[t]; zb[t] = DZ[t]; }
     // Jx: cumulative along x, weights W_yz
 #pragma unroll(R)
     for (int k = 0; k < L; k++) {
         const float sz0 = za[0], dsz = zb[0];
 #pragma unroll
         for (int j = 0; j < L; j++) {
-            const float sy0 = Y0[j], dsy = Y1[j] - sy0;
-            const float wyz = sy0 * sz0 + 0.5f * dsy * sz0 + 0.5f * sy0 * dsz + third * dsy * dsz;
+            const float wyz = __builtin_fmaf(dsz, QY[j], sz0 * PY[j]);
             if (wyz == 0.0f) continue;  // a plane outside both supports: only zeros to add
 #pragma unroll
             for (int i = 0; i < L - 1; i++)
@@ -221,14 +231,13 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
     }
     // Jy: cumulative along y, weights W_xz
 #pragma unroll
-    for (int t = 0; t < L; t++) { za[t] = Z0[t]; zb[t] = Z1[t] - Z0[t]; }
+    for (int t = 0; t < L; t++) { za[t] = Z0[t]; zb[t] = DZ[t]; }
 #pragma unroll(R)
     for (int k = 0; k < L; k++) {
         const float sz0 = za[0], dsz = zb[0];
 #pragma unroll
         for (int i = 0; i < L; i++) {
-            const float sx0 = X0[i], dsx = X1[i] - sx0;
-            const float wxz = sx0 * sz0 + 0.5f * dsx * sz0 + 0.5f * sx0 * dsz + third * dsx * dsz;
+            const float wxz = __builtin_fmaf(dsz, QX[i], sz0 * PX[i]);
             if (wxz == 0.0f) continue;
 #pragma unroll
             for (int j = 0; j < L - 1; j++)
@@ -239,14 +248,13 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
     // Jz: cumulative along z, weights W_xy
     float ya[L], yb[L];
 #pragma unroll
-    for (int t = 0; t < L; t++) { ya[t] = Y0[t]; yb[t] = Y1[t] - Y0[t]; }
+    for (int t = 0; t < L; t++) { ya[t] = Y0[t]; yb[t] = DY[t]; }
 #pragma unroll(R)
     for (int j = 0; j < L; j++) {
         const float sy0 = ya[0], dsy = yb[0];
 #pragma unroll
         for (int i = 0; i < L; i++) {
-            const float sx0 = X0[i], dsx = X1[i] - sx0;
-            const float wxy = sx0 * sy0 + 0.5f * dsx * sy0 + 0.5f * sx0 * dsy + third * dsx * dsy;
+            const float wxy = __builtin_fmaf(dsy, QX[i], sy0 * PX[i]);
             if (wxy == 0.0f) continue;
 #pragma unroll
             for (int k = 0; k < L - 1; k++)

@@ -127,16 +127,17 @@ __device__ __forceinline__ int spin_step(const P64& P, int64_t p, const double* 
 // box-relative compile-time (k, j, i); comp 0..2 = J, 3 = rho.  The
 // contribution is the product a * b, where a already carries the sink's
 // scale D.scale(comp) (1 for a sink that adds values, 1/unit for one that
-// adds fixed-point integers): with scale 1 every product is the
-// reference's term bit for bit.
+// adds fixed-point integers).
 //
 // The reference walks a box [min(i1,i2)-H, max(i1,i2)+H] per axis (M+1 or
 // M+2 points).  Here the box is always L = M+2 points with the old/new
 // shape weights placed at compile-time offsets (selected by d0/d1 in {0,1}),
 // so the loops unroll into registers with immediate LDS offsets.  Callers
-// guarantee |x_new - x_old| < 1 cell per axis, hence |i2 - i1| <= 1.  Values
-// and the order of each cumulative sum are the reference's, so every term
-// below is the reference's bit for bit.  Points of the L-box skipped:
+// guarantee |x_new - x_old| < 1 cell per axis, hence |i2 - i1| <= 1.  The
+// shape weights and the order of each cumulative sum are the reference's;
+// the transverse row weights are regrouped (below), so a term can differ
+// from the reference's in its last bits (J and rho are compared at 1e-12:
+// the order of the atomic adds differs anyway).  Points of the L-box skipped:
 // (a) planes outside the reference's box along the non-cumulative axes,
 // whose weights are exactly zero (adding +-0 changes no value), and (b) the
 // last point of the reference's box along the cumulative axis (box point
@@ -190,18 +191,31 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
             CZ[t] = -(sz * az);
         }
     }
+    // Transverse row weights: the reference's s0 t0 + ds t0 / 2 + s0 dt / 2 +
+    // ds dt / 3 (particle.cpp:432-436, s along one transverse axis, t along
+    // the other, ds and dt the new-minus-old differences) regrouped as
+    // s0 * (t0 + dt / 2) + ds * (t0 / 2 + dt / 3), the two factors in
+    // parentheses precomputed per axis point (P, Q): two operations per row
+    // instead of ten.  A row outside both supports still gets exactly 0.
+    double PX[L], QX[L], PY[L], QY[L], DY[L], DZ[L];
+#pragma unroll
+    for (int t = 0; t < L; t++) {
+        const double dx_ = X1[t] - X0[t], dy_ = Y1[t] - Y0[t], dz_ = Z1[t] - Z0[t];
+        PX[t] = X0[t] + 0.5 * dx_; QX[t] = 0.5 * X0[t] + (1.0 / 3.0) * dx_;
+        PY[t] = Y0[t] + 0.5 * dy_; QY[t] = 0.5 * Y0[t] + (1.0 / 3.0) * dy_;
+        DY[t] = dy_; DZ[t] = dz_;
+    }
     dst.base(bz, by, bx);
     // Jx: cumulative sum along x, weights W_yz (particle.cpp:427-450)
     double za[L], zb[L];
-    copy_arr<L>(Z0, za); copy_arr<L>(Z1, zb);
+    copy_arr<L>(Z0, za); copy_arr<L>(DZ, zb);
 #pragma unroll 1
     for (int k = 0; k < L; k++) {
-        const double sz0 = za[0], dsz = zb[0] - sz0;
+        const double sz0 = za[0], dsz = zb[0];
         shift_down<L>(za); shift_down<L>(zb);
 #pragma unroll
         for (int j = 0; j < L; j++) {
-            const double sy0 = Y0[j], dsy = Y1[j] - sy0;
-            const double wyz = sy0 * sz0 + 0.5 * dsy * sz0 + 0.5 * sy0 * dsz + (1.0 / 3.0) * dsy * dsz;
+            const double wyz = __builtin_fma(dsz, QY[j], sz0 * PY[j]);
             if (wyz == 0.0) continue;  // a plane outside both supports: only +-0 to add
 #pragma unroll
             for (int i = 0; i < L - 1; i++)
@@ -209,15 +223,14 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
         }
     }
     // Jy (particle.cpp:452-475)
-    copy_arr<L>(Z0, za); copy_arr<L>(Z1, zb);
+    copy_arr<L>(Z0, za); copy_arr<L>(DZ, zb);
 #pragma unroll 1
     for (int k = 0; k < L; k++) {
-        const double sz0 = za[0], dsz = zb[0] - sz0;
+        const double sz0 = za[0], dsz = zb[0];
         shift_down<L>(za); shift_down<L>(zb);
 #pragma unroll
         for (int i = 0; i < L; i++) {
-            const double sx0 = X0[i], dsx = X1[i] - sx0;
-            const double wxz = sx0 * sz0 + 0.5 * dsx * sz0 + 0.5 * sx0 * dsz + (1.0 / 3.0) * dsx * dsz;
+            const double wxz = __builtin_fma(dsz, QX[i], sz0 * PX[i]);
             if (wxz == 0.0) continue;
 #pragma unroll
             for (int j = 0; j < L - 1; j++)
@@ -226,15 +239,14 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
     }
     // Jz (particle.cpp:477-500)
     double ya[L], yb[L];
-    copy_arr<L>(Y0, ya); copy_arr<L>(Y1, yb);
+    copy_arr<L>(Y0, ya); copy_arr<L>(DY, yb);
 #pragma unroll 1
     for (int j = 0; j < L; j++) {
-        const double sy0 = ya[0], dsy = yb[0] - sy0;
+        const double sy0 = ya[0], dsy = yb[0];
         shift_down<L>(ya); shift_down<L>(yb);
 #pragma unroll
         for (int i = 0; i < L; i++) {
-            const double sx0 = X0[i], dsx = X1[i] - sx0;
-            const double wxy = sx0 * sy0 + 0.5 * dsx * sy0 + 0.5 * sx0 * dsy + (1.0 / 3.0) * dsx * dsy;
+            const double wxy = __builtin_fma(dsy, QX[i], sy0 * PX[i]);
             if (wxy == 0.0) continue;
 #pragma unroll
             for (int k = 0; k < L - 1; k++)
