@@ -302,7 +302,7 @@ struct HbmSink {  // global FP64 atomics on the ghosted J/rho arrays
 // two's-complement integer.  Added with ds_add_u64 (measured about 20 %
 // cheaper per LDS instruction than ds_add_f64); a chunk of at most 2^12
 // particles sums to below 2^60 units.
-template <int AX, int AY, int VOL>
+template <typename BOX>
 struct LdsSink {
     unsigned long long* acc;
     const double* inv_unit;
@@ -310,13 +310,13 @@ struct LdsSink {
     int off;
     __device__ __forceinline__ double scale(int c) const { return inv_unit[c]; }
     __device__ __forceinline__ void base(int z, int y, int x) {
-        off = ((z - oz) * AY + (y - oy)) * AX + (x - ox);
+        off = (z - oz) * BOX::PXY + (y - oy) * BOX::PX + (x - ox);
     }
     __device__ __forceinline__ void put(int c, int k, int j, int i, double a, double b) {
         const double t = __builtin_fma(a, b, 0x1.8p52);
         const unsigned long long q =
             (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull;  // bits of 1.5 * 2^52
-        lds_add_u64(&acc[off + c * VOL + (k * AY + j) * AX + i], q);
+        lds_add_u64(&acc[off + BOX::aoff(c) + k * BOX::PXY + j * BOX::PX + i], q);
     }
 };
 
@@ -404,24 +404,49 @@ __global__ __launch_bounds__(256) void k_push64_direct(P64 P, Fields64 F, Grid g
 // edges (tx, ty, tz): buffer indices [T0 + m - 2 - H, T0 + t - 1 + m + 1 + H],
 // i.e. t + m + 3 points per axis (particles move < 1 cell per step).
 //
-// Accumulator rows are stored with an x pitch of 12 for 4-wide tiles: the
-// particles of a wave sit in consecutive cells of the tile (rank order
-// within a chunk), and LDS serves 16 lanes of an instruction at a time;
-// with pitch 12 the 16 cells of a 4 x 4 face fall on distinct banks (pitch
-// 11 maps three pairs of them to one bank).  Two workgroups of the quartic
-// kernel still fit in a CU's LDS.
+// LDS bank layout.  LDS serves the 8-byte reads and atomics of 16 lanes at
+// a time over 16 bank pairs (8-byte word index mod 16).  Each wave's
+// particles are dealt over its lanes so that the 16 lanes of a group have
+// deposit bases on distinct bank pairs (bank_spread_lane).  The field tile
+// uses row and plane pitches congruent mod 16 to the accumulator box's, so
+// the same dealing also puts the gather's primal bases (the deposit base
+// for a particle that stays on its nodes) on distinct bank pairs; only the
+// half-cell shifts of the staggered components remain.  Each component's
+// field box is one point shorter along its staggered (dual) axes, each
+// current box one point shorter along its cumulative axis, and boxes are
+// packed back to back, so two workgroups of the quartic kernel fit a CU.
+__host__ __device__ constexpr int pitch16(int at_least, int residue) {
+    return at_least + ((residue - at_least) % 16 + 16) % 16;
+}
 template <int M, int TX, int TY, int TZ>
 struct AccBox {
     static constexpr int H = M / 2;
+    // accumulator box: tile + deposit halo, x pitch 1, row pitch BX, plane pitch BX * BY
     static constexpr int BX = TX + M + 3, BY = TY + M + 3, BZ = TZ + M + 3;
-    static constexpr int PX = (TX == 4 && BX <= 12) ? 12 : BX;  // x pitch
-    static constexpr int VOL = PX * BY * BZ;
-    // field tile: buffer indices [T0 + m - 1 - H, T0 + T + m - 1 + H]
+    static constexpr int PX = BX, PXY = BX * BY;
+    // extents per component (Jx, Jy, Jz one shorter along their cumulative axis; rho)
+    static constexpr int asize(int c) {
+        return (BZ - (c == 2) - 1) * PXY + (BY - (c == 1) - 1) * PX + (BX - (c == 0));
+    }
+    static constexpr int aoff(int c) { return c == 0 ? 0 : aoff(c - 1) + asize(c - 1); }
+    static constexpr int AVOL = aoff(4);
+    // field tile: buffer indices [T0 + m - 1 - H, T0 + T + m - 1 + H] on a
+    // primal axis, one fewer on a dual axis.  Staggering (x, y, z) per
+    // component: Ex (d,p,p) Ey (p,d,p) Ez (p,p,d) Bx (p,d,d) By (d,p,d) Bz (d,d,p)
     static constexpr int FX = TX + M + 1, FY = TY + M + 1, FZ = TZ + M + 1;
-    static constexpr int FVOL = FX * FY * FZ;
-    static constexpr size_t LDS = (size_t)(6 * FVOL + 4 * VOL) * sizeof(double);
+    static constexpr int dual(int c, int axis) {
+        return (c == 0 && axis == 0) || (c == 1 && axis == 1) || (c == 2 && axis == 2) ||
+               (c == 3 && axis != 0) || (c == 4 && axis != 1) || (c == 5 && axis != 2);
+    }
+    static constexpr int fext(int c, int axis) { return (axis == 0 ? FX : axis == 1 ? FY : FZ) - dual(c, axis); }
+    static constexpr int FPY = pitch16(FX, PX);                                   // row pitch
+    static constexpr int fpz(int c) { return pitch16(FPY * fext(c, 1), PXY); }  // plane pitch
+    static constexpr int fsize(int c) { return (fext(c, 2) - 1) * fpz(c) + (fext(c, 1) - 1) * FPY + fext(c, 0); }
+    static constexpr int foff(int c) { return c == 0 ? 0 : foff(c - 1) + fsize(c - 1); }
+    static constexpr int FVOL = foff(6);
+    static constexpr size_t LDS = (size_t)(FVOL + AVOL) * sizeof(double);
 };
-static_assert(2 * AccBox<4, 4, 4, 4>::LDS <= 160 * 1024, "two quartic workgroups per CU");
+static_assert(2 * (AccBox<4, 4, 4, 4>::LDS + 256) <= 160 * 1024, "two quartic workgroups per CU");
 
 template <int M, int TX, int TY, int TZ, bool SPIN>
 __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid g, PushConsts K,
@@ -430,8 +455,8 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
                                                       const int64_t* __restrict__ nwork, int chunk) {
     using B = AccBox<M, TX, TY, TZ>;
     extern __shared__ double lds64[];
-    double* fld = lds64;  // 6 * FVOL: the tile's fields
-    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds64 + 6 * B::FVOL);  // 4 * VOL: J, rho
+    double* fld = lds64;  // FVOL: the tile's fields
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds64 + B::FVOL);  // AVOL: J, rho
     const int64_t wi = blockIdx.x;
     if (wi >= *nwork) return;
     const int64_t item = work[wi];            // tile << 20 | chunk index
@@ -446,23 +471,25 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
     const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
     const double* fp[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
     const int fox = T0x + g.m - 1 - B::H, foy = T0y + g.m - 1 - B::H, foz = T0z + g.m - 1 - B::H;
-    for (int e = threadIdx.x; e < 6 * B::FVOL; e += blockDim.x) {
-        const int c = e / B::FVOL, r = e % B::FVOL;
-        const int lx = r % B::FX, ly = (r / B::FX) % B::FY, lz = r / (B::FX * B::FY);
-        // reads past a partial last tile are clamped (never used)
-        const int gx = min(fox + lx, g.fe[0] - 1), gy = min(foy + ly, g.fe[1] - 1), gz = min(foz + lz, g.fe[2] - 1);
-        fld[e] = fp[c][g.fi(gz, gy, gx)];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const int ex = B::fext(c, 0), ey = B::fext(c, 1), n = ex * ey * B::fext(c, 2);
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int lx = e % ex, ly = (e / ex) % ey, lz = e / (ex * ey);
+            // reads past a partial last tile are clamped (never used)
+            const int gx = min(fox + lx, g.fe[0] - 1), gy = min(foy + ly, g.fe[1] - 1), gz = min(foz + lz, g.fe[2] - 1);
+            fld[B::foff(c) + lz * B::fpz(c) + ly * B::FPY + lx] = fp[c][g.fi(gz, gy, gx)];
+        }
     }
-    for (int e = threadIdx.x; e < 4 * B::VOL; e += blockDim.x) acc[e] = 0ull;
+    for (int e = threadIdx.x; e < B::AVOL; e += blockDim.x) acc[e] = 0ull;
     __shared__ unsigned rg_hist[256 / 64][8];  // per wave: bank-spreading counters
     if (lane_id() < 8) rg_hist[threadIdx.x >> 6][lane_id()] = 0u;
     __syncthreads();
     // gather from the LDS field tile (an in-tile particle's stencil lies in it)
     auto fr = [&](int c, int k, int j, int i) -> double {
-        return fld[c * B::FVOL + ((k - foz) * B::FY + (j - foy)) * B::FX + (i - fox)];
+        return fld[B::foff(c) + (k - foz) * B::fpz(c) + (j - foy) * B::FPY + (i - fox)];
     };
-    LdsSink<B::PX, B::BY, B::VOL> sink{acc, K.inv_unit, T0z + g.m - 2 - B::H, T0y + g.m - 2 - B::H,
-                                      T0x + g.m - 2 - B::H, 0};
+    LdsSink<B> sink{acc, K.inv_unit, T0z + g.m - 2 - B::H, T0y + g.m - 2 - B::H, T0x + g.m - 2 - B::H, 0};
     int64_t oob = 0;
     int spin_bad = 0;
     uint32_t strays = 0;  // iterations whose particle was outside the tile
@@ -484,8 +511,16 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
             ok = wp == wp;
             if (ok && !wrap_check(x, y, z, K)) { oob++; ok = false; }
             if (ok) {
-                // cell after the wrap (x == N belongs to the last cell)
-                const int cx = min((int)x, g.n[0] - 1), cy = min((int)y, g.n[1] - 1), cz = min((int)z, g.n[2] - 1);
+                // cell after the wrap, from the shape functions' own argument
+                // x + m - 1 (rounded: x just below a cell edge can land on
+                // it), so that the dual stencil (which starts at its floor)
+                // stays in the tile's field box.  A particle exactly at
+                // x == N (kept by the strict wrap test) or rounded onto the
+                // next tile's first cell is sorted into this tile but takes
+                // the HBM pass like a particle outside its tile.
+                const double org = (double)(1 - g.m);
+                const int cx = (int)(x - org) - (g.m - 1), cy = (int)(y - org) - (g.m - 1),
+                          cz = (int)(z - org) - (g.m - 1);
                 if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
                     (unsigned)(cz - T0z) >= (unsigned)TZ) {
                     strays |= 1u << it;  // left its tile since the last sort: HBM pass below
@@ -498,7 +533,7 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
                 const double org = (double)(1 - g.m);
                 const int ib = (int)(x - org + 0.5) - B::H, jb = (int)(y - org + 0.5) - B::H,
                           kb = (int)(z - org + 0.5) - B::H;
-                const int ofs = ((kb - (T0z + g.m - 2 - B::H)) * B::BY + (jb - (T0y + g.m - 2 - B::H))) * B::PX +
+                const int ofs = (kb - (T0z + g.m - 2 - B::H)) * B::PXY + (jb - (T0y + g.m - 2 - B::H)) * B::PX +
                                 (ib - (T0x + g.m - 2 - B::H));
                 r = (2 * ofs) & 31;
             }
@@ -536,12 +571,13 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
     // flush: one global atomic per nonzero accumulator point
     double* jp[4] = {F.jx, F.jy, F.jz, F.rho};
     const int ox = T0x + g.m - 2 - B::H, oy = T0y + g.m - 2 - B::H, oz = T0z + g.m - 2 - B::H;
-    for (int e = threadIdx.x; e < 4 * B::VOL; e += blockDim.x) {
+    for (int e = threadIdx.x; e < B::AVOL; e += blockDim.x) {
         const long long q = (long long)acc[e];
         if (q == 0) continue;
-        const int c = e / B::VOL, r = e % B::VOL;
+        const int c = e >= B::aoff(3) ? 3 : e >= B::aoff(2) ? 2 : e >= B::aoff(1) ? 1 : 0;
+        const int r = e - (c == 3 ? B::aoff(3) : c == 2 ? B::aoff(2) : c == 1 ? B::aoff(1) : 0);
         const double v = (double)q * K.unit[c];
-        const int lx = r % B::PX, ly = (r / B::PX) % B::BY, lz = r / (B::PX * B::BY);
+        const int lx = r % B::PX, ly = (r / B::PX) % B::BY, lz = r / B::PXY;
         const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
         // a partial last tile's box reaches past the arrays; those points
         // are never deposited into, but keep the flush in bounds regardless
