@@ -399,21 +399,34 @@ struct Box32 {
     // nearest node), so it needs one point fewer there.  Bit a of dual(c):
     // component c (Ex, Ey, Ez, Bx, By, Bz) is dual along axis a.  The saving
     // (3.6 KB at the 8x4x4 tile) lets three workgroups share a CU: push
-    // 10.0 -> 9.4 ms at 41.9M electrons.  The quadratic kernel gains no
-    // workgroup from it and measured 1 % slower (its extra index
-    // arithmetic), so it keeps one box shape for all six.
+    // 10.0 -> 9.4 ms at 41.9M electrons.
+    //
+    // The quadratic fixed-point kernel (CONG) keeps one box shape for all
+    // six components (one base register: trimmed boxes took it from 127 to
+    // 166 VGPRs, half the waves) but lays the boxes out with row and plane
+    // pitches congruent mod 32 (4-byte banks) to the accumulator box's:
+    // lanes are dealt so that deposit bases fall on distinct banks
+    // (bank_spread_lane), and with congruent pitches the gather's primal
+    // bases then do too.  At 8x8x8 this still fits two workgroups per CU.
+    static constexpr int AX = TX + M + 3, AY = TY + M + 3, AZ = TZ + M + 3;
+    static constexpr bool CONG = M == 2 && sizeof(ACC) == 4;
     static constexpr int dual(int c) {
         return M != 4 ? 0 : c == 0 ? 1 : c == 1 ? 2 : c == 2 ? 4 : c == 3 ? 6 : c == 4 ? 5 : 3;
     }
     static constexpr int cdim(int c, int a) { return (a == 0 ? FX : a == 1 ? FY : FZ) - ((dual(c) >> a) & 1); }
-    static constexpr int cvol(int c) { return cdim(c, 0) * cdim(c, 1) * cdim(c, 2); }
-    static constexpr int coff(int c) {
-        return (c > 0 ? cvol(0) : 0) + (c > 1 ? cvol(1) : 0) + (c > 2 ? cvol(2) : 0) + (c > 3 ? cvol(3) : 0) +
-               (c > 4 ? cvol(4) : 0) + (c > 5 ? cvol(5) : 0);
+    static constexpr int pitch32(int at_least, int residue) {
+        return at_least + ((residue - at_least) % 32 + 32) % 32;
     }
+    static constexpr int rowp(int c) { return CONG ? pitch32(FX, AX) : cdim(c, 0); }
+    static constexpr int planep(int c) {
+        return CONG ? pitch32(rowp(c) * cdim(c, 1), AX * AY) : cdim(c, 0) * cdim(c, 1);
+    }
+    static constexpr int csize(int c) {
+        return (cdim(c, 2) - 1) * planep(c) + (cdim(c, 1) - 1) * rowp(c) + cdim(c, 0);
+    }
+    static constexpr int coff(int c) { return c == 0 ? 0 : coff(c - 1) + csize(c - 1); }
     static constexpr int FTOT = coff(6);
     // accumulator box: [T0 + m - 2 - H, T0 + T + m + H]
-    static constexpr int AX = TX + M + 3, AY = TY + M + 3, AZ = TZ + M + 3;
     static constexpr int AVOL = AX * AY * AZ;
     // accumulators first (8-byte aligned for ACC = double), then fields
     static constexpr size_t LDS = (size_t)4 * AVOL * sizeof(ACC) + (size_t)FTOT * sizeof(float);
@@ -463,10 +476,10 @@ __global__ __launch_bounds__((Push32Threads<M, ACC, SPIN>::N), 2) void k_push32_
     auto load = [&](int c, const float* src) {
         const int DX = B::cdim(c, 0), DY = B::cdim(c, 1);
         float* dst = fld + B::coff(c);
-        for (int r = threadIdx.x; r < B::cvol(c); r += blockDim.x) {
+        for (int r = threadIdx.x; r < DX * DY * B::cdim(c, 2); r += blockDim.x) {
             const int lx = r % DX, ly = (r / DX) % DY, lz = r / (DX * DY);
             const int gx = min(fox + lx, g.fe[0] - 1), gy = min(foy + ly, g.fe[1] - 1), gz = min(foz + lz, g.fe[2] - 1);
-            dst[r] = src[g.fi(gz, gy, gx)];
+            dst[lz * B::planep(c) + ly * B::rowp(c) + lx] = src[g.fi(gz, gy, gx)];
         }
     };
     load(0, F.ex); load(1, F.ey); load(2, F.ez); load(3, F.bx); load(4, F.by); load(5, F.bz);
@@ -528,9 +541,9 @@ __global__ __launch_bounds__((Push32Threads<M, ACC, SPIN>::N), 2) void k_push32_
         // field reads relative to the LDS tile; deposits relative to the
         // accumulator box (an in-tile particle's boxes lie inside both)
         auto fr = [&](int c, int k, int j, int i) -> float {
-            const int DX = B::cdim(c, 0), DY = B::cdim(c, 1);
-            const float* fb = fld + B::coff(c) + ((cz + m - 1 - foz) * DY + (cy + m - 1 - foy)) * DX + (cx + m - 1 - fox);
-            return fb[((k - (cz + m - 1)) * DY + (j - (cy + m - 1))) * DX + (i - (cx + m - 1))];
+            const float* fb = fld + B::coff(c) + (cz + m - 1 - foz) * B::planep(c) + (cy + m - 1 - foy) * B::rowp(c) +
+                              (cx + m - 1 - fox);
+            return fb[(k - (cz + m - 1)) * B::planep(c) + (j - (cy + m - 1)) * B::rowp(c) + (i - (cx + m - 1))];
         };
         LdsSink32<B::AX, B::AY, B::AVOL, ACC> sink{acc, K.inv_ufix,
                                               ((cz + m - 1 - aoz) * B::AY + (cy + m - 1 - aoy)) * B::AX + (cx + m - 1 - aox), 0};
