@@ -84,10 +84,10 @@ __global__ __launch_bounds__(256) void k_hist64(Soa64 A, const int64_t* __restri
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            int head, len;
-            wave_runs(key[r], valid[r], head, len);
-            if (valid[r] && head == lane_id())
-                atomicAdd((unsigned long long*)&tile_cnt[key[r]], (unsigned long long)len);
+            int leader, rank, count;
+            wave_groups(key[r], valid[r], leader, rank, count);
+            if (valid[r] && leader == lane_id())
+                atomicAdd((unsigned long long*)&tile_cnt[key[r]], (unsigned long long)count);
         }
     }
 }
@@ -234,14 +234,14 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
             if (valid) key = bin_of64(x, y, z, T);
             else if (perm) { valid = true; key = T.tl.nbins(); }
         }
-        int head, len;
-        wave_runs(key, valid, head, len);
+        int leader, rank, count;
+        wave_groups(key, valid, leader, rank, count);
         int64_t base = 0;
-        if (valid && head == lane_id())
-            base = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key], (unsigned long long)len);
-        base = __shfl(base, head);
+        if (valid && leader == lane_id())
+            base = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key], (unsigned long long)count);
+        base = __shfl(base, leader);
         if (valid) {
-            const int64_t q = placed_slot(base + (lane_id() - head), key, T.tl.ntiles, tile_off, scramble);
+            const int64_t q = placed_slot(base + rank, key, T.tl.ntiles, tile_off, scramble);
             B.x[q] = x; B.y[q] = y; B.z[q] = z; B.w[q] = w;
             B.ux[q] = A.ux[p]; B.uy[q] = A.uy[p]; B.uz[q] = A.uz[p];
             B.id[q] = A.id[p];

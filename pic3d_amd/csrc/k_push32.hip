@@ -692,10 +692,10 @@ __global__ __launch_bounds__(256) void k_hist32(const int32_t* __restrict__ cell
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            int head, len;
-            wave_runs(key[r], valid[r], head, len);
-            if (valid[r] && head == lane_id())
-                atomicAdd((unsigned long long*)&tile_cnt[key[r]], (unsigned long long)len);
+            int leader, rank, count;
+            wave_groups(key[r], valid[r], leader, rank, count);
+            if (valid[r] && leader == lane_id())
+                atomicAdd((unsigned long long*)&tile_cnt[key[r]], (unsigned long long)count);
         }
     }
 }
@@ -750,12 +750,13 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
         } else {
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                int len;
-                wave_runs(key[r], valid[r], head[r], len);
+                int leader, rank, count;
+                wave_groups(key[r], valid[r], leader, rank, count);
                 base[r] = 0;
-                if (valid[r] && head[r] == lane_id())
-                    base[r] = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key[r]], (unsigned long long)len);
-                base[r] = __shfl(base[r], head[r]);
+                if (valid[r] && leader == lane_id())
+                    base[r] = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key[r]], (unsigned long long)count);
+                base[r] = __shfl(base[r], leader) + rank;
+                head[r] = lane_id();
             }
         }
 #pragma unroll

@@ -11,23 +11,29 @@ namespace pic {
 // ---------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-// For lanes holding a key (valid), group consecutive lanes with equal keys:
-// returns the lane of the run head and the run length (at the head).
-__device__ __forceinline__ void wave_runs(int64_t key, bool valid, int& head_lane, int& run_len) {
+// Groups the wave's valid lanes by key, wherever they sit (a few particles
+// that crossed into a neighbouring tile no longer split the home tile's
+// lanes into many runs): leader = the group's lowest lane, rank = this
+// lane's position among the group's lanes in lane order, count = the
+// group's size (at the leader).  One pass per distinct key.
+__device__ __forceinline__ void wave_groups(int64_t key, bool valid, int& leader, int& rank, int& count) {
     const int lane = lane_id();
-    const int64_t prev = __shfl_up(key, 1);
-    const bool prev_valid = __shfl_up((int)valid, 1) != 0;
-    const bool head = valid && (lane == 0 || !prev_valid || prev != key);
-    const uint64_t heads = __ballot(head);
-    const uint64_t vmask = __ballot(valid);
-    // head of my run: highest head lane <= lane
-    const uint64_t below = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
-    head_lane = below ? 63 - __clzll(below) : lane;
-    // run length at the head: up to the next head or the next invalid lane
-    const uint64_t stop = heads | ~vmask;
-    const uint64_t higher = (lane == 63) ? 0ull : (stop & (~0ull << (lane + 1)));
-    const int next = higher ? __ffsll((unsigned long long)higher) - 1 : 64;
-    run_len = next - lane;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint64_t rem = __ballot(valid);
+    leader = lane;
+    rank = 0;
+    count = 0;
+    while (rem) {
+        const int first = __ffsll((unsigned long long)rem) - 1;
+        const int64_t gk = __shfl(key, first);
+        const uint64_t msk = __ballot(valid && key == gk) & rem;
+        if ((msk >> lane) & 1ull) {
+            leader = first;
+            rank = __popcll(msk & below);
+        }
+        if (lane == first) count = __popcll(msk);
+        rem &= ~msk;
+    }
 }
 
 // Bank-spreading lane assignment.  Lanes whose LDS bases fall in the same
