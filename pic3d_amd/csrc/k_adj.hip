@@ -742,7 +742,12 @@ struct AdjView64 {
 
 struct AdjView32 {
     using T = float;
-    static constexpr int MIN_WAVES = 1;
+    // two waves per SIMD: the quartic 4x4x4 tile's boxes (68.5 KB) let two
+    // workgroups share a CU, and the register cap at 256 (444 bytes of
+    // spills per lane) still measured faster than one wave per SIMD
+    // (backward 5.90 -> 5.14 ms per step at the 4.17M-electron ramp case);
+    // the quadratic kernels fit 256 registers without spills
+    static constexpr int MIN_WAVES = 2;
     AdjPush32 A;
     AdjK<float> K;
     int n[3];
@@ -1564,10 +1569,11 @@ AdjK<float> adj_consts32(const pic_ctx* c, const Species& S) {
 
 int tile_kind(const pic_ctx* c) { return c->fp32 ? (c->fix32 ? 2 : 1) : 0; }
 
-// Parts per tile (split along x).  Whole tiles: the kernels hold 256
-// registers (one wave per SIMD), and a split that let two workgroups'
-// LDS fit a CU needed the registers capped at two waves per SIMD, which
-// spilled and measured slower.
+// Parts per tile (split along x).  Whole tiles: a split of the 8x4x4
+// quartic tile that let two workgroups' LDS fit a CU (the FP32 adjoint then
+// held 357 registers, one wave per SIMD) needed the registers capped at two
+// waves per SIMD, which spilled and measured slower; the 4x4x4 quartic tile
+// of the fixed-point FP32 push fits two workgroups whole (AdjView32).
 template <int M, int KIND> struct AdjSplit { static constexpr int S = 1; };
 
 template <int M, int KIND, typename V, typename LACC = double>

@@ -398,18 +398,19 @@ struct Box32 {
     // where it sits on the dual grid its stencil starts at the cell (not the
     // nearest node), so it needs one point fewer there.  Bit a of dual(c):
     // component c (Ex, Ey, Ez, Bx, By, Bz) is dual along axis a.  The saving
-    // (3.6 KB at the 8x4x4 tile) lets three workgroups share a CU: push
+    // (3.6 KB at the 8x4x4 tile) let three workgroups share a CU: push
     // 10.0 -> 9.4 ms at 41.9M electrons.
     //
-    // The quadratic fixed-point kernel (CONG) keeps one box shape for all
-    // six components (one base register: trimmed boxes took it from 127 to
-    // 166 VGPRs, half the waves) but lays the boxes out with row and plane
-    // pitches congruent mod 32 (4-byte banks) to the accumulator box's:
-    // lanes are dealt so that deposit bases fall on distinct banks
+    // The fixed-point kernels (CONG) lay the field boxes out with row and
+    // plane pitches congruent mod 32 (4-byte banks) to the accumulator
+    // box's: lanes are dealt so that deposit bases fall on distinct banks
     // (bank_spread_lane), and with congruent pitches the gather's primal
-    // bases then do too.  At 8x8x8 this still fits two workgroups per CU.
+    // bases then do too.  The quadratic one keeps one box shape for all six
+    // components (one base register: trimmed boxes took it from 127 to 166
+    // VGPRs, half the waves); at 8x8x8 this still fits two workgroups per
+    // CU, and the quartic one at 4x4x4 three.
     static constexpr int AX = TX + M + 3, AY = TY + M + 3, AZ = TZ + M + 3;
-    static constexpr bool CONG = M == 2 && sizeof(ACC) == 4;
+    static constexpr bool CONG = sizeof(ACC) == 4;
     static constexpr int dual(int c) {
         return M != 4 ? 0 : c == 0 ? 1 : c == 1 ? 2 : c == 2 ? 4 : c == 3 ? 6 : c == 4 ? 5 : 3;
     }

@@ -251,18 +251,21 @@ __device__ __forceinline__ int64_t placed_slot(int64_t q, int64_t tile, int ntil
 // accumulator box; a CU has 160 KB).  KIND: 0 = FP64 fields and FP64
 // accumulators; 1 = FP32 fields, FP64 accumulators; 2 = FP32 fields, int32
 // fixed-point accumulators.
-//   0: quadratic 8x4x4 (60 KB), quartic 4x4x4 (78 KB): two workgroups per CU
+//   0: quadratic 8x4x4 (60 KB), quartic 4x4x4 (79 KB): two workgroups per CU
 //   1: quadratic 8x8x4 (69 KB), quartic 4x4x4 (60 KB)
-//   2: quadratic 8x8x8 (67 KB), quartic 8x4x4 (54 KB).  Measured at the
+//   2: quadratic 8x8x8 (77 KB), quartic 4x4x4 (42 KB).  Measured at the
 //      laser-slab case: quartic 8x4x4 (three workgroups per CU) 12.8 ms vs
-//      8x8x4 (two) 14.5 ms; quadratic 8x8x8 6.2 ms vs 8x8x4 6.8 ms.
+//      8x8x4 (two) 14.5 ms; quadratic 8x8x8 6.2 ms vs 8x8x4 6.8 ms.  The
+//      quartic kernel moved to 4x4x4 with field boxes whose pitches are
+//      congruent to the accumulator box's (still three workgroups per CU):
+//      push 9.49 -> 8.74 ms at steps 4-23, 10.74 -> 10.51 ms at 401-420.
 template <int KIND, int M> struct TileShape;
 template <> struct TileShape<0, 2> { static constexpr int X = 8, Y = 4, Z = 4; };
 template <> struct TileShape<0, 4> { static constexpr int X = 4, Y = 4, Z = 4; };
 template <> struct TileShape<1, 2> { static constexpr int X = 8, Y = 8, Z = 4; };
 template <> struct TileShape<1, 4> { static constexpr int X = 4, Y = 4, Z = 4; };
 template <> struct TileShape<2, 2> { static constexpr int X = 8, Y = 8, Z = 8; };
-template <> struct TileShape<2, 4> { static constexpr int X = 8, Y = 4, Z = 4; };
+template <> struct TileShape<2, 4> { static constexpr int X = 4, Y = 4, Z = 4; };
 
 // Set c->tl for the context's precision, order and accumulator kind.
 void set_tiling(pic_ctx* c);
