@@ -193,7 +193,7 @@ enum pic_option {
                                   adjoint push (HBM kernel after a direct forward);
                                   1 dual-number kernel (FP64; FP32 takes 2);
                                   2 separable HBM kernel */
-    PIC_OPT_XMAX_BOUNDARY = 7  /* x-max Silver-Mueller write (x open):
+    PIC_OPT_XMAX_BOUNDARY = 7, /* x-max Silver-Mueller write (x open):
                                   0 (default) the reference's: column
                                   local_xcells+1 from column local_xcells
                                   (boundary.cpp:92-96), three columns inside
@@ -202,6 +202,14 @@ enum pic_option {
                                   write: the last dual column n+mem-1 from
                                   E, Bx and J at the primal column n+mem-1
                                   and By, Bz at dual column n+mem-2 */
+    PIC_OPT_CELL_ORDER = 8     /* FP32 quartic with fixed-point accumulators:
+                                  1 (default) sorts bin particles by cell, and
+                                  the push gathers with the lanes of a wave in
+                                  neighbouring cells, then deposits from an LDS
+                                  stage with the lanes spread over cells; 0 tile
+                                  bins and one pass with scattered lanes.
+                                  Results are identical up to the order of the
+                                  fixed-point sums, which is exact. */
 };
 int pic_set_option(pic_ctx* ctx, int option, int64_t value);
 

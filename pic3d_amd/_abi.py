@@ -22,6 +22,7 @@ PIC_OPT_PUSH_KERNEL, PIC_OPT_KEEP_OLD, PIC_OPT_SORT_INTERVAL, PIC_OPT_TIMING = 1
 PIC_OPT_FP32_ACCUM = 5
 PIC_OPT_ADJ_PUSH = 6
 PIC_OPT_XMAX_BOUNDARY = 7
+PIC_OPT_CELL_ORDER = 8
 TIMING_CATS = ["fields", "clear", "push", "sort", "boundary", "fold"]
 
 

@@ -1412,6 +1412,7 @@ int replay_sort(pic_ctx* c, DiffState& D, const int32_t* perm, const int64_t* to
     PIC_HIP(c, hipMemcpyAsync(S.tile_off, toff, (size_t)(c->tl.ntiles + 2) * 8, hipMemcpyDeviceToDevice, c->stream));
     S.cur = b;
     S.sorted = 1;
+    S.cellsorted = 0;  // a recorded tile sort
     S.pushes_since_sort = 0;
     return PIC_OK;
 }

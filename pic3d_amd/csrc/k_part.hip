@@ -169,16 +169,17 @@ __global__ __launch_bounds__(256) void k_scan3(int64_t* __restrict__ out, int64_
 // tile t starts at its bin's offset; with the removed slots' bin,
 // tile_off[ntiles] is that bin's start and tile_off[ntiles+1] the total.
 // The total is the new slot count.
+// With cell bins (cpt per tile) tile t starts at its first cell's bin.
 __global__ __launch_bounds__(256) void k_tile_off(const int64_t* __restrict__ cur, int64_t nbins,
-                                                  const int64_t* __restrict__ total, int ntiles,
+                                                  const int64_t* __restrict__ total, int ntiles, int cpt,
                                                   int64_t* __restrict__ tile_off, int64_t* __restrict__ slot_cnt) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t tot = *total;
-    if (t <= ntiles + 1) tile_off[t] = t < nbins ? cur[t] : tot;
+    if (t <= ntiles + 1) tile_off[t] = (int64_t)t * cpt < nbins ? cur[(int64_t)t * cpt] : tot;
     if (t == 0) slot_cnt[CNT_SLOTS] = tot;
 }
 
-int launch_scan_bins(pic_ctx* c, Species& S, int64_t nbins) {
+int launch_scan_bins(pic_ctx* c, Species& S, int64_t nbins, int cpt) {
     const int64_t nb = (nbins + SCAN_B - 1) / SCAN_B;
     k_scan1<<<(unsigned)nb, SCAN_T, 0, c->stream>>>(c->tile_cnt, c->tile_cur, nbins, c->scan_sums);
     PIC_LAUNCH_CHECK(c);
@@ -188,7 +189,7 @@ int launch_scan_bins(pic_ctx* c, Species& S, int64_t nbins) {
                                                                                        c->scan_sums);
     PIC_LAUNCH_CHECK(c);
     k_tile_off<<<blocks_for(c->tl.ntiles + 2, 256), 256, 0, c->stream>>>(c->tile_cur, nbins, c->scan_sums + nb,
-                                                                         c->tl.ntiles, S.tile_off, S.d_cnt);
+                                                                         c->tl.ntiles, cpt, S.tile_off, S.d_cnt);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }

@@ -601,6 +601,232 @@ __global__ __launch_bounds__((Push32Threads<M, ACC, SPIN>::N), 2) void k_push32_
     if (SPIN && spin_bad) atomicAdd((unsigned long long*)&cnt[CNT_SPIN], (unsigned long long)spin_bad);
 }
 
+// ---------------------------------------------------------------------
+// cell-coherent tiled push (fixed-point accumulators; tiles kept in cell
+// order by the sort).  A workgroup pass takes NT consecutive slots:
+//  phase 1: thread t pushes slot t0 + t.  The 64 lanes of a wave hold the
+//   particles of a few neighbouring cells, so a lane's field reads mostly
+//   fall on the same LDS words as its cell-mates' (one broadcast read, no
+//   bank conflict; with lanes in 64 different cells a read instruction
+//   costs about twice the LDS clocks).  Gather, Boris, move and the
+//   particle write-back are coalesced; the deposit inputs go to an LDS
+//   stage.
+//  phase 2: thread t deposits stage entry coh_pick(t): each 16-lane group
+//   holds entries 16 slots apart, in different cells, so one atomic
+//   instruction rarely hits an address twice (same-address LDS atomics
+//   serialise, which made a single cell-ordered pass slower than the
+//   scattered one).
+// Stage entry: f0[3], f1[3] (old offsets, unwrapped new offsets), q w, the
+// old cell's accumulator box offset (-1: nothing to deposit).  The stage
+// is indexed through coh_addr, which keeps both phases' 32-lane halves on
+// 32 distinct banks.
+// ---------------------------------------------------------------------
+constexpr int COH_NT = 256;
+constexpr int COH_WORDS = 8;
+// stage entry read by thread t in phase 2: wave w, lane l -> w + 4 (l / 16) + 16 (l % 16)
+__device__ __forceinline__ int coh_pick(int t) {
+    const int w = t >> 6, l = t & 63;
+    return w + 4 * (l >> 4) + 16 * (l & 15);
+}
+// word of entry e: bits 0, 1 and 3 of e xor-ed with bits 5, 6 and 7, so
+// that entries e..e+31 (phase 1) and the 32 entries of a phase-2 half-wave
+// (bits 2, 4, 5, 6, 7 varying) each map onto 32 distinct banks
+__device__ __forceinline__ int coh_addr(int e) { return e ^ ((e >> 5) & 3) ^ ((e >> 4) & 8); }
+
+template <int M, int TX, int TY, int TZ, bool SPIN>
+__global__ __launch_bounds__(COH_NT, 3) void k_push32_coh(Soa32 P, Fields32 F, Grid g, Consts32 K,
+                                                           Tiling tl, const int64_t* __restrict__ tile_off,
+                                                           const int64_t* __restrict__ work,
+                                                           const int64_t* __restrict__ nwork, int chunk,
+                                                           int64_t* __restrict__ cnt) {
+    using B = Box32<M, TX, TY, TZ, int>;
+    static_assert(COH_NT == 256, "coh_pick and coh_addr assume 256 entries");
+    extern __shared__ double lds_d[];
+    int* acc = reinterpret_cast<int*>(lds_d);                    // 4 * AVOL accumulators
+    float* fld = reinterpret_cast<float*>(acc + 4 * B::AVOL);    // the six field boxes
+    float* stg = fld + B::FTOT;                                   // COH_WORDS x COH_NT stage
+    __shared__ unsigned cnt12[12];
+    __shared__ unsigned rg_hist[COH_NT / 64][8];  // per wave: bank-spreading counters
+    const int64_t wi = blockIdx.x;
+    if (wi >= *nwork) return;
+    const int64_t item = work[wi];
+    const int tile = (int)(item >> 20);
+    const int64_t cidx = item & ((1 << 20) - 1);
+    const int64_t t_end = tile_off[tile + 1];
+    const int64_t beg = tile_off[tile] + cidx * chunk;
+    const int64_t end = min(t_end, beg + chunk);
+    const int txi = tile % tl.nt[0], tyi = (tile / tl.nt[0]) % tl.nt[1], tzi = tile / (tl.nt[0] * tl.nt[1]);
+    const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
+    const int m = g.m;
+    const int fox = T0x + m - 1 - B::H, foy = T0y + m - 1 - B::H, foz = T0z + m - 1 - B::H;
+    const float* fsrc[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
+    auto load = [&](int c, const float* src) {
+        const int DX = B::cdim(c, 0), DY = B::cdim(c, 1);
+        float* dst = fld + B::coff(c);
+        for (int r = threadIdx.x; r < DX * DY * B::cdim(c, 2); r += blockDim.x) {
+            const int lx = r % DX, ly = (r / DX) % DY, lz = r / (DX * DY);
+            const int gx = min(fox + lx, g.fe[0] - 1), gy = min(foy + ly, g.fe[1] - 1), gz = min(foz + lz, g.fe[2] - 1);
+            dst[lz * B::planep(c) + ly * B::rowp(c) + lx] = src[g.fi(gz, gy, gx)];
+        }
+    };
+    load(0, F.ex); load(1, F.ey); load(2, F.ez); load(3, F.bx); load(4, F.by); load(5, F.bz);
+    for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) acc[e] = 0;
+    if (threadIdx.x < 12) cnt12[threadIdx.x] = 0;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
+    if (lane < 8) rg_hist[wv][lane] = 0u;
+    __syncthreads();
+    const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
+    int64_t oob = 0;
+    int spin_bad = 0;
+    uint32_t strays = 0;  // passes whose particle was outside the tile
+    const int len = (int)(end - beg);
+    const int sw = coh_addr(tid), sr = coh_addr(coh_pick(tid));
+    // a pass's particle loads are issued during the previous pass's
+    // deposit phase, so their latency hides behind its LDS atomics
+    int pc = -1;
+    float pf[3] = {0.0f, 0.0f, 0.0f}, pu[3] = {0.0f, 0.0f, 0.0f}, pw = 0.0f, ps[3] = {0.0f, 0.0f, 0.0f};
+    auto fetch = [&](int t) {
+        pc = -1;
+        if (t < len) {
+            const int64_t p = beg + t;
+            pc = P.cell[p];
+            pf[0] = P.fx[p]; pf[1] = P.fy[p]; pf[2] = P.fz[p];
+            pu[0] = P.ux[p]; pu[1] = P.uy[p]; pu[2] = P.uz[p];
+            pw = P.w[p];
+            if (SPIN) { ps[0] = P.sx[p]; ps[1] = P.sy[p]; ps[2] = P.sz[p]; }
+        }
+    };
+    fetch(tid);
+    int it = 0;
+    for (int t0 = 0; t0 < len; t0 += COH_NT, it++) {
+        // phase 1: gather, Boris, move, write-back; stage the deposit
+        const int t = t0 + tid;
+        int bo = -1;
+        float f0[3] = {0.0f, 0.0f, 0.0f}, f1[3] = {0.0f, 0.0f, 0.0f}, qw = 0.0f;
+        if (t < len) {
+            const int64_t p = beg + t;
+            const int cell = pc;
+            if (cell >= 0) {
+                int cx, cy, cz;
+                K.cs.split(cell, cx, cy, cz);
+                if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
+                    (unsigned)(cz - T0z) >= (unsigned)TZ) {
+                    strays |= 1u << it;  // moved out of the tile since the last sort
+                } else {
+                    float f[3] = {pf[0], pf[1], pf[2]};
+                    float u[3] = {pu[0], pu[1], pu[2]};
+                    const float w = pw;
+                    float sp[3] = {ps[0], ps[1], ps[2]};
+                    auto fr = [&](int c, int k, int j, int i) -> float {
+                        const float* fb = fld + B::coff(c) + (cz + m - 1 - foz) * B::planep(c) +
+                                          (cy + m - 1 - foy) * B::rowp(c) + (cx + m - 1 - fox);
+                        return fb[(k - (cz + m - 1)) * B::planep(c) + (j - (cy + m - 1)) * B::rowp(c) +
+                                  (i - (cx + m - 1))];
+                    };
+                    const int Cx = cx + m - 1, Cy = cy + m - 1, Cz = cz + m - 1;
+                    float E[3], Bf[3], W0[3][M + 1];
+                    gather32<M>(Cx, Cy, Cz, f[0], f[1], f[2], E, Bf, fr, W0[0], W0[1], W0[2]);
+                    boris32(u, E, Bf, K.qm);
+                    if constexpr (SPIN) {
+                        const float gam = sqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+                        if (!spin_precess<float>(E, Bf, K.qm, u[0], u[1], u[2], gam, K.spin, sp[0], sp[1], sp[2]))
+                            spin_bad++;
+                    }
+                    const float ig = rsqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+                    const float d[3] = {u[0] * ig * K.cfx, u[1] * ig * K.cfy, u[2] * ig * K.cfz};
+                    // a non-finite momentum would put the deposit box outside the arrays
+                    if (!(fabsf(d[0]) < 1.0f && fabsf(d[1]) < 1.0f && fabsf(d[2]) < 1.0f)) {
+                        oob++;
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) { f0[a] = f[a]; f1[a] = f[a] + d[a]; }
+                        qw = K.q * w;
+                        bo = ((cz + m - 1 - aoz) * B::AY + (cy + m - 1 - aoy)) * B::AX + (cx + m - 1 - aox);
+                        const int nc = move32(cx, cy, cz, f, d, K, cnt12);
+                        P.cell[p] = nc;
+                        P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
+                        P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
+                        if (SPIN) { P.sx[p] = sp[0]; P.sy[p] = sp[1]; P.sz[p] = sp[2]; }
+                    }
+                }
+            }
+        }
+        stg[0 * COH_NT + sw] = f0[0]; stg[1 * COH_NT + sw] = f0[1]; stg[2 * COH_NT + sw] = f0[2];
+        stg[3 * COH_NT + sw] = f1[0]; stg[4 * COH_NT + sw] = f1[1]; stg[5 * COH_NT + sw] = f1[2];
+        stg[6 * COH_NT + sw] = qw;
+        stg[7 * COH_NT + sw] = __int_as_float(bo);
+        __syncthreads();
+        fetch(t0 + COH_NT + tid);
+        // phase 2: deposit a staged particle, lanes spread over cells and
+        // then dealt by the LDS bank of their deposit base (bank_spread_lane)
+        int bd = __float_as_int(stg[7 * COH_NT + sr]);
+        float a0[3] = {stg[0 * COH_NT + sr], stg[1 * COH_NT + sr], stg[2 * COH_NT + sr]};
+        float a1[3] = {stg[3 * COH_NT + sr], stg[4 * COH_NT + sr], stg[5 * COH_NT + sr]};
+        float q = stg[6 * COH_NT + sr];
+        {
+            // bank of the deposit base: the cell's box offset moved to the
+            // nearest node of the old position
+            const int r = bd < 0 ? 32
+                                 : (bd + (a0[0] >= 0.5f) + (a0[1] >= 0.5f) * B::AX + (a0[2] >= 0.5f) * (B::AX * B::AY)) & 31;
+            const int to = bank_spread_lane(r, rg_hist[wv]);
+            bd = lane_send(to, bd);
+#pragma unroll
+            for (int a = 0; a < 3; a++) { a0[a] = lane_send(to, a0[a]); a1[a] = lane_send(to, a1[a]); }
+            q = lane_send(to, q);
+        }
+        if (bd >= 0) {
+            // the old position's primal weights, as the gather evaluated them
+            float S0[3][M + 1];
+#pragma unroll
+            for (int a = 0; a < 3; a++) shape32<M>(a0[a] - (float)(a0[a] >= 0.5f ? 1 : 0), S0[a]);
+            LdsSink32<B::AX, B::AY, B::AVOL, int> sink{acc, K.inv_ufix, bd, 0};
+            esirkepov32<M>(q, a0, a1, S0[0], S0[1], S0[2], K, sink);
+        }
+        __syncthreads();
+    }
+    if (strays) {  // HBM path for the particles outside the tile
+        float* jdst[4] = {F.jx, F.jy, F.jz, F.rho};
+        it = 0;
+        for (int t = threadIdx.x; t < len; t += blockDim.x, it++) {
+            if (!(strays & (1u << it))) continue;
+            const int64_t p = beg + t;
+            const int cell = P.cell[p];
+            int cx, cy, cz;
+            K.cs.split(cell, cx, cy, cz);
+            float f[3] = {P.fx[p], P.fy[p], P.fz[p]};
+            float u[3] = {P.ux[p], P.uy[p], P.uz[p]};
+            auto fr = [&](int c, int k, int j, int i) -> float { return fsrc[c][g.fi(k, j, i)]; };
+            HbmSink32 sink{{jdst[0], jdst[1], jdst[2], jdst[3]}, g, cz + m - 1, cy + m - 1, cx + m - 1, 0, 0, 0};
+            int nc;
+            float sp[3];
+            if (SPIN) { sp[0] = P.sx[p]; sp[1] = P.sy[p]; sp[2] = P.sz[p]; }
+            if (!push32_core<M, SPIN>(cx, cy, cz, m, f, u, P.w[p], K, fr, sink, nc, cnt12, sp, spin_bad)) {
+                oob++;
+                continue;
+            }
+            P.cell[p] = nc;
+            P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
+            P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
+            if (SPIN) { P.sx[p] = sp[0]; P.sy[p] = sp[1]; P.sz[p] = sp[2]; }
+        }
+    }
+    __syncthreads();
+    float* jdst[4] = {F.jx, F.jy, F.jz, F.rho};
+    for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) {
+        const int a = acc[e];
+        if (a == 0) continue;
+        const int c = e / B::AVOL, r = e % B::AVOL;
+        const float v = (float)a * K.ufix[c];
+        const int gx = aox + r % B::AX, gy = aoy + (r / B::AX) % B::AY, gz = aoz + r / (B::AX * B::AY);
+        if (gx > g.je[0] - 2 || gy > g.je[1] - 2 || gz > g.je[2] - 2) continue;
+        hbm_add(&jdst[c][g.ji(gz, gy, gx)], v);
+    }
+    if (threadIdx.x < 12 && cnt12[threadIdx.x])
+        atomicAdd((unsigned long long*)&cnt[CNT_PEND + threadIdx.x], (unsigned long long)cnt12[threadIdx.x]);
+    if (oob) atomicAdd((unsigned long long*)&cnt[CNT_OOB], (unsigned long long)oob);
+    if (SPIN && spin_bad) atomicAdd((unsigned long long*)&cnt[CNT_SPIN], (unsigned long long)spin_bad);
+}
+
 // direct push: any particle order, gather from HBM, global atomics
 template <int M, bool SPIN>
 __global__ __launch_bounds__(256) void k_push32_direct(Soa32 P, Fields32 F, Grid g, Consts32 K,
@@ -650,13 +876,19 @@ struct TileArgs32 {
     Tiling tl;
     CellSplit cs;
     FastDiv td[3];  // tile edges
+    int cpt;        // bins per tile: 1, or its cells (cell order inside tiles)
 };
 
-// sort bin of a cell: its tile
+// sort bin of a cell: its tile, or (cpt > 1) the tile's first bin plus the
+// cell's index inside the tile, x fastest
 __device__ __forceinline__ int64_t bin_of32(int cell, const TileArgs32& A) {
     int cx, cy, cz;
     A.cs.split(cell, cx, cy, cz);
-    return (A.td[2].div(cz) * A.tl.nt[1] + A.td[1].div(cy)) * A.tl.nt[0] + A.td[0].div(cx);
+    const int tx = A.td[0].div(cx), ty = A.td[1].div(cy), tz = A.td[2].div(cz);
+    const int64_t tile = (tz * A.tl.nt[1] + ty) * A.tl.nt[0] + tx;
+    if (A.cpt == 1) return tile;
+    const int lx = cx - tx * A.tl.t[0], ly = cy - ty * A.tl.t[1], lz = cz - tz * A.tl.t[2];
+    return tile * A.cpt + (lz * A.tl.t[1] + ly) * A.tl.t[0] + lx;
 }
 
 constexpr int SORT_R32 = 4;  // rows of 64 slots per wave and block iteration
@@ -778,10 +1010,14 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
 
 int sort32(pic_ctx* c, int s, int32_t* perm) {
     Species& S = c->sp[s];
-    const int64_t nbins = c->tl.nbins() + (perm ? 1 : 0);
+    // cell bins for the cell-coherent push; differentiated runs (which
+    // record and replay tile-sort permutations) keep tile bins
+    const bool fine = c->tl.fine && !perm && !c->diff;
+    TileArgs32 T;
+    T.cpt = fine ? c->tl.cells() : 1;
+    const int64_t nbins = c->tl.nbins() * T.cpt + (perm ? 1 : 0);
     int r = launch_fill(c, c->tile_cnt, nbins * 8);
     if (r) return r;
-    TileArgs32 T;
     for (int a = 0; a < 3; a++) { T.n[a] = c->g.n[a]; T.td[a] = FastDiv(c->tl.t[a]); }
     T.tl = c->tl;
     T.cs = CellSplit(c->g.n[0], c->g.n[1]);
@@ -790,12 +1026,16 @@ int sort32(pic_ctx* c, int s, int32_t* perm) {
     PIC_LAUNCH_CHECK(c);
     PIC_HIP(c, hipMemcpyAsync(S.d_cnt + (CNT_N - 1), S.d_cnt + CNT_SLOTS, 8,
                               hipMemcpyDeviceToDevice, c->stream));
-    r = launch_scan_bins(c, S, nbins);
+    r = launch_scan_bins(c, S, nbins, T.cpt);
     if (r) return r;
-    const int scramble = !S.scrambled;
+    // cell bins leave every tile in cell order (a later tile sort
+    // transposes its chunks again)
+    const int scramble = fine ? 0 : !S.scrambled;
     k_scatter32<<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), soa32(S, 1 - S.cur), S.d_cnt,
                                                c->tile_cur, T, perm, S.tile_off, scramble);
     if (scramble) S.scrambled = 1;
+    if (fine) S.scrambled = 0;
+    S.cellsorted = fine ? 1 : 0;
     PIC_LAUNCH_CHECK(c);
     S.cur = 1 - S.cur;
     S.sorted = 1;
@@ -952,6 +1192,20 @@ static int launch_tiled32(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
                   : launch_tiled32_s<M, KIND, ACC, false>(c, S, Fd, K);
 }
 
+template <int M, bool SPIN>
+static int launch_coh32_s(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
+    constexpr int TX = TileShape<2, M>::X, TY = TileShape<2, M>::Y, TZ = TileShape<2, M>::Z;
+    using B = Box32<M, TX, TY, TZ, int>;
+    constexpr size_t lds = B::LDS + (size_t)COH_WORDS * COH_NT * sizeof(float);
+    const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push32_coh<M, TX, TY, TZ, SPIN>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_push32_coh<M, TX, TY, TZ, SPIN><<<(unsigned)max_items, COH_NT, lds, c->stream>>>(
+        soa32(S, S.cur), Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, S.d_cnt);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
 int launch_push32(pic_ctx* c, int s) {
     Species& S = c->sp[s];
     Fields32 Fd{(const float*)c->F[0], (const float*)c->F[1], (const float*)c->F[2],
@@ -984,6 +1238,8 @@ int launch_push32(pic_ctx* c, int s) {
     }
     S.pushes_since_sort++;
     PhaseTimer t(c, PIC_T_PUSH);
+    if (c->fix32 && c->g.m == 4 && S.cellsorted)
+        return S.spin ? launch_coh32_s<4, true>(c, S, Fd, K) : launch_coh32_s<4, false>(c, S, Fd, K);
     if (c->fix32) {
         if (c->g.m == 2) return launch_tiled32<2, 2, int>(c, S, Fd, K);
         return launch_tiled32<4, 2, int>(c, S, Fd, K);

@@ -41,6 +41,8 @@ void set_tiling(pic_ctx* c) {
                  t[2] = q ? TileShape<2, 2>::Z : TileShape<2, 4>::Z; break;
     }
     Tiling& tl = c->tl;
+    // cell-sorted tiles for the push kernels that visit them cell-coherently
+    tl.fine = c->fp32 && c->fix32 && !q && c->cell_order;
     tl.ntiles = 1;
     for (int a = 0; a < 3; a++) {
         tl.t[a] = t[a];
@@ -50,7 +52,7 @@ void set_tiling(pic_ctx* c) {
 }
 
 int alloc_sort_buffers(pic_ctx* c) {
-    const int64_t bins = c->tl.nbins() + 1;  // + the removed slots' bin
+    const int64_t bins = c->tl.max_bins() + 1;  // + the removed slots' bin
     if (bins <= c->sort_cap) return PIC_OK;
     for (int64_t** p : {&c->tile_cnt, &c->tile_cur, &c->scan_sums}) {
         if (*p) hipFree(*p);
@@ -451,10 +453,18 @@ int pic_set_option(pic_ctx* c, int option, int64_t value) {
                 for (auto& e : c->tev) PIC_HIP(c, hipEventCreate(&e));
             }
             return PIC_OK;
+        case PIC_OPT_CELL_ORDER:
         case PIC_OPT_FP32_ACCUM: {
-            if (value != 0 && value != 1) return set_err(c, PIC_ERR_ARG, "accumulator kind must be 0 or 1");
-            if (!c->fp32) return value ? set_err(c, PIC_ERR_ARG, "fixed-point accumulation is an FP32 option") : PIC_OK;
-            c->fix32 = (int)value;
+            if (value != 0 && value != 1)
+                return set_err(c, PIC_ERR_ARG, option == PIC_OPT_CELL_ORDER ? "cell order must be 0 or 1"
+                                                                            : "accumulator kind must be 0 or 1");
+            if (option == PIC_OPT_CELL_ORDER) {
+                c->cell_order = (int)value;
+            } else {
+                if (!c->fp32)
+                    return value ? set_err(c, PIC_ERR_ARG, "fixed-point accumulation is an FP32 option") : PIC_OK;
+                c->fix32 = (int)value;
+            }
             const int old_tiles = c->tl.ntiles;
             set_tiling(c);
             for (int s = 0; s < PIC_MAX_SPECIES; s++) c->sp[s].sorted = 0;

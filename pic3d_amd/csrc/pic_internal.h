@@ -34,13 +34,19 @@ struct Grid {
 };
 
 // Tiling used by the sorted push: cells are grouped in tiles of
-// tx*ty*tz; particles are kept ordered by tile.
+// tx*ty*tz; particles are kept ordered by tile, and with 'fine' by cell
+// inside each tile (one sort bin per cell, cells of a tile consecutive,
+// x fastest).
 struct Tiling {
     int t[3];     // tile edge in cells
     int nt[3];    // tiles per axis
     int ntiles;
+    int fine;     // sorts of this context may bin by cell (cell-coherent push)
     // sort bins: one per tile (a keep-dead sort adds one for removed slots)
     __host__ __device__ int64_t nbins() const { return (int64_t)ntiles; }
+    __host__ __device__ int cells() const { return t[0] * t[1] * t[2]; }
+    // bins to allocate: per cell of every tile when fine
+    __host__ __device__ int64_t max_bins() const { return fine ? (int64_t)ntiles * cells() : (int64_t)ntiles; }
 };
 
 // Per-species state.  FP64: positions in global cell units.  FP32: a linear
@@ -73,6 +79,7 @@ struct Species {
     void* spin_v[3][2] = {};
     int sorted = 0;               // tile_off valid for the current order
     int scrambled = 0;            // the in-tile order has been transposed (stored_slot)
+    int cellsorted = 0;           // the last sort binned by cell (cell order inside tiles)
 };
 
 // d_cnt layout.  CNT_PEND: FP32 pushes classify boundary crossings as they
@@ -121,6 +128,7 @@ struct pic_ctx {
     int keep_old = 0;
     int xmax_face = 0;            // PIC_OPT_XMAX_BOUNDARY
     int fix32 = 0;                // FP32: int32 fixed-point LDS accumulators
+    int cell_order = 1;           // PIC_OPT_CELL_ORDER: cell-sorted tiles + cell-coherent push
     void* jtmp = nullptr;         // J-sized scratch of the current filter (lazy)
     void* stage = nullptr;        // output staging buffer (lazy, k_stage.hip)
     size_t stage_bytes = 0;
@@ -182,7 +190,8 @@ int launch_sort(pic_ctx* c, int s, int32_t* perm = nullptr);
 int build_work(pic_ctx* c, int s, int chunk);
 // Exclusive scan of the sort histogram (nbins bins) into the scatter
 // cursors; the species' tile offsets and slot count from it.
-int launch_scan_bins(pic_ctx* c, Species& S, int64_t nbins);
+// With cpt > 1 the bins are cells, cpt consecutive bins per tile.
+int launch_scan_bins(pic_ctx* c, Species& S, int64_t nbins, int cpt = 1);
 // Allocate the sort buffers for the context's tiling (grows only).
 int alloc_sort_buffers(pic_ctx* c);
 // the work list of a tile grouping into (work, nwork), capacity c->work_cap
