@@ -183,7 +183,8 @@ def run_gpu(args, precision: str) -> dict:
     sim = Simulation(p, push_kernel=args.kernel, sort_interval=args.sort_interval)
     if precision == "fp32":
         sim.set_option(PIC_OPT_FP32_ACCUM, args.fp32_accum)
-    sim.set_option(PIC_OPT_CELL_ORDER, args.cell_order)
+    if args.cell_order is not None:
+        sim.set_option(PIC_OPT_CELL_ORDER, args.cell_order)
     if getattr(args, "spin", False):
         # the electron spin precession on (particle.cpp:318-368) at the
         # reference's E_schwinger, where chi stays below the table (g is
@@ -451,9 +452,10 @@ def main():
                          "80.0 ms per step measured, one sort taking 18-22 ms there)")
     ap.add_argument("--fp32-accum", type=int, default=1,
                     help="FP32 deposit accumulators: 1 int32 fixed point (default), 0 FP64 in LDS")
-    ap.add_argument("--cell-order", type=int, default=1,
-                    help="1 (default): cell-sorted tiles and the cell-coherent push where the library "
-                         "has one; 0: tile order, one push pass with scattered lanes")
+    ap.add_argument("--cell-order", type=int, default=None,
+                    help="N >= 1: cell-sorted tiles (cell bins every N-th sort) and the cell-coherent "
+                         "push where the library has one; 0: tile order, one push pass with scattered "
+                         "lanes (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None,
                     help="CPU baseline threads (default: lscpu's physical cores, capped at the "

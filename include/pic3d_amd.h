@@ -203,13 +203,16 @@ enum pic_option {
                                   E, Bx and J at the primal column n+mem-1
                                   and By, Bz at dual column n+mem-2 */
     PIC_OPT_CELL_ORDER = 8     /* FP32 quartic with fixed-point accumulators:
-                                  1 (default) sorts bin particles by cell, and
-                                  the push gathers with the lanes of a wave in
-                                  neighbouring cells, then deposits from an LDS
-                                  stage with the lanes spread over cells; 0 tile
-                                  bins and one pass with scattered lanes.
-                                  Results are identical up to the order of the
-                                  fixed-point sums, which is exact. */
+                                  N >= 1 keeps each tile's particles in cell
+                                  order (every N-th sort bins by cell, the tile
+                                  sorts in between keep the order; default 8),
+                                  and the push gathers with the lanes of a wave
+                                  in neighbouring cells, then deposits from an
+                                  LDS stage with its lanes spread over cells;
+                                  0: tile bins (chunks stored transposed) and
+                                  one pass with scattered lanes.  Results are
+                                  the same up to the order of the fixed-point
+                                  sums, which is exact. */
 };
 int pic_set_option(pic_ctx* ctx, int option, int64_t value);
 

@@ -1010,9 +1010,13 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
 
 int sort32(pic_ctx* c, int s, int32_t* perm) {
     Species& S = c->sp[s];
-    // cell bins for the cell-coherent push; differentiated runs (which
-    // record and replay tile-sort permutations) keep tile bins
-    const bool fine = c->tl.fine && !perm && !c->diff;
+    // Cell order for the cell-coherent push: every cell_order-th sort bins
+    // by cell; the tile sorts in between keep each tile's relative order
+    // (only particles that changed cell inside their tile, or arrived from
+    // another tile, then sit out of cell order).  Differentiated runs
+    // (which record and replay tile-sort permutations) keep tile bins.
+    const bool cellmode = c->tl.fine && !perm && !c->diff;
+    const bool fine = cellmode && (!S.cellsorted || S.sorts_since_fine + 1 >= c->cell_order);
     TileArgs32 T;
     T.cpt = fine ? c->tl.cells() : 1;
     const int64_t nbins = c->tl.nbins() * T.cpt + (perm ? 1 : 0);
@@ -1028,14 +1032,19 @@ int sort32(pic_ctx* c, int s, int32_t* perm) {
                               hipMemcpyDeviceToDevice, c->stream));
     r = launch_scan_bins(c, S, nbins, T.cpt);
     if (r) return r;
-    // cell bins leave every tile in cell order (a later tile sort
-    // transposes its chunks again)
-    const int scramble = fine ? 0 : !S.scrambled;
+    // in cell mode no chunk is transposed (a tile sort outside it
+    // transposes the chunks once again)
+    const int scramble = cellmode ? 0 : !S.scrambled;
     k_scatter32<<<blocks, 256, 0, c->stream>>>(soa32(S, S.cur), soa32(S, 1 - S.cur), S.d_cnt,
                                                c->tile_cur, T, perm, S.tile_off, scramble);
     if (scramble) S.scrambled = 1;
-    if (fine) S.scrambled = 0;
-    S.cellsorted = fine ? 1 : 0;
+    if (cellmode) {
+        S.scrambled = 0;
+        S.cellsorted = 1;
+        S.sorts_since_fine = fine ? 0 : S.sorts_since_fine + 1;
+    } else {
+        S.cellsorted = 0;
+    }
     PIC_LAUNCH_CHECK(c);
     S.cur = 1 - S.cur;
     S.sorted = 1;

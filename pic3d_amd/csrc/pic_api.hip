@@ -41,8 +41,9 @@ void set_tiling(pic_ctx* c) {
                  t[2] = q ? TileShape<2, 2>::Z : TileShape<2, 4>::Z; break;
     }
     Tiling& tl = c->tl;
-    // cell-sorted tiles for the push kernels that visit them cell-coherently
-    tl.fine = c->fp32 && c->fix32 && !q && c->cell_order;
+    // cell-sorted tiles for the push kernel that visits them cell-coherently
+    // (FP32 quartic with fixed-point accumulators)
+    tl.fine = c->fp32 && c->fix32 && !q && c->cell_order > 0;
     tl.ntiles = 1;
     for (int a = 0; a < 3; a++) {
         tl.t[a] = t[a];
@@ -455,8 +456,8 @@ int pic_set_option(pic_ctx* c, int option, int64_t value) {
             return PIC_OK;
         case PIC_OPT_CELL_ORDER:
         case PIC_OPT_FP32_ACCUM: {
-            if (value != 0 && value != 1)
-                return set_err(c, PIC_ERR_ARG, option == PIC_OPT_CELL_ORDER ? "cell order must be 0 or 1"
+            if (option == PIC_OPT_CELL_ORDER ? (value < 0 || value > 1000) : (value != 0 && value != 1))
+                return set_err(c, PIC_ERR_ARG, option == PIC_OPT_CELL_ORDER ? "cell order must be 0 .. 1000"
                                                                             : "accumulator kind must be 0 or 1");
             if (option == PIC_OPT_CELL_ORDER) {
                 c->cell_order = (int)value;
@@ -629,6 +630,7 @@ int pic_particles_set(pic_ctx* c, int s, int64_t n, const double* x, const doubl
     PIC_HIP(c, hipStreamSynchronize(c->stream));
     S.sorted = 0;
     S.scrambled = 0;
+    S.cellsorted = 0;
     return PIC_OK;
 }
 
@@ -696,6 +698,7 @@ int pic_particles_load(pic_ctx* c, int s, const pic_loader* l) {
     dfree(d_pre);
     S.sorted = 0;
     S.scrambled = 0;
+    S.cellsorted = 0;
     return r;
 }
 
@@ -722,6 +725,7 @@ int pic_particles_lattice(pic_ctx* c, int s, const pic_lattice* l) {
     PIC_HIP(c, hipStreamSynchronize(c->stream));
     S.sorted = 0;
     S.scrambled = 0;
+    S.cellsorted = 0;
     return PIC_OK;
 }
 

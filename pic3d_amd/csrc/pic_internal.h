@@ -79,7 +79,9 @@ struct Species {
     void* spin_v[3][2] = {};
     int sorted = 0;               // tile_off valid for the current order
     int scrambled = 0;            // the in-tile order has been transposed (stored_slot)
-    int cellsorted = 0;           // the last sort binned by cell (cell order inside tiles)
+    int cellsorted = 0;           // tiles hold their particles in cell order (from the last
+                                  // cell-bin sort, kept by the tile sorts since)
+    int sorts_since_fine = 0;     // tile sorts since the last cell-bin sort
 };
 
 // d_cnt layout.  CNT_PEND: FP32 pushes classify boundary crossings as they
@@ -128,7 +130,7 @@ struct pic_ctx {
     int keep_old = 0;
     int xmax_face = 0;            // PIC_OPT_XMAX_BOUNDARY
     int fix32 = 0;                // FP32: int32 fixed-point LDS accumulators
-    int cell_order = 1;           // PIC_OPT_CELL_ORDER: cell-sorted tiles + cell-coherent push
+    int cell_order = 8;           // PIC_OPT_CELL_ORDER: 0 off; N: cell bins every N-th sort
     void* jtmp = nullptr;         // J-sized scratch of the current filter (lazy)
     void* stage = nullptr;        // output staging buffer (lazy, k_stage.hip)
     size_t stage_bytes = 0;
