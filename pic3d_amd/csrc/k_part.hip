@@ -320,10 +320,62 @@ __global__ __launch_bounds__(1024) void k_work(const int64_t* __restrict__ off, 
     if (t == T - 1) *nwork = min(part[T - 1], cap);
 }
 
-int build_work(pic_ctx* c, int s, int chunk) {
-    return build_work_on(c, c->sp[s].tile_off, chunk, c->work, c->nwork, c->stream);
+// The same list from three grid-wide launches: per block of 1024 tiles,
+// each tile's item count and its exclusive scan inside the block (wk_loc)
+// and the block's total (wk_bsum); one workgroup scans the block totals;
+// every tile then writes its items at its block's offset plus its own.
+constexpr int WORK_B = 1024;
+
+__global__ __launch_bounds__(WORK_B) void k_work_count(const int64_t* __restrict__ off, int ntiles, int chunk,
+                                                       int64_t* __restrict__ loc, int64_t* __restrict__ bsum) {
+    __shared__ int64_t wsum[WORK_B / 64];
+    const int t = blockIdx.x * WORK_B + threadIdx.x;
+    const int lane = lane_id(), w = threadIdx.x / 64;
+    const int64_t n = t < ntiles ? (off[t + 1] - off[t] + chunk - 1) / chunk : 0;
+    const int64_t x = wave_incl_scan(n);
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int64_t v = lane < WORK_B / 64 ? wsum[lane] : 0;
+        v = wave_incl_scan(v);
+        if (lane < WORK_B / 64) wsum[lane] = v;
+    }
+    __syncthreads();
+    if (t < ntiles) loc[t] = (w ? wsum[w - 1] : 0) + x - n;
+    if (threadIdx.x == WORK_B - 1) bsum[blockIdx.x] = wsum[WORK_B / 64 - 1];
 }
 
+__global__ __launch_bounds__(WORK_B) void k_work_write(const int64_t* __restrict__ off, int ntiles, int chunk,
+                                                       const int64_t* __restrict__ loc,
+                                                       const int64_t* __restrict__ bsum, int nblocks,
+                                                       int64_t* __restrict__ work, int64_t* __restrict__ nwork,
+                                                       int64_t cap) {
+    const int t = blockIdx.x * WORK_B + threadIdx.x;
+    if (t == 0) *nwork = min(bsum[nblocks], cap);  // bsum holds the exclusive scan and the total
+    if (t >= ntiles) return;
+    const int64_t k = (off[t + 1] - off[t] + chunk - 1) / chunk;
+    int64_t pos = bsum[blockIdx.x] + loc[t];
+    for (int64_t q = 0; q < k; q++, pos++)
+        if (pos < cap) work[pos] = ((int64_t)t << 20) | q;
+}
+
+int build_work(pic_ctx* c, int s, int chunk) {
+    const int nt = c->tl.ntiles;
+    const int nb = (nt + WORK_B - 1) / WORK_B;
+    if (nt > c->wk_cap) return build_work_on(c, c->sp[s].tile_off, chunk, c->work, c->nwork, c->stream);
+    const int64_t* off = c->sp[s].tile_off;
+    k_work_count<<<nb, WORK_B, 0, c->stream>>>(off, nt, chunk, c->wk_loc, c->wk_bsum);
+    PIC_LAUNCH_CHECK(c);
+    k_scan2<<<1, 1024, 0, c->stream>>>(c->wk_bsum, nb);  // exclusive, total at wk_bsum[nb]
+    PIC_LAUNCH_CHECK(c);
+    k_work_write<<<nb, WORK_B, 0, c->stream>>>(off, nt, chunk, c->wk_loc, c->wk_bsum, nb, c->work, c->nwork,
+                                               c->work_cap);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+// One workgroup, no scratch: the differentiated run builds its adjoint
+// work lists with it on its own stream, beside the forward's.
 int build_work_on(pic_ctx* c, const int64_t* tile_off, int chunk, int64_t* work, int64_t* nwork,
                   hipStream_t stream) {
     k_work<<<1, 1024, 0, stream>>>(tile_off, c->tl.ntiles, chunk, work, nwork, c->work_cap);

@@ -66,6 +66,18 @@ int alloc_sort_buffers(pic_ctx* c) {
         hipMalloc((void**)&c->scan_sums, (size_t)nb * 8) != hipSuccess)
         return set_err(c, PIC_ERR_NOMEM, "sort buffers for %lld bins", (long long)bins);
     c->sort_cap = bins;
+    if (c->tl.ntiles > c->wk_cap) {  // the work-list scan (k_part.hip build_work)
+        for (int64_t** p : {&c->wk_loc, &c->wk_bsum}) {
+            if (*p) hipFree(*p);
+            *p = nullptr;
+        }
+        c->wk_cap = 0;
+        const int64_t nt = c->tl.ntiles;
+        if (hipMalloc((void**)&c->wk_loc, (size_t)(nt + 1) * 8) != hipSuccess ||
+            hipMalloc((void**)&c->wk_bsum, (size_t)(nt / 1024 + 2) * 8) != hipSuccess)
+            return set_err(c, PIC_ERR_NOMEM, "work-list scratch for %lld tiles", (long long)nt);
+        c->wk_cap = nt;
+    }
     return PIC_OK;
 }
 
@@ -408,6 +420,7 @@ int pic_ctx_destroy(pic_ctx* c) {
     for (int f = 0; f < 6; f++) if (c->F[f]) hipFree(c->F[f]);
     for (int f = 0; f < 3; f++) if (c->J[f]) hipFree(c->J[f]);
     dfree(c->tile_cnt); dfree(c->tile_cur); dfree(c->scan_sums);
+    dfree(c->wk_loc); dfree(c->wk_bsum);
     dfree(c->work); dfree(c->nwork); dfree(c->diag_buf);
     dfree(c->rho_prev); dfree(c->gauss_ref);
     dfree(c->jtmp);

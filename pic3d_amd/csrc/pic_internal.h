@@ -117,6 +117,9 @@ struct pic_ctx {
     int64_t* work = nullptr;      // push work items (tile << 20 | chunk)
     int64_t work_cap = 0;
     int64_t* nwork = nullptr;     // number of work items (device)
+    int64_t* wk_loc = nullptr;    // work-list scan scratch: per tile, its first item inside its block
+    int64_t* wk_bsum = nullptr;   // per block of 1024 tiles, its item count, then scanned
+    int64_t wk_cap = 0;           // tiles the work-list scratch holds
     int sort_interval = 1;        // re-sort the tiled species every N pushes
     double* diag_buf = nullptr;   // diagnostics reduction buffer
     // Gauss-law diagnostics: the mobile charge as it stood at the last
