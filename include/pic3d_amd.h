@@ -202,13 +202,16 @@ enum pic_option {
                                   write: the last dual column n+mem-1 from
                                   E, Bx and J at the primal column n+mem-1
                                   and By, Bz at dual column n+mem-2 */
-    PIC_OPT_CELL_ORDER = 8     /* FP32 quartic with fixed-point accumulators:
-                                  N >= 1 keeps each tile's particles in cell
-                                  order (every N-th sort bins by cell, the tile
-                                  sorts in between keep the order; default 8),
-                                  and the push gathers with the lanes of a wave
-                                  in neighbouring cells, then deposits from an
-                                  LDS stage with its lanes spread over cells;
+    PIC_OPT_CELL_ORDER = 8     /* FP64, and FP32 quartic with fixed-point
+                                  accumulators: N >= 1 keeps each tile's
+                                  particles in cell order (every N-th sort bins
+                                  by cell, the tile sorts in between keep the
+                                  order; default 8), and the tiled push gathers
+                                  with the lanes of a wave in neighbouring cells
+                                  and deposits with its lanes spread over cells
+                                  (FP64: two kernels, the deposit reading the
+                                  old positions the gather kernel wrote; FP32:
+                                  one kernel in two phases over an LDS stage);
                                   0: tile bins (chunks stored transposed) and
                                   one pass with scattered lanes.  Results are
                                   the same up to the order of the fixed-point

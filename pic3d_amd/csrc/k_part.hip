@@ -21,13 +21,15 @@ struct TileArgs {
     int n[3];
     Tiling tl;
     FastDiv td[3];  // tile edges
+    int cpt;        // bins per tile: 1, or its cells (cell order inside tiles)
 };
 
-// sort bin of a particle: its tile
+// sort bin of a particle: its tile, or (cpt > 1) the tile's first bin plus
+// the cell's index inside the tile, x fastest
 __device__ __forceinline__ int64_t bin_of64(double x, double y, double z, const TileArgs& A) {
     double p[3] = {x, y, z};
     const bool out = x < 0.0 || x > A.N[0] || y < 0.0 || y > A.N[1] || z < 0.0 || z > A.N[2];
-    int t[3];
+    int t[3], l[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         double v = p[a];
@@ -35,8 +37,11 @@ __device__ __forceinline__ int64_t bin_of64(double x, double y, double z, const 
         int c = (int)floor(v);
         c = c < 0 ? 0 : (c >= A.n[a] ? A.n[a] - 1 : c);
         t[a] = A.td[a].div(c);
+        l[a] = c - t[a] * A.tl.t[a];
     }
-    return (t[2] * A.tl.nt[1] + t[1]) * A.tl.nt[0] + t[0];
+    const int64_t tile = (t[2] * A.tl.nt[1] + t[1]) * A.tl.nt[0] + t[0];
+    if (A.cpt == 1) return tile;
+    return tile * A.cpt + (l[2] * A.tl.t[1] + l[1]) * A.tl.t[0] + l[0];
 }
 
 struct Soa64 {
@@ -203,6 +208,7 @@ static TileArgs tile_args(const pic_ctx* c) {
         T.td[a] = FastDiv(c->tl.t[a]);
     }
     T.tl = c->tl;
+    T.cpt = 1;
     return T;
 }
 
@@ -253,12 +259,22 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
     }
 }
 
-int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int blocks, int32_t* perm) {
-    const int scramble = !S.scrambled;
+int launch_scatter64(pic_ctx* c, Species& S, Soa64 A, Soa64 B, TileArgs T, int blocks, int32_t* perm,
+                     bool cellmode) {
+    // in cell mode no chunk is transposed (a tile sort outside it
+    // transposes the chunks once again)
+    const int scramble = cellmode ? 0 : !S.scrambled;
     k_scatter64_dev<<<blocks, 256, 0, c->stream>>>(A, B, S.d_cnt, c->tile_cur, T,
                                                    c->keep_old && S.xo[0] && S.xo[1], perm, S.tile_off, scramble);
     PIC_LAUNCH_CHECK(c);
     if (scramble) S.scrambled = 1;
+    if (cellmode) {
+        S.scrambled = 0;
+        S.cellsorted = 1;
+        S.sorts_since_fine = T.cpt > 1 ? 0 : S.sorts_since_fine + 1;
+    } else {
+        S.cellsorted = 0;
+    }
     S.cur = 1 - S.cur;
     S.sorted = 1;
     S.pushes_since_sort = 0;
@@ -274,10 +290,17 @@ int sort32(pic_ctx* c, int s, int32_t* perm);  // k_push32.hip
 int launch_sort(pic_ctx* c, int s, int32_t* perm) {
     Species& S = c->sp[s];
     if (c->fp32) return sort32(c, s, perm);
-    const int64_t nbins = c->tl.nbins() + (perm ? 1 : 0);
+    // Cell order for the two-kernel FP64 push: every cell_order-th sort bins
+    // by cell, the tile sorts in between keep each tile's relative order.
+    // Differentiated runs (which record and replay tile-sort permutations)
+    // keep tile bins.
+    const bool cellmode = c->tl.fine && !perm && !c->diff;
+    const bool fine = cellmode && (!S.cellsorted || S.sorts_since_fine + 1 >= c->cell_order);
+    TileArgs T = tile_args(c);
+    T.cpt = fine ? c->tl.cells() : 1;
+    const int64_t nbins = c->tl.nbins() * T.cpt + (perm ? 1 : 0);
     int r = launch_fill(c, c->tile_cnt, nbins * 8);
     if (r) return r;
-    TileArgs T = tile_args(c);
     const int blocks = (int)std::min<int64_t>(blocks_for(std::max<int64_t>(S.cap, 1), 256), 256 * 32);
     Soa64 A = soa64(c, S, S.cur), B = soa64(c, S, 1 - S.cur);
     k_hist64<<<blocks, 256, 0, c->stream>>>(A, S.d_cnt, c->tile_cnt, T, perm ? 1 : 0);
@@ -285,9 +308,9 @@ int launch_sort(pic_ctx* c, int s, int32_t* perm) {
     // the scatter needs the pre-sort slot count: keep it in d_cnt[CNT_N-1]
     PIC_HIP(c, hipMemcpyAsync(S.d_cnt + (CNT_N - 1), S.d_cnt + CNT_SLOTS, 8,
                               hipMemcpyDeviceToDevice, c->stream));
-    r = launch_scan_bins(c, S, nbins);
+    r = launch_scan_bins(c, S, nbins, T.cpt);
     if (r) return r;
-    return launch_scatter64(c, S, A, B, T, blocks, perm);
+    return launch_scatter64(c, S, A, B, T, blocks, perm, cellmode);
 }
 
 // ---------------------------------------------------------------------

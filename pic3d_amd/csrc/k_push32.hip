@@ -611,7 +611,7 @@ __global__ __launch_bounds__((Push32Threads<M, ACC, SPIN>::N), 2) void k_push32_
 //   costs about twice the LDS clocks).  Gather, Boris, move and the
 //   particle write-back are coalesced; the deposit inputs go to an LDS
 //   stage.
-//  phase 2: thread t deposits stage entry coh_pick(t): each 16-lane group
+//  phase 2: thread t deposits stage entry spread_pick(t): each 16-lane group
 //   holds entries 16 slots apart, in different cells, so one atomic
 //   instruction rarely hits an address twice (same-address LDS atomics
 //   serialise, which made a single cell-ordered pass slower than the
@@ -623,11 +623,6 @@ __global__ __launch_bounds__((Push32Threads<M, ACC, SPIN>::N), 2) void k_push32_
 // ---------------------------------------------------------------------
 constexpr int COH_NT = 256;
 constexpr int COH_WORDS = 8;
-// stage entry read by thread t in phase 2: wave w, lane l -> w + 4 (l / 16) + 16 (l % 16)
-__device__ __forceinline__ int coh_pick(int t) {
-    const int w = t >> 6, l = t & 63;
-    return w + 4 * (l >> 4) + 16 * (l & 15);
-}
 // word of entry e: bits 0, 1 and 3 of e xor-ed with bits 5, 6 and 7, so
 // that entries e..e+31 (phase 1) and the 32 entries of a phase-2 half-wave
 // (bits 2, 4, 5, 6, 7 varying) each map onto 32 distinct banks
@@ -640,7 +635,7 @@ __global__ __launch_bounds__(COH_NT, 3) void k_push32_coh(Soa32 P, Fields32 F, G
                                                            const int64_t* __restrict__ nwork, int chunk,
                                                            int64_t* __restrict__ cnt) {
     using B = Box32<M, TX, TY, TZ, int>;
-    static_assert(COH_NT == 256, "coh_pick and coh_addr assume 256 entries");
+    static_assert(COH_NT == 256, "spread_pick and coh_addr assume 256 entries");
     extern __shared__ double lds_d[];
     int* acc = reinterpret_cast<int*>(lds_d);                    // 4 * AVOL accumulators
     float* fld = reinterpret_cast<float*>(acc + 4 * B::AVOL);    // the six field boxes
@@ -680,7 +675,7 @@ __global__ __launch_bounds__(COH_NT, 3) void k_push32_coh(Soa32 P, Fields32 F, G
     int spin_bad = 0;
     uint32_t strays = 0;  // passes whose particle was outside the tile
     const int len = (int)(end - beg);
-    const int sw = coh_addr(tid), sr = coh_addr(coh_pick(tid));
+    const int sw = coh_addr(tid), sr = coh_addr(spread_pick(tid));
     // a pass's particle loads are issued during the previous pass's
     // deposit phase, so their latency hides behind its LDS atomics
     int pc = -1;

@@ -43,7 +43,13 @@ __device__ __forceinline__ void copy_arr(const double* S, double* D) {
 // The primal weights at the old position and their centre nodes are
 // returned in (Wxp, Wyp, Wzp, ip, jp, kp): the deposit's old-position
 // weights are the same values (the same function of the same argument).
-template <int M, typename FR>
+//
+// LOWREG: the k loop recomputes its WX[i] * WY[j] products instead of
+// holding the 25 (quartic) k-invariant ones across it (an empty asm
+// statement makes the x weights opaque at each iteration, so they are not
+// hoisted).  The same products, the same values: 50 fewer live registers,
+// which lets the gather-only kernel run three waves per SIMD.
+template <int M, typename FR, bool LOWREG = false>
 __device__ __forceinline__ void gather_boris(double x, double y, double z, double& ux, double& uy,
                                              double& uz, double& xn, double& yn, double& zn,
                                              const PushConsts& K, int m, FR fr, double* Wxp, double* Wyp,
@@ -63,15 +69,20 @@ __device__ __forceinline__ void gather_boris(double x, double y, double z, doubl
     auto comp = [&](int c, const double* WX, const double* WY, const double* WZ, int b0, int b1,
                     int b2) -> double {
         double e = 0.0;
-        double wzr[M + 1];
+        double wzr[M + 1], wx[M + 1];
         copy_arr<M + 1>(WZ, wzr);
+        copy_arr<M + 1>(WX, wx);
 #pragma unroll 1
         for (int k = 0; k <= M; k++) {
             const double wz = wzr[0];
+            if constexpr (LOWREG) {
+#pragma unroll
+                for (int t = 0; t <= M; t++) asm volatile("" : "+v"(wx[t]));
+            }
 #pragma unroll
             for (int j = 0; j <= M; j++)
 #pragma unroll
-                for (int i = 0; i <= M; i++) e += WX[i] * WY[j] * wz * fr(c, b2 + k - H, b1 + j - H, b0 + i - H);
+                for (int i = 0; i <= M; i++) e += wx[i] * WY[j] * wz * fr(c, b2 + k - H, b1 + j - H, b0 + i - H);
             shift_down<M + 1>(wzr);
         }
         return e;
@@ -323,7 +334,7 @@ struct LdsSink {
 // Full push of one particle against HBM fields and HBM currents (the
 // direct kernel, and the tiled kernel's path for a particle outside its
 // tile).  Returns 1 if the particle was skipped as out of bounds.
-template <int M, bool SPIN>
+template <int M, bool SPIN, bool LOWREG = false>
 __device__ __forceinline__ int push_one_hbm(const P64& P, int64_t p, double wp, const Fields64& F,
                                          const Grid& g, const PushConsts& K, int& spin_bad) {
     double x = P.x[p], y = P.y[p], z = P.z[p];
@@ -334,7 +345,8 @@ __device__ __forceinline__ int push_one_hbm(const P64& P, int64_t p, double wp, 
     double Sx0[M + 1], Sy0[M + 1], Sz0[M + 1];
     int i1, j1, k1;
     double ebq[6], gam;
-    gather_boris<M>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr, Sx0, Sy0, Sz0, i1, j1, k1, ebq, gam);
+    gather_boris<M, decltype(fr), LOWREG>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr, Sx0, Sy0, Sz0, i1, j1, k1,
+                                          ebq, gam);
     P.x[p] = xn; P.y[p] = yn; P.z[p] = zn;
     P.ux[p] = ux; P.uy[p] = uy; P.uz[p] = uz;
     if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
@@ -589,6 +601,187 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
 }
 
 // ---------------------------------------------------------------------
+// Two-kernel push over cell-ordered tiles (the sort keeps each tile's
+// particles in cell order).
+//  k_push64_gather: gather, Boris and advance of a tile chunk's particles
+//   in storage order, the field tile alone in LDS.  The 64 lanes of a wave
+//   hold the particles of a few neighbouring cells, so most of a lane's
+//   field reads fall on the LDS words its cell-mates read (one broadcast
+//   read, no bank conflict).  It writes the particle state back and, for
+//   every particle the deposit kernel is to deposit, its wrapped old
+//   position (dxo, dyo, dzo; NaN: nothing to deposit).  A particle outside
+//   its tile takes the HBM path of k_push64_tiled, deposit included.
+//  k_push64_deposit: the Esirkepov deposit of those particles into the
+//   accumulator box alone in LDS.  Per pass of 256 slots, thread t takes
+//   slot spread_pick(t), so a wave's lanes sit in different cells, then
+//   the particles are dealt by the bank of their deposit base.
+// Each kernel holds half of the one-kernel form's LDS and fewer registers,
+// so more workgroups share a CU.  The particle arithmetic is the same code:
+// the deposit's old-position weights come from the same shape function of
+// the same argument.
+// ---------------------------------------------------------------------
+template <int M, int TX, int TY, int TZ, bool SPIN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_push64_gather(
+    P64 P, Fields64 F, Grid g, PushConsts K, Tiling tl, const int64_t* __restrict__ tile_off,
+    const int64_t* __restrict__ work, const int64_t* __restrict__ nwork, int chunk, double* __restrict__ dxo,
+    double* __restrict__ dyo, double* __restrict__ dzo) {
+    using B = AccBox<M, TX, TY, TZ>;
+    extern __shared__ double lds64[];
+    double* fld = lds64;  // FVOL: the tile's fields
+    const int64_t wi = blockIdx.x;
+    if (wi >= *nwork) return;
+    const int64_t item = work[wi];
+    const int tile = (int)(item >> 20);
+    const int64_t cidx = item & ((1 << 20) - 1);
+    const int64_t beg = tile_off[tile] + cidx * chunk;
+    const int64_t end = min(tile_off[tile + 1], beg + chunk);
+    const int tzi = tile / (tl.nt[0] * tl.nt[1]);
+    const int tyi = (tile / tl.nt[0]) % tl.nt[1];
+    const int txi = tile % tl.nt[0];
+    const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
+    const double* fp[6] = {F.ex, F.ey, F.ez, F.bx, F.by, F.bz};
+    const int fox = T0x + g.m - 1 - B::H, foy = T0y + g.m - 1 - B::H, foz = T0z + g.m - 1 - B::H;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const int ex = B::fext(c, 0), ey = B::fext(c, 1), n = ex * ey * B::fext(c, 2);
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int lx = e % ex, ly = (e / ex) % ey, lz = e / (ex * ey);
+            const int gx = min(fox + lx, g.fe[0] - 1), gy = min(foy + ly, g.fe[1] - 1), gz = min(foz + lz, g.fe[2] - 1);
+            fld[B::foff(c) + lz * B::fpz(c) + ly * B::FPY + lx] = fp[c][g.fi(gz, gy, gx)];
+        }
+    }
+    __syncthreads();
+    auto fr = [&](int c, int k, int j, int i) -> double {
+        return fld[B::foff(c) + (k - foz) * B::fpz(c) + (j - foy) * B::FPY + (i - fox)];
+    };
+    int64_t oob = 0;
+    int spin_bad = 0;
+    uint32_t strays = 0;  // passes whose particle was outside the tile
+    const int len = (int)(end - beg);
+    const double org = (double)(1 - g.m);
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    int it = 0;
+    for (int t0 = 0; t0 < len; t0 += blockDim.x, it++) {
+        const int t = t0 + threadIdx.x;
+        if (t >= len) continue;
+        const int64_t p = beg + t;
+        const double wp = P.w[p];
+        double x = P.x[p], y = P.y[p], z = P.z[p];
+        double ox = nan, oy = nan, oz = nan;
+        bool ok = wp == wp;
+        if (ok && !wrap_check(x, y, z, K)) { oob++; ok = false; }
+        if (ok) {
+            // the cell from the shape functions' own argument (k_push64_tiled)
+            const int cx = (int)(x - org) - (g.m - 1), cy = (int)(y - org) - (g.m - 1),
+                      cz = (int)(z - org) - (g.m - 1);
+            if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
+                (unsigned)(cz - T0z) >= (unsigned)TZ) {
+                strays |= 1u << it;  // left its tile since the last sort: HBM pass below
+                ok = false;
+            }
+        }
+        if (ok) {
+            double ux = P.ux[p], uy = P.uy[p], uz = P.uz[p], xn, yn, zn;
+            double Sx0[M + 1], Sy0[M + 1], Sz0[M + 1];
+            int i1, j1, k1;
+            double ebq[6], gam;
+            gather_boris<M, decltype(fr), true>(x, y, z, ux, uy, uz, xn, yn, zn, K, g.m, fr, Sx0, Sy0, Sz0, i1, j1,
+                                                k1, ebq, gam);
+            P.x[p] = xn; P.y[p] = yn; P.z[p] = zn;
+            P.ux[p] = ux; P.uy[p] = uy; P.uz[p] = uz;
+            if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
+            if (SPIN) spin_bad += spin_step(P, p, ebq, ux, uy, uz, gam, K);
+            // a non-finite momentum would put the deposit box outside the arrays
+            if (fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0) { ox = x; oy = y; oz = z; }
+            else oob++;
+        }
+        dxo[p] = ox; dyo[p] = oy; dzo[p] = oz;
+    }
+    if (strays) {
+        it = 0;
+        for (int t = threadIdx.x; t < len; t += blockDim.x, it++)
+            if (strays & (1u << it)) {
+                const int64_t p = beg + t;
+                oob += push_one_hbm<M, SPIN, true>(P, p, P.w[p], F, g, K, spin_bad);
+            }
+    }
+    if (oob) atomicAdd((unsigned long long*)&P.cnt[CNT_OOB], (unsigned long long)oob);
+    if (SPIN && spin_bad) atomicAdd((unsigned long long*)&P.cnt[CNT_SPIN], (unsigned long long)spin_bad);
+}
+
+template <int M, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_push64_deposit(
+    P64 P, Fields64 F, Grid g, PushConsts K, Tiling tl, const int64_t* __restrict__ tile_off,
+    const int64_t* __restrict__ work, const int64_t* __restrict__ nwork, int chunk,
+    const double* __restrict__ dxo, const double* __restrict__ dyo, const double* __restrict__ dzo) {
+    using B = AccBox<M, TX, TY, TZ>;
+    extern __shared__ double lds64[];
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds64);  // AVOL: J, rho
+    const int64_t wi = blockIdx.x;
+    if (wi >= *nwork) return;
+    const int64_t item = work[wi];
+    const int tile = (int)(item >> 20);
+    const int64_t cidx = item & ((1 << 20) - 1);
+    const int64_t beg = tile_off[tile] + cidx * chunk;
+    const int64_t end = min(tile_off[tile + 1], beg + chunk);
+    const int tzi = tile / (tl.nt[0] * tl.nt[1]);
+    const int tyi = (tile / tl.nt[0]) % tl.nt[1];
+    const int txi = tile % tl.nt[0];
+    const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
+    for (int e = threadIdx.x; e < B::AVOL; e += blockDim.x) acc[e] = 0ull;
+    __shared__ unsigned rg_hist[256 / 64][8];  // per wave: bank-spreading counters
+    if (lane_id() < 8) rg_hist[threadIdx.x >> 6][lane_id()] = 0u;
+    __syncthreads();
+    const int ox = T0x + g.m - 2 - B::H, oy = T0y + g.m - 2 - B::H, oz = T0z + g.m - 2 - B::H;
+    LdsSink<B> sink{acc, K.inv_unit, oz, oy, ox, 0};
+    const int len = (int)(end - beg);
+    const int wv = threadIdx.x >> 6;
+    const int pick = spread_pick(threadIdx.x);
+    const double org = (double)(1 - g.m);
+    for (int t0 = 0; t0 < len; t0 += blockDim.x) {
+        const int t = t0 + pick;
+        double x0 = 0.0, y0 = 0.0, z0 = 0.0, xn = 0.0, yn = 0.0, zn = 0.0, qw = 0.0;
+        if (t < len) {
+            const int64_t p = beg + t;
+            x0 = dxo[p];
+            if (x0 == x0) {
+                y0 = dyo[p]; z0 = dzo[p];
+                xn = P.x[p]; yn = P.y[p]; zn = P.z[p];
+                qw = K.q * P.w[p];
+            }
+        }
+        // deal by the bank of the deposit box base, the nearest node of the
+        // old position (q w == 0: nothing to add)
+        int r = 32;
+        if (qw != 0.0) {
+            const int ib = (int)(x0 - org + 0.5) - B::H, jb = (int)(y0 - org + 0.5) - B::H,
+                      kb = (int)(z0 - org + 0.5) - B::H;
+            r = (2 * ((kb - oz) * B::PXY + (jb - oy) * B::PX + (ib - ox))) & 31;
+        }
+        const int to = bank_spread_lane(r, rg_hist[wv]);
+        x0 = lane_send_d(to, x0); y0 = lane_send_d(to, y0); z0 = lane_send_d(to, z0);
+        xn = lane_send_d(to, xn); yn = lane_send_d(to, yn); zn = lane_send_d(to, zn);
+        qw = lane_send_d(to, qw);
+        // an in-tile particle that moved < 1 cell: its whole box lies in the LDS box
+        if (qw != 0.0) esirkepov<M>(qw, x0, y0, z0, xn, yn, zn, K, g.m, sink);
+    }
+    __syncthreads();
+    // flush: one global atomic per nonzero accumulator point
+    double* jp[4] = {F.jx, F.jy, F.jz, F.rho};
+    for (int e = threadIdx.x; e < B::AVOL; e += blockDim.x) {
+        const long long q = (long long)acc[e];
+        if (q == 0) continue;
+        const int c = e >= B::aoff(3) ? 3 : e >= B::aoff(2) ? 2 : e >= B::aoff(1) ? 1 : 0;
+        const int r = e - (c == 3 ? B::aoff(3) : c == 2 ? B::aoff(2) : c == 1 ? B::aoff(1) : 0);
+        const double v = (double)q * K.unit[c];
+        const int lx = r % B::PX, ly = (r / B::PX) % B::BY, lz = r / B::PXY;
+        const int gx = ox + lx, gy = oy + ly, gz = oz + lz;
+        if (gx > g.je[0] - 2 || gy > g.je[1] - 2 || gz > g.je[2] - 2) continue;
+        hbm_add(&jp[c][g.ji(gz, gy, gx)], v);
+    }
+}
+
+// ---------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------
 PushConsts push_consts64(const pic_ctx* c, const Species& S);
@@ -629,6 +822,25 @@ static int launch_tiled64(pic_ctx* c, Species& S, const P64& P, const Fields64& 
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
     k_push64_tiled<M, TS::X, TS::Y, TS::Z, SPIN><<<(unsigned)max_items, 256, B::LDS, c->stream>>>(
         P, Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK);
+    PIC_LAUNCH_CHECK(c);
+    return PIC_OK;
+}
+
+template <int M, bool SPIN>
+static int launch_split64(pic_ctx* c, Species& S, const P64& P, const Fields64& Fd, const PushConsts& K,
+                          int64_t max_items) {
+    using TS = TileShape<0, M>;
+    using B = AccBox<M, TS::X, TS::Y, TS::Z>;
+    constexpr size_t lds_g = (size_t)B::FVOL * sizeof(double), lds_d = (size_t)B::AVOL * sizeof(double);
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push64_gather<M, TS::X, TS::Y, TS::Z, SPIN>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_g));
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push64_deposit<M, TS::X, TS::Y, TS::Z>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_d));
+    k_push64_gather<M, TS::X, TS::Y, TS::Z, SPIN><<<(unsigned)max_items, 256, lds_g, c->stream>>>(
+        P, Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, S.dxo, S.dyo, S.dzo);
+    PIC_LAUNCH_CHECK(c);
+    k_push64_deposit<M, TS::X, TS::Y, TS::Z><<<(unsigned)max_items, 256, lds_d, c->stream>>>(
+        P, Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, S.dxo, S.dyo, S.dzo);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
@@ -688,6 +900,24 @@ int launch_push64(pic_ctx* c, int s) {
     S.pushes_since_sort++;
     const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / chunk + 1;
     const P64 P = p64(c, S);  // after the sort
+    if (S.cellsorted) {  // cell-ordered tiles: the two-kernel push
+        if (!S.dxo) {
+            const size_t bytes = (size_t)std::max<int64_t>(S.cap, 1) * sizeof(double);
+            if (hipMalloc((void**)&S.dxo, bytes) != hipSuccess || hipMalloc((void**)&S.dyo, bytes) != hipSuccess ||
+                hipMalloc((void**)&S.dzo, bytes) != hipSuccess) {
+                for (double** q : {&S.dxo, &S.dyo, &S.dzo}) {
+                    if (*q) (void)hipFree(*q);
+                    *q = nullptr;
+                }
+                return set_err(c, PIC_ERR_NOMEM, "deposit records for %lld particles", (long long)S.cap);
+            }
+        }
+        if (S.spin)
+            return c->g.m == 2 ? launch_split64<2, true>(c, S, P, Fd, K, max_items)
+                               : launch_split64<4, true>(c, S, P, Fd, K, max_items);
+        return c->g.m == 2 ? launch_split64<2, false>(c, S, P, Fd, K, max_items)
+                           : launch_split64<4, false>(c, S, P, Fd, K, max_items);
+    }
     if (S.spin)
         return c->g.m == 2 ? launch_tiled64<2, true>(c, S, P, Fd, K, max_items)
                            : launch_tiled64<4, true>(c, S, P, Fd, K, max_items);

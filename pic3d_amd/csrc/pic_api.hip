@@ -41,9 +41,9 @@ void set_tiling(pic_ctx* c) {
                  t[2] = q ? TileShape<2, 2>::Z : TileShape<2, 4>::Z; break;
     }
     Tiling& tl = c->tl;
-    // cell-sorted tiles for the push kernel that visits them cell-coherently
-    // (FP32 quartic with fixed-point accumulators)
-    tl.fine = c->fp32 && c->fix32 && !q && c->cell_order > 0;
+    // cell-sorted tiles for the push kernels that visit them cell-coherently
+    // (FP64; FP32 quartic with fixed-point accumulators)
+    tl.fine = c->cell_order > 0 && (!c->fp32 || (c->fix32 && !q));
     tl.ntiles = 1;
     for (int a = 0; a < 3; a++) {
         tl.t[a] = t[a];
@@ -148,6 +148,7 @@ void free_species(pic_ctx* c, Species& S) {
     }
     dfree(S.d_cnt);
     dfree(S.tile_off);
+    dfree(S.dxo); dfree(S.dyo); dfree(S.dzo);
     S = Species();
     (void)c;
 }

@@ -65,6 +65,9 @@ struct Species {
     float* vx[2] = {}; float* vy[2] = {}; float* vz[2] = {}; float* fw[2] = {};
     int64_t* id[2] = {};
     double* xo[2] = {}; double* yo[2] = {}; double* zo[2] = {};  // optional x_old (FP64)
+    // FP64 two-kernel push: the wrapped old position of each slot the
+    // deposit kernel deposits (NaN: nothing to deposit), by slot
+    double* dxo = nullptr; double* dyo = nullptr; double* dzo = nullptr;
     int pushes_since_sort = 0;
     int cur = 0;             // which buffer is current
     // device counters: [0] slots in use, [1] live particles,

@@ -71,6 +71,16 @@ __device__ __forceinline__ int bank_spread_lane(int r, unsigned* hist) {
     return q * 16 + (pos >> 2);
 }
 
+// Lanes of a 256-thread workgroup spread over the cells of a pass of 256
+// cell-ordered slots: thread t (wave w, lane l) takes slot w + 4 (l / 16) +
+// 16 (l % 16) of the pass, so the 16 lanes of a group hold slots 16 apart
+// (different cells at up to 16 particles per cell) and a wave's slots lie
+// within the pass.
+__device__ __forceinline__ int spread_pick(int t) {
+    const int w = t >> 6, l = t & 63;
+    return w + 4 * (l >> 4) + 16 * (l & 15);
+}
+
 // value v of this lane sent to lane 'to' (a permutation of the wave)
 __device__ __forceinline__ int lane_send(int to, int v) { return __builtin_amdgcn_ds_permute(to << 2, v); }
 __device__ __forceinline__ float lane_send(int to, float v) {
