@@ -80,7 +80,7 @@ VECTOR_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}  # MI355X vector (non-matrix)
 
 
 def load_util(name: str):
-    """VALU / LDS utilisation of the push kernel from the committed
+    """VALU / LDS utilisation of the push kernels from the committed
     rocprofv3 --pmc summary (rocprofv3's derived VALUBusy, LdsUtil,
     LDSBankConflict; executed FP instructions per push), or None when that
     profile is absent.  These counters cannot be read from inside the timed
@@ -95,7 +95,7 @@ def load_util(name: str):
             for part in line[len("mean: "):].split(","):
                 k, v = part.split()[:2]
                 out[k] = round(float(v) / 100.0, 4)
-        elif line.startswith("flop per dispatch"):
+        elif line.startswith("flop per dispatch") or line.startswith("flop per push call"):
             out["flop_per_push"] = float(line.split("=")[2].split()[0])
     return out
 

@@ -602,54 +602,45 @@ __global__ __launch_bounds__((Push32Threads<M, ACC, SPIN>::N), 2) void k_push32_
 }
 
 // ---------------------------------------------------------------------
-// cell-coherent tiled push (fixed-point accumulators; tiles kept in cell
-// order by the sort).  A workgroup pass takes NT consecutive slots:
-//  phase 1: thread t pushes slot t0 + t.  The 64 lanes of a wave hold the
-//   particles of a few neighbouring cells, so a lane's field reads mostly
-//   fall on the same LDS words as its cell-mates' (one broadcast read, no
-//   bank conflict; with lanes in 64 different cells a read instruction
-//   costs about twice the LDS clocks).  Gather, Boris, move and the
-//   particle write-back are coalesced; the deposit inputs go to an LDS
-//   stage.
-//  phase 2: thread t deposits stage entry spread_pick(t): each 16-lane group
-//   holds entries 16 slots apart, in different cells, so one atomic
-//   instruction rarely hits an address twice (same-address LDS atomics
-//   serialise, which made a single cell-ordered pass slower than the
-//   scattered one).
-// Stage entry: f0[3], f1[3] (old offsets, unwrapped new offsets), q w, the
-// old cell's accumulator box offset (-1: nothing to deposit).  The stage
-// is indexed through coh_addr, which keeps both phases' 32-lane halves on
-// 32 distinct banks.
+// Two-kernel push over cell-ordered tiles (fixed-point accumulators).
+//  k_push32_gat: gather, Boris, move, spin and the write-back of a tile
+//   chunk's particles in storage order, the field tile alone in LDS; the
+//   lanes of a wave hold the particles of a few neighbouring cells (the
+//   field reads mostly broadcast).  Per slot it writes a deposit record:
+//   old cell (-1: nothing to deposit), old offsets, unwrapped new offsets.
+//   A particle outside its tile takes the HBM path, deposit included.
+//  k_push32_dep: the deposit of those records into the accumulator box
+//   alone in LDS; per pass of 256 slots thread t takes slot spread_pick(t),
+//   then the particles are dealt by the bank of their deposit base.
+// Each kernel holds half of the LDS and fewer registers than the one-kernel
+// forms, so more waves share a CU.
 // ---------------------------------------------------------------------
-constexpr int COH_NT = 256;
-constexpr int COH_WORDS = 8;
-// word of entry e: bits 0, 1 and 3 of e xor-ed with bits 5, 6 and 7, so
-// that entries e..e+31 (phase 1) and the 32 entries of a phase-2 half-wave
-// (bits 2, 4, 5, 6, 7 varying) each map onto 32 distinct banks
-__device__ __forceinline__ int coh_addr(int e) { return e ^ ((e >> 5) & 3) ^ ((e >> 4) & 8); }
+struct Rec32 {
+    int32_t* cell;
+    float *f0x, *f0y, *f0z, *f1x, *f1y, *f1z;
+};
+static Rec32 rec32(Species& S) {
+    const int64_t n = std::max<int64_t>(S.cap, 1);
+    float* b = (float*)S.rec32;
+    return Rec32{(int32_t*)b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n, b + 6 * n};
+}
 
 template <int M, int TX, int TY, int TZ, bool SPIN>
-__global__ __launch_bounds__(COH_NT, 3) void k_push32_coh(Soa32 P, Fields32 F, Grid g, Consts32 K,
-                                                           Tiling tl, const int64_t* __restrict__ tile_off,
-                                                           const int64_t* __restrict__ work,
-                                                           const int64_t* __restrict__ nwork, int chunk,
-                                                           int64_t* __restrict__ cnt) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_push32_gat(
+    Soa32 P, Fields32 F, Grid g, Consts32 K, Tiling tl, const int64_t* __restrict__ tile_off,
+    const int64_t* __restrict__ work, const int64_t* __restrict__ nwork, int chunk, Rec32 R,
+    int64_t* __restrict__ cnt) {
     using B = Box32<M, TX, TY, TZ, int>;
-    static_assert(COH_NT == 256, "spread_pick and coh_addr assume 256 entries");
     extern __shared__ double lds_d[];
-    int* acc = reinterpret_cast<int*>(lds_d);                    // 4 * AVOL accumulators
-    float* fld = reinterpret_cast<float*>(acc + 4 * B::AVOL);    // the six field boxes
-    float* stg = fld + B::FTOT;                                   // COH_WORDS x COH_NT stage
+    float* fld = reinterpret_cast<float*>(lds_d);  // the six field boxes
     __shared__ unsigned cnt12[12];
-    __shared__ unsigned rg_hist[COH_NT / 64][8];  // per wave: bank-spreading counters
     const int64_t wi = blockIdx.x;
     if (wi >= *nwork) return;
     const int64_t item = work[wi];
     const int tile = (int)(item >> 20);
     const int64_t cidx = item & ((1 << 20) - 1);
-    const int64_t t_end = tile_off[tile + 1];
     const int64_t beg = tile_off[tile] + cidx * chunk;
-    const int64_t end = min(t_end, beg + chunk);
+    const int64_t end = min(tile_off[tile + 1], beg + chunk);
     const int txi = tile % tl.nt[0], tyi = (tile / tl.nt[0]) % tl.nt[1], tzi = tile / (tl.nt[0] * tl.nt[1]);
     const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
     const int m = g.m;
@@ -665,119 +656,65 @@ __global__ __launch_bounds__(COH_NT, 3) void k_push32_coh(Soa32 P, Fields32 F, G
         }
     };
     load(0, F.ex); load(1, F.ey); load(2, F.ez); load(3, F.bx); load(4, F.by); load(5, F.bz);
-    for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) acc[e] = 0;
     if (threadIdx.x < 12) cnt12[threadIdx.x] = 0;
-    const int tid = threadIdx.x, wv = tid >> 6, lane = lane_id();
-    if (lane < 8) rg_hist[wv][lane] = 0u;
     __syncthreads();
-    const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
     int64_t oob = 0;
     int spin_bad = 0;
     uint32_t strays = 0;  // passes whose particle was outside the tile
     const int len = (int)(end - beg);
-    const int sw = coh_addr(tid), sr = coh_addr(spread_pick(tid));
-    // a pass's particle loads are issued during the previous pass's
-    // deposit phase, so their latency hides behind its LDS atomics
-    int pc = -1;
-    float pf[3] = {0.0f, 0.0f, 0.0f}, pu[3] = {0.0f, 0.0f, 0.0f}, pw = 0.0f, ps[3] = {0.0f, 0.0f, 0.0f};
-    auto fetch = [&](int t) {
-        pc = -1;
-        if (t < len) {
-            const int64_t p = beg + t;
-            pc = P.cell[p];
-            pf[0] = P.fx[p]; pf[1] = P.fy[p]; pf[2] = P.fz[p];
-            pu[0] = P.ux[p]; pu[1] = P.uy[p]; pu[2] = P.uz[p];
-            pw = P.w[p];
-            if (SPIN) { ps[0] = P.sx[p]; ps[1] = P.sy[p]; ps[2] = P.sz[p]; }
-        }
-    };
-    fetch(tid);
     int it = 0;
-    for (int t0 = 0; t0 < len; t0 += COH_NT, it++) {
-        // phase 1: gather, Boris, move, write-back; stage the deposit
-        const int t = t0 + tid;
-        int bo = -1;
-        float f0[3] = {0.0f, 0.0f, 0.0f}, f1[3] = {0.0f, 0.0f, 0.0f}, qw = 0.0f;
-        if (t < len) {
-            const int64_t p = beg + t;
-            const int cell = pc;
-            if (cell >= 0) {
-                int cx, cy, cz;
-                K.cs.split(cell, cx, cy, cz);
-                if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
-                    (unsigned)(cz - T0z) >= (unsigned)TZ) {
-                    strays |= 1u << it;  // moved out of the tile since the last sort
+    for (int t0 = 0; t0 < len; t0 += blockDim.x, it++) {
+        const int t = t0 + threadIdx.x;
+        if (t >= len) continue;
+        const int64_t p = beg + t;
+        const int cell = P.cell[p];
+        int rc = -1;
+        float f0[3] = {0.0f, 0.0f, 0.0f}, f1[3] = {0.0f, 0.0f, 0.0f};
+        if (cell >= 0) {
+            int cx, cy, cz;
+            K.cs.split(cell, cx, cy, cz);
+            if ((unsigned)(cx - T0x) >= (unsigned)TX || (unsigned)(cy - T0y) >= (unsigned)TY ||
+                (unsigned)(cz - T0z) >= (unsigned)TZ) {
+                strays |= 1u << it;  // moved out of the tile since the last sort
+            } else {
+                float f[3] = {P.fx[p], P.fy[p], P.fz[p]};
+                float u[3] = {P.ux[p], P.uy[p], P.uz[p]};
+                float sp[3] = {0.0f, 0.0f, 0.0f};
+                if (SPIN) { sp[0] = P.sx[p]; sp[1] = P.sy[p]; sp[2] = P.sz[p]; }
+                auto fr = [&](int c, int k, int j, int i) -> float {
+                    const float* fb = fld + B::coff(c) + (cz + m - 1 - foz) * B::planep(c) +
+                                      (cy + m - 1 - foy) * B::rowp(c) + (cx + m - 1 - fox);
+                    return fb[(k - (cz + m - 1)) * B::planep(c) + (j - (cy + m - 1)) * B::rowp(c) +
+                              (i - (cx + m - 1))];
+                };
+                float E[3], Bf[3], W0[3][M + 1];
+                gather32<M>(cx + m - 1, cy + m - 1, cz + m - 1, f[0], f[1], f[2], E, Bf, fr, W0[0], W0[1], W0[2]);
+                boris32(u, E, Bf, K.qm);
+                if constexpr (SPIN) {
+                    const float gam = sqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+                    if (!spin_precess<float>(E, Bf, K.qm, u[0], u[1], u[2], gam, K.spin, sp[0], sp[1], sp[2]))
+                        spin_bad++;
+                }
+                const float ig = rsqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+                const float d[3] = {u[0] * ig * K.cfx, u[1] * ig * K.cfy, u[2] * ig * K.cfz};
+                // a non-finite momentum would put the deposit box outside the arrays
+                if (!(fabsf(d[0]) < 1.0f && fabsf(d[1]) < 1.0f && fabsf(d[2]) < 1.0f)) {
+                    oob++;
                 } else {
-                    float f[3] = {pf[0], pf[1], pf[2]};
-                    float u[3] = {pu[0], pu[1], pu[2]};
-                    const float w = pw;
-                    float sp[3] = {ps[0], ps[1], ps[2]};
-                    auto fr = [&](int c, int k, int j, int i) -> float {
-                        const float* fb = fld + B::coff(c) + (cz + m - 1 - foz) * B::planep(c) +
-                                          (cy + m - 1 - foy) * B::rowp(c) + (cx + m - 1 - fox);
-                        return fb[(k - (cz + m - 1)) * B::planep(c) + (j - (cy + m - 1)) * B::rowp(c) +
-                                  (i - (cx + m - 1))];
-                    };
-                    const int Cx = cx + m - 1, Cy = cy + m - 1, Cz = cz + m - 1;
-                    float E[3], Bf[3], W0[3][M + 1];
-                    gather32<M>(Cx, Cy, Cz, f[0], f[1], f[2], E, Bf, fr, W0[0], W0[1], W0[2]);
-                    boris32(u, E, Bf, K.qm);
-                    if constexpr (SPIN) {
-                        const float gam = sqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-                        if (!spin_precess<float>(E, Bf, K.qm, u[0], u[1], u[2], gam, K.spin, sp[0], sp[1], sp[2]))
-                            spin_bad++;
-                    }
-                    const float ig = rsqrtf(1.0f + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-                    const float d[3] = {u[0] * ig * K.cfx, u[1] * ig * K.cfy, u[2] * ig * K.cfz};
-                    // a non-finite momentum would put the deposit box outside the arrays
-                    if (!(fabsf(d[0]) < 1.0f && fabsf(d[1]) < 1.0f && fabsf(d[2]) < 1.0f)) {
-                        oob++;
-                    } else {
 #pragma unroll
-                        for (int a = 0; a < 3; a++) { f0[a] = f[a]; f1[a] = f[a] + d[a]; }
-                        qw = K.q * w;
-                        bo = ((cz + m - 1 - aoz) * B::AY + (cy + m - 1 - aoy)) * B::AX + (cx + m - 1 - aox);
-                        const int nc = move32(cx, cy, cz, f, d, K, cnt12);
-                        P.cell[p] = nc;
-                        P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
-                        P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
-                        if (SPIN) { P.sx[p] = sp[0]; P.sy[p] = sp[1]; P.sz[p] = sp[2]; }
-                    }
+                    for (int a = 0; a < 3; a++) { f0[a] = f[a]; f1[a] = f[a] + d[a]; }
+                    rc = cell;
+                    const int nc = move32(cx, cy, cz, f, d, K, cnt12);
+                    P.cell[p] = nc;
+                    P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
+                    P.ux[p] = u[0]; P.uy[p] = u[1]; P.uz[p] = u[2];
+                    if (SPIN) { P.sx[p] = sp[0]; P.sy[p] = sp[1]; P.sz[p] = sp[2]; }
                 }
             }
         }
-        stg[0 * COH_NT + sw] = f0[0]; stg[1 * COH_NT + sw] = f0[1]; stg[2 * COH_NT + sw] = f0[2];
-        stg[3 * COH_NT + sw] = f1[0]; stg[4 * COH_NT + sw] = f1[1]; stg[5 * COH_NT + sw] = f1[2];
-        stg[6 * COH_NT + sw] = qw;
-        stg[7 * COH_NT + sw] = __int_as_float(bo);
-        __syncthreads();
-        fetch(t0 + COH_NT + tid);
-        // phase 2: deposit a staged particle, lanes spread over cells and
-        // then dealt by the LDS bank of their deposit base (bank_spread_lane)
-        int bd = __float_as_int(stg[7 * COH_NT + sr]);
-        float a0[3] = {stg[0 * COH_NT + sr], stg[1 * COH_NT + sr], stg[2 * COH_NT + sr]};
-        float a1[3] = {stg[3 * COH_NT + sr], stg[4 * COH_NT + sr], stg[5 * COH_NT + sr]};
-        float q = stg[6 * COH_NT + sr];
-        {
-            // bank of the deposit base: the cell's box offset moved to the
-            // nearest node of the old position
-            const int r = bd < 0 ? 32
-                                 : (bd + (a0[0] >= 0.5f) + (a0[1] >= 0.5f) * B::AX + (a0[2] >= 0.5f) * (B::AX * B::AY)) & 31;
-            const int to = bank_spread_lane(r, rg_hist[wv]);
-            bd = lane_send(to, bd);
-#pragma unroll
-            for (int a = 0; a < 3; a++) { a0[a] = lane_send(to, a0[a]); a1[a] = lane_send(to, a1[a]); }
-            q = lane_send(to, q);
-        }
-        if (bd >= 0) {
-            // the old position's primal weights, as the gather evaluated them
-            float S0[3][M + 1];
-#pragma unroll
-            for (int a = 0; a < 3; a++) shape32<M>(a0[a] - (float)(a0[a] >= 0.5f ? 1 : 0), S0[a]);
-            LdsSink32<B::AX, B::AY, B::AVOL, int> sink{acc, K.inv_ufix, bd, 0};
-            esirkepov32<M>(q, a0, a1, S0[0], S0[1], S0[2], K, sink);
-        }
-        __syncthreads();
+        R.cell[p] = rc;
+        R.f0x[p] = f0[0]; R.f0y[p] = f0[1]; R.f0z[p] = f0[2];
+        R.f1x[p] = f1[0]; R.f1y[p] = f1[1]; R.f1z[p] = f1[2];
     }
     if (strays) {  // HBM path for the particles outside the tile
         float* jdst[4] = {F.jx, F.jy, F.jz, F.rho};
@@ -806,6 +743,74 @@ __global__ __launch_bounds__(COH_NT, 3) void k_push32_coh(Soa32 P, Fields32 F, G
         }
     }
     __syncthreads();
+    if (threadIdx.x < 12 && cnt12[threadIdx.x])
+        atomicAdd((unsigned long long*)&cnt[CNT_PEND + threadIdx.x], (unsigned long long)cnt12[threadIdx.x]);
+    if (oob) atomicAdd((unsigned long long*)&cnt[CNT_OOB], (unsigned long long)oob);
+    if (SPIN && spin_bad) atomicAdd((unsigned long long*)&cnt[CNT_SPIN], (unsigned long long)spin_bad);
+}
+
+template <int M, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_push32_dep(
+    Soa32 P, Fields32 F, Grid g, Consts32 K, Tiling tl, const int64_t* __restrict__ tile_off,
+    const int64_t* __restrict__ work, const int64_t* __restrict__ nwork, int chunk, Rec32 R) {
+    using B = Box32<M, TX, TY, TZ, int>;
+    extern __shared__ double lds_d[];
+    int* acc = reinterpret_cast<int*>(lds_d);  // 4 * AVOL accumulators
+    __shared__ unsigned rg_hist[256 / 64][8];   // per wave: bank-spreading counters
+    const int64_t wi = blockIdx.x;
+    if (wi >= *nwork) return;
+    const int64_t item = work[wi];
+    const int tile = (int)(item >> 20);
+    const int64_t cidx = item & ((1 << 20) - 1);
+    const int64_t beg = tile_off[tile] + cidx * chunk;
+    const int64_t end = min(tile_off[tile + 1], beg + chunk);
+    const int txi = tile % tl.nt[0], tyi = (tile / tl.nt[0]) % tl.nt[1], tzi = tile / (tl.nt[0] * tl.nt[1]);
+    const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
+    const int m = g.m;
+    for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) acc[e] = 0;
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    if (lane < 8) rg_hist[wv][lane] = 0u;
+    __syncthreads();
+    const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
+    const int len = (int)(end - beg);
+    const int pick = spread_pick(threadIdx.x);
+    for (int t0 = 0; t0 < len; t0 += blockDim.x) {
+        const int t = t0 + pick;
+        int bd = -1;
+        float a0[3] = {0.0f, 0.0f, 0.0f}, a1[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f;
+        if (t < len) {
+            const int64_t p = beg + t;
+            const int cell = R.cell[p];
+            if (cell >= 0) {
+                int cx, cy, cz;
+                K.cs.split(cell, cx, cy, cz);
+                bd = ((cz + m - 1 - aoz) * B::AY + (cy + m - 1 - aoy)) * B::AX + (cx + m - 1 - aox);
+                a0[0] = R.f0x[p]; a0[1] = R.f0y[p]; a0[2] = R.f0z[p];
+                a1[0] = R.f1x[p]; a1[1] = R.f1y[p]; a1[2] = R.f1z[p];
+                q = K.q * P.w[p];
+            }
+        }
+        {
+            // bank of the deposit base: the cell's box offset moved to the
+            // nearest node of the old position
+            const int r = bd < 0 ? 32
+                                 : (bd + (a0[0] >= 0.5f) + (a0[1] >= 0.5f) * B::AX + (a0[2] >= 0.5f) * (B::AX * B::AY)) & 31;
+            const int to = bank_spread_lane(r, rg_hist[wv]);
+            bd = lane_send(to, bd);
+#pragma unroll
+            for (int a = 0; a < 3; a++) { a0[a] = lane_send(to, a0[a]); a1[a] = lane_send(to, a1[a]); }
+            q = lane_send(to, q);
+        }
+        if (bd >= 0) {
+            // the old position's primal weights, as the gather evaluated them
+            float S0[3][M + 1];
+#pragma unroll
+            for (int a = 0; a < 3; a++) shape32<M>(a0[a] - (float)(a0[a] >= 0.5f ? 1 : 0), S0[a]);
+            LdsSink32<B::AX, B::AY, B::AVOL, int> sink{acc, K.inv_ufix, bd, 0};
+            esirkepov32<M>(q, a0, a1, S0[0], S0[1], S0[2], K, sink);
+        }
+    }
+    __syncthreads();
     float* jdst[4] = {F.jx, F.jy, F.jz, F.rho};
     for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) {
         const int a = acc[e];
@@ -816,10 +821,6 @@ __global__ __launch_bounds__(COH_NT, 3) void k_push32_coh(Soa32 P, Fields32 F, G
         if (gx > g.je[0] - 2 || gy > g.je[1] - 2 || gz > g.je[2] - 2) continue;
         hbm_add(&jdst[c][g.ji(gz, gy, gx)], v);
     }
-    if (threadIdx.x < 12 && cnt12[threadIdx.x])
-        atomicAdd((unsigned long long*)&cnt[CNT_PEND + threadIdx.x], (unsigned long long)cnt12[threadIdx.x]);
-    if (oob) atomicAdd((unsigned long long*)&cnt[CNT_OOB], (unsigned long long)oob);
-    if (SPIN && spin_bad) atomicAdd((unsigned long long*)&cnt[CNT_SPIN], (unsigned long long)spin_bad);
 }
 
 // direct push: any particle order, gather from HBM, global atomics
@@ -1005,7 +1006,7 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
 
 int sort32(pic_ctx* c, int s, int32_t* perm) {
     Species& S = c->sp[s];
-    // Cell order for the cell-coherent push: every cell_order-th sort bins
+    // Cell order for the two-kernel push: every cell_order-th sort bins
     // by cell; the tile sorts in between keep each tile's relative order
     // (only particles that changed cell inside their tile, or arrived from
     // another tile, then sit out of cell order).  Differentiated runs
@@ -1197,15 +1198,25 @@ static int launch_tiled32(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
 }
 
 template <int M, bool SPIN>
-static int launch_coh32_s(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
+static int launch_split32_s(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
     constexpr int TX = TileShape<2, M>::X, TY = TileShape<2, M>::Y, TZ = TileShape<2, M>::Z;
     using B = Box32<M, TX, TY, TZ, int>;
-    constexpr size_t lds = B::LDS + (size_t)COH_WORDS * COH_NT * sizeof(float);
+    constexpr size_t lds_g = (size_t)B::FTOT * sizeof(float), lds_d = (size_t)4 * B::AVOL * sizeof(int);
+    if (!S.rec32 && hipMalloc(&S.rec32, (size_t)7 * std::max<int64_t>(S.cap, 1) * 4) != hipSuccess) {
+        S.rec32 = nullptr;
+        return set_err(c, PIC_ERR_NOMEM, "deposit records for %lld particles", (long long)S.cap);
+    }
+    const Rec32 R = rec32(S);
     const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
-    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push32_coh<M, TX, TY, TZ, SPIN>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_push32_coh<M, TX, TY, TZ, SPIN><<<(unsigned)max_items, COH_NT, lds, c->stream>>>(
-        soa32(S, S.cur), Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, S.d_cnt);
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push32_gat<M, TX, TY, TZ, SPIN>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_g));
+    PIC_HIP(c, hipFuncSetAttribute((const void*)k_push32_dep<M, TX, TY, TZ>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_d));
+    k_push32_gat<M, TX, TY, TZ, SPIN><<<(unsigned)max_items, 256, lds_g, c->stream>>>(
+        soa32(S, S.cur), Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, R, S.d_cnt);
+    PIC_LAUNCH_CHECK(c);
+    k_push32_dep<M, TX, TY, TZ><<<(unsigned)max_items, 256, lds_d, c->stream>>>(
+        soa32(S, S.cur), Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, R);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
@@ -1243,7 +1254,7 @@ int launch_push32(pic_ctx* c, int s) {
     S.pushes_since_sort++;
     PhaseTimer t(c, PIC_T_PUSH);
     if (c->fix32 && c->g.m == 4 && S.cellsorted)
-        return S.spin ? launch_coh32_s<4, true>(c, S, Fd, K) : launch_coh32_s<4, false>(c, S, Fd, K);
+        return S.spin ? launch_split32_s<4, true>(c, S, Fd, K) : launch_split32_s<4, false>(c, S, Fd, K);
     if (c->fix32) {
         if (c->g.m == 2) return launch_tiled32<2, 2, int>(c, S, Fd, K);
         return launch_tiled32<4, 2, int>(c, S, Fd, K);

@@ -41,7 +41,7 @@ void set_tiling(pic_ctx* c) {
                  t[2] = q ? TileShape<2, 2>::Z : TileShape<2, 4>::Z; break;
     }
     Tiling& tl = c->tl;
-    // cell-sorted tiles for the push kernels that visit them cell-coherently
+    // cell-sorted tiles for the two-kernel pushes
     // (FP64; FP32 quartic with fixed-point accumulators)
     tl.fine = c->cell_order > 0 && (!c->fp32 || (c->fix32 && !q));
     tl.ntiles = 1;
@@ -149,6 +149,10 @@ void free_species(pic_ctx* c, Species& S) {
     dfree(S.d_cnt);
     dfree(S.tile_off);
     dfree(S.dxo); dfree(S.dyo); dfree(S.dzo);
+    {
+        char* q = (char*)S.rec32;
+        dfree(q);
+    }
     S = Species();
     (void)c;
 }

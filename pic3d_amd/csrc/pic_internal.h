@@ -41,7 +41,7 @@ struct Tiling {
     int t[3];     // tile edge in cells
     int nt[3];    // tiles per axis
     int ntiles;
-    int fine;     // sorts of this context may bin by cell (cell-coherent push)
+    int fine;     // sorts of this context may bin by cell (two-kernel push)
     // sort bins: one per tile (a keep-dead sort adds one for removed slots)
     __host__ __device__ int64_t nbins() const { return (int64_t)ntiles; }
     __host__ __device__ int cells() const { return t[0] * t[1] * t[2]; }
@@ -68,6 +68,9 @@ struct Species {
     // FP64 two-kernel push: the wrapped old position of each slot the
     // deposit kernel deposits (NaN: nothing to deposit), by slot
     double* dxo = nullptr; double* dyo = nullptr; double* dzo = nullptr;
+    // FP32 two-kernel push: 7 words per slot (old cell, old offsets,
+    // unwrapped new offsets; old cell -1: nothing to deposit), by slot
+    void* rec32 = nullptr;
     int pushes_since_sort = 0;
     int cur = 0;             // which buffer is current
     // device counters: [0] slots in use, [1] live particles,
