@@ -64,15 +64,15 @@ def cell_bytes(precision: str) -> float:
 
 
 def load_traffic(name: str):
-    """HBM bytes per push-kernel launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
-    None when that profile is absent."""
+    """HBM bytes per push call (all of its kernels) from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel factor), or None when that profile is absent."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
+            return float(line.split(":")[1].split()[0])
     return None
 
 
