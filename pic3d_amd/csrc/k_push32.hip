@@ -151,10 +151,15 @@ __device__ __forceinline__ void boris32(float* u, const float* E, const float* B
 // axis, else L-2) carries sum(S1) - sum(S0), zero in exact arithmetic, and
 // is not deposited; the extra planes along the other axes carry exact zeros.
 // S0x, S0y, S0z: the weights at f0 (as the gather evaluates them)
-template <int M, typename D>
+// STILL: the caller knows that the particle keeps its nearest node along
+// every axis (floor(f1 + 0.5) == (f0 >= 0.5)); the box's last line along
+// each axis then carries only zeros and is left out at compile time (the
+// same contributions as the general form).
+template <int M, typename D, bool STILL = false>
 __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const float* f1, const float* S0x,
                                             const float* S0y, const float* S0z, const Consts32& K, D& dst) {
     constexpr int H = M / 2, L = M + 2;
+    constexpr int LR = STILL ? L - 1 : L;  // box lines that can carry weight
     constexpr int R = (M == 2) ? L : 1;  // unroll factor of the outer loops
     const float* S0[3] = {S0x, S0y, S0z};
     float S1[3][M + 1];
@@ -163,10 +168,10 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const int i1 = f0[a] >= 0.5f ? 1 : 0;
-        const int i2 = (int)floorf(f1[a] + 0.5f);  // -1 .. 2
+        const int i2 = STILL ? i1 : (int)floorf(f1[a] + 0.5f);  // -1 .. 2
         shape32<M>(f1[a] - (float)i2, S1[a]);
         b[a] = min(i1, i2) - H;
-        moved[a] = i1 != i2;
+        moved[a] = STILL ? false : i1 != i2;
         d0[a] = i1 - H - b[a];
         d1[a] = i2 - H - b[a];
     }
@@ -217,10 +222,10 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
     for (int t = 0; t < L; t++) { za[t] = Z0[t]; zb[t] = DZ[t]; }
     // Jx: cumulative along x, weights W_yz
 #pragma unroll(R)
-    for (int k = 0; k < L; k++) {
+    for (int k = 0; k < LR; k++) {
         const float sz0 = za[0], dsz = zb[0];
 #pragma unroll
-        for (int j = 0; j < L; j++) {
+        for (int j = 0; j < LR; j++) {
             const float wyz = __builtin_fmaf(dsz, QY[j], sz0 * PY[j]);
             if (wyz == 0.0f) continue;  // a plane outside both supports: only zeros to add
 #pragma unroll
@@ -233,10 +238,10 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
 #pragma unroll
     for (int t = 0; t < L; t++) { za[t] = Z0[t]; zb[t] = DZ[t]; }
 #pragma unroll(R)
-    for (int k = 0; k < L; k++) {
+    for (int k = 0; k < LR; k++) {
         const float sz0 = za[0], dsz = zb[0];
 #pragma unroll
-        for (int i = 0; i < L; i++) {
+        for (int i = 0; i < LR; i++) {
             const float wxz = __builtin_fmaf(dsz, QX[i], sz0 * PX[i]);
             if (wxz == 0.0f) continue;
 #pragma unroll
@@ -250,10 +255,10 @@ __device__ __forceinline__ void esirkepov32(float qw, const float* f0, const flo
 #pragma unroll
     for (int t = 0; t < L; t++) { ya[t] = Y0[t]; yb[t] = DY[t]; }
 #pragma unroll(R)
-    for (int j = 0; j < L; j++) {
+    for (int j = 0; j < LR; j++) {
         const float sy0 = ya[0], dsy = yb[0];
 #pragma unroll
-        for (int i = 0; i < L; i++) {
+        for (int i = 0; i < LR; i++) {
             const float wxy = __builtin_fmaf(dsy, QX[i], sy0 * PX[i]);
             if (wxy == 0.0f) continue;
 #pragma unroll
@@ -607,11 +612,17 @@ __global__ __launch_bounds__((Push32Threads<M, ACC, SPIN>::N), 2) void k_push32_
 //   chunk's particles in storage order, the field tile alone in LDS; the
 //   lanes of a wave hold the particles of a few neighbouring cells (the
 //   field reads mostly broadcast).  Per slot it writes a deposit record:
-//   old cell (-1: nothing to deposit), old offsets, unwrapped new offsets.
+//   the old cell c (-1: nothing to deposit; -2 - c: the particle changed
+//   its nearest node along some axis), old offsets, unwrapped new offsets.
 //   A particle outside its tile takes the HBM path, deposit included.
 //  k_push32_dep: the deposit of those records into the accumulator box
-//   alone in LDS; per pass of 256 slots thread t takes slot spread_pick(t),
-//   then the particles are dealt by the bank of their deposit base.
+//   alone in LDS.  The chunk's particles are listed in two classes, those
+//   that kept their nearest node along every axis (a box one line shorter
+//   per axis, fixed at compile time) and the rest; each list is deposited
+//   in passes of 256 where thread t takes entry spread_pick(t), then the
+//   particles are dealt by the bank of their deposit base.  A wave runs the
+//   union of its lanes' box lines, so a class of its own for the still
+//   particles saves the lines only moving neighbours need.
 // Each kernel holds half of the LDS and fewer registers than the one-kernel
 // forms, so more waves share a CU.
 // ---------------------------------------------------------------------
@@ -701,9 +712,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 if (!(fabsf(d[0]) < 1.0f && fabsf(d[1]) < 1.0f && fabsf(d[2]) < 1.0f)) {
                     oob++;
                 } else {
+                    bool moved = false;
 #pragma unroll
-                    for (int a = 0; a < 3; a++) { f0[a] = f[a]; f1[a] = f[a] + d[a]; }
-                    rc = cell;
+                    for (int a = 0; a < 3; a++) {
+                        f0[a] = f[a];
+                        f1[a] = f[a] + d[a];
+                        moved |= (f0[a] >= 0.5f ? 1 : 0) != (int)floorf(f1[a] + 0.5f);
+                    }
+                    rc = moved ? -2 - cell : cell;
                     const int nc = move32(cx, cy, cz, f, d, K, cnt12);
                     P.cell[p] = nc;
                     P.fx[p] = f[0]; P.fy[p] = f[1]; P.fz[p] = f[2];
@@ -754,9 +770,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     Soa32 P, Fields32 F, Grid g, Consts32 K, Tiling tl, const int64_t* __restrict__ tile_off,
     const int64_t* __restrict__ work, const int64_t* __restrict__ nwork, int chunk, Rec32 R) {
     using B = Box32<M, TX, TY, TZ, int>;
+    static_assert(PIC_PUSH_CHUNK <= 65536, "chunk slots are listed as 16-bit offsets");
     extern __shared__ double lds_d[];
-    int* acc = reinterpret_cast<int*>(lds_d);  // 4 * AVOL accumulators
-    __shared__ unsigned rg_hist[256 / 64][8];   // per wave: bank-spreading counters
+    int* acc = reinterpret_cast<int*>(lds_d);                     // 4 * AVOL accumulators
+    uint16_t* lst = reinterpret_cast<uint16_t*>(acc + 4 * B::AVOL);  // chunk slots by class
+    __shared__ unsigned rg_hist[256 / 64][8];  // per wave: bank-spreading counters
+    __shared__ int ncls[2];
     const int64_t wi = blockIdx.x;
     if (wi >= *nwork) return;
     const int64_t item = work[wi];
@@ -770,18 +789,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) acc[e] = 0;
     const int wv = threadIdx.x >> 6, lane = lane_id();
     if (lane < 8) rg_hist[wv][lane] = 0u;
+    if (threadIdx.x < 2) ncls[threadIdx.x] = 0;
+    __syncthreads();
+    const int len = (int)(end - beg);
+    // the two lists: still particles from the front, the others from the back
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int t = threadIdx.x; t - lane < len; t += blockDim.x) {
+        const int c = t < len ? R.cell[beg + t] : -1;
+        const uint64_t m0 = __ballot(c >= 0), m1 = __ballot(c < -1);
+        int o0 = 0, o1 = 0;
+        if (lane == 0) {
+            o0 = m0 ? atomicAdd(&ncls[0], __popcll(m0)) : 0;
+            o1 = m1 ? atomicAdd(&ncls[1], __popcll(m1)) : 0;
+        }
+        o0 = __shfl(o0, 0);
+        o1 = __shfl(o1, 0);
+        if (c >= 0) lst[o0 + __popcll(m0 & below)] = (uint16_t)t;
+        if (c < -1) lst[len - 1 - (o1 + __popcll(m1 & below))] = (uint16_t)t;
+    }
     __syncthreads();
     const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
-    const int len = (int)(end - beg);
     const int pick = spread_pick(threadIdx.x);
-    for (int t0 = 0; t0 < len; t0 += blockDim.x) {
-        const int t = t0 + pick;
-        int bd = -1;
-        float a0[3] = {0.0f, 0.0f, 0.0f}, a1[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f;
-        if (t < len) {
-            const int64_t p = beg + t;
-            const int cell = R.cell[p];
-            if (cell >= 0) {
+    auto run = [&](auto still) {
+        constexpr bool STILL = decltype(still)::value;
+        const int n = ncls[STILL ? 0 : 1];
+        for (int t0 = 0; t0 < n; t0 += blockDim.x) {
+            const int e = t0 + pick;
+            int bd = -1;
+            float a0[3] = {0.0f, 0.0f, 0.0f}, a1[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f;
+            if (e < n) {
+                const int64_t p = beg + lst[STILL ? e : len - 1 - e];
+                const int rc = R.cell[p];
+                const int cell = STILL ? rc : -2 - rc;
                 int cx, cy, cz;
                 K.cs.split(cell, cx, cy, cz);
                 bd = ((cz + m - 1 - aoz) * B::AY + (cy + m - 1 - aoy)) * B::AX + (cx + m - 1 - aox);
@@ -789,27 +828,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                 a1[0] = R.f1x[p]; a1[1] = R.f1y[p]; a1[2] = R.f1z[p];
                 q = K.q * P.w[p];
             }
-        }
-        {
-            // bank of the deposit base: the cell's box offset moved to the
-            // nearest node of the old position
-            const int r = bd < 0 ? 32
-                                 : (bd + (a0[0] >= 0.5f) + (a0[1] >= 0.5f) * B::AX + (a0[2] >= 0.5f) * (B::AX * B::AY)) & 31;
-            const int to = bank_spread_lane(r, rg_hist[wv]);
-            bd = lane_send(to, bd);
+            {
+                // bank of the deposit base: the cell's box offset moved to
+                // the nearest node of the old position
+                const int r = bd < 0 ? 32
+                                     : (bd + (a0[0] >= 0.5f) + (a0[1] >= 0.5f) * B::AX +
+                                        (a0[2] >= 0.5f) * (B::AX * B::AY)) & 31;
+                const int to = bank_spread_lane(r, rg_hist[wv]);
+                bd = lane_send(to, bd);
 #pragma unroll
-            for (int a = 0; a < 3; a++) { a0[a] = lane_send(to, a0[a]); a1[a] = lane_send(to, a1[a]); }
-            q = lane_send(to, q);
-        }
-        if (bd >= 0) {
-            // the old position's primal weights, as the gather evaluated them
-            float S0[3][M + 1];
+                for (int a = 0; a < 3; a++) { a0[a] = lane_send(to, a0[a]); a1[a] = lane_send(to, a1[a]); }
+                q = lane_send(to, q);
+            }
+            if (bd >= 0) {
+                // the old position's primal weights, as the gather evaluated them
+                float S0[3][M + 1];
 #pragma unroll
-            for (int a = 0; a < 3; a++) shape32<M>(a0[a] - (float)(a0[a] >= 0.5f ? 1 : 0), S0[a]);
-            LdsSink32<B::AX, B::AY, B::AVOL, int> sink{acc, K.inv_ufix, bd, 0};
-            esirkepov32<M>(q, a0, a1, S0[0], S0[1], S0[2], K, sink);
+                for (int a = 0; a < 3; a++) shape32<M>(a0[a] - (float)(a0[a] >= 0.5f ? 1 : 0), S0[a]);
+                LdsSink32<B::AX, B::AY, B::AVOL, int> sink{acc, K.inv_ufix, bd, 0};
+                esirkepov32<M, decltype(sink), STILL>(q, a0, a1, S0[0], S0[1], S0[2], K, sink);
+            }
         }
-    }
+    };
+    run(std::true_type{});
+    run(std::false_type{});
     __syncthreads();
     float* jdst[4] = {F.jx, F.jy, F.jz, F.rho};
     for (int e = threadIdx.x; e < 4 * B::AVOL; e += blockDim.x) {
@@ -1201,7 +1243,8 @@ template <int M, bool SPIN>
 static int launch_split32_s(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
     constexpr int TX = TileShape<2, M>::X, TY = TileShape<2, M>::Y, TZ = TileShape<2, M>::Z;
     using B = Box32<M, TX, TY, TZ, int>;
-    constexpr size_t lds_g = (size_t)B::FTOT * sizeof(float), lds_d = (size_t)4 * B::AVOL * sizeof(int);
+    constexpr size_t lds_g = (size_t)B::FTOT * sizeof(float),
+                     lds_d = (size_t)4 * B::AVOL * sizeof(int) + (size_t)PIC_PUSH_CHUNK * sizeof(uint16_t);
     if (!S.rec32 && hipMalloc(&S.rec32, (size_t)7 * std::max<int64_t>(S.cap, 1) * 4) != hipSuccess) {
         S.rec32 = nullptr;
         return set_err(c, PIC_ERR_NOMEM, "deposit records for %lld particles", (long long)S.cap);

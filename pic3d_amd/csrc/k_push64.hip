@@ -14,6 +14,8 @@
 //  * k_push64_tiled: one workgroup per (tile, chunk) of the tile-sorted
 //    particle list; currents are accumulated in LDS (ds_add_f64) over the
 //    tile plus deposit halo and flushed to HBM once per workgroup.
+#include <type_traits>
+
 #include "pic_internal.h"
 #include "atomics.h"
 #include "push64.h"
@@ -87,12 +89,49 @@ __device__ __forceinline__ void gather_boris(double x, double y, double z, doubl
         }
         return e;
     };
-    double e0 = comp(0, Wxd, Wyp, Wzp, id, jp, kp);
-    double e1 = comp(1, Wxp, Wyd, Wzp, ip, jd, kp);
-    double e2 = comp(2, Wxp, Wyp, Wzd, ip, jp, kd);
-    double b0 = comp(3, Wxp, Wyd, Wzd, ip, jd, kd);
-    double b1 = comp(4, Wxd, Wyp, Wzd, id, jp, kd);
-    double b2 = comp(5, Wxd, Wyd, Wzp, id, jd, kp);
+    // two components with the same x and y weights in one k loop: their
+    // WX[i] * WY[j] products are computed once per k (each sum keeps its
+    // own order and terms)
+    auto pair = [&](int c0, int c1, const double* WX, const double* WY, const double* WZ0, const double* WZ1,
+                    int b0, int b1, int z0, int z1, double& e0, double& e1) {
+        double s0 = 0.0, s1 = 0.0;
+        double wz0[M + 1], wz1[M + 1], wx[M + 1];
+        copy_arr<M + 1>(WZ0, wz0);
+        copy_arr<M + 1>(WZ1, wz1);
+        copy_arr<M + 1>(WX, wx);
+#pragma unroll 1
+        for (int k = 0; k <= M; k++) {
+            const double za = wz0[0], zb = wz1[0];
+#pragma unroll
+            for (int t = 0; t <= M; t++) asm volatile("" : "+v"(wx[t]));
+#pragma unroll
+            for (int j = 0; j <= M; j++)
+#pragma unroll
+                for (int i = 0; i <= M; i++) {
+                    const double xy = wx[i] * WY[j];
+                    s0 += xy * za * fr(c0, z0 + k - H, b1 + j - H, b0 + i - H);
+                    s1 += xy * zb * fr(c1, z1 + k - H, b1 + j - H, b0 + i - H);
+                }
+            shift_down<M + 1>(wz0);
+            shift_down<M + 1>(wz1);
+        }
+        e0 = s0;
+        e1 = s1;
+    };
+    double e0, e1, e2, b0, b1, b2;
+    if constexpr (LOWREG) {
+        pair(0, 4, Wxd, Wyp, Wzp, Wzd, id, jp, kp, kd, e0, b1);  // Ex, By
+        pair(1, 3, Wxp, Wyd, Wzp, Wzd, ip, jd, kp, kd, e1, b0);  // Ey, Bx
+        e2 = comp(2, Wxp, Wyp, Wzd, ip, jp, kd);
+        b2 = comp(5, Wxd, Wyd, Wzp, id, jd, kp);
+    } else {
+        e0 = comp(0, Wxd, Wyp, Wzp, id, jp, kp);
+        e1 = comp(1, Wxp, Wyd, Wzp, ip, jd, kp);
+        e2 = comp(2, Wxp, Wyp, Wzd, ip, jp, kd);
+        b0 = comp(3, Wxp, Wyd, Wzd, ip, jd, kd);
+        b1 = comp(4, Wxd, Wyp, Wzd, id, jp, kd);
+        b2 = comp(5, Wxd, Wyd, Wzp, id, jd, kp);
+    }
     // Boris, particle.cpp:271-311
     e0 *= K.qm; e1 *= K.qm; e2 *= K.qm; b0 *= K.qm; b1 *= K.qm; b2 *= K.qm;
     const double px1 = ux + e0, py1 = uy + e1, pz1 = uz + e2;
@@ -168,19 +207,23 @@ __device__ __forceinline__ void place(const double* S, int d, double* A) {
 }
 
 // deposit with the old-position weights (Sx0, Sy0, Sz0 at nodes i1, j1, k1)
-// given
-template <int M, typename D>
+// given.  STILL: the caller knows that the particle keeps its nearest node
+// along every axis (i2 == i1, j2 == j1, k2 == k1); the box's last line along
+// each axis then carries only zeros and is left out at compile time (the
+// same contributions as the general form).
+template <int M, typename D, bool STILL = false>
 __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const double* Sy0,
                                              const double* Sz0, int i1, int j1, int k1, double xn,
                                              double yn, double zn, const PushConsts& K, int m, D& dst) {
     constexpr int H = M / 2, L = M + 2;
+    constexpr int LR = STILL ? L - 1 : L;  // box lines that can carry weight
     const double org = (double)(1 - m);
     double Sx1[M + 1], Sy1[M + 1], Sz1[M + 1];
-    const int i2 = shape_at<M>(xn - org, Sx1);
-    const int j2 = shape_at<M>(yn - org, Sy1);
-    const int k2 = shape_at<M>(zn - org, Sz1);
+    const int i2 = STILL ? (shape_at<M>(xn - org, Sx1), i1) : shape_at<M>(xn - org, Sx1);
+    const int j2 = STILL ? (shape_at<M>(yn - org, Sy1), j1) : shape_at<M>(yn - org, Sy1);
+    const int k2 = STILL ? (shape_at<M>(zn - org, Sz1), k1) : shape_at<M>(zn - org, Sz1);
     const int bx = min(i1, i2) - H, by = min(j1, j2) - H, bz = min(k1, k2) - H;
-    const bool crx = i1 != i2, cry = j1 != j2, crz = k1 != k2;
+    const bool crx = !STILL && i1 != i2, cry = !STILL && j1 != j2, crz = !STILL && k1 != k2;
     double X0[L], X1[L], Y0[L], Y1[L], Z0[L], Z1[L];
     place<M>(Sx0, i1 - H - bx, X0); place<M>(Sx1, i2 - H - bx, X1);
     place<M>(Sy0, j1 - H - by, Y0); place<M>(Sy1, j2 - H - by, Y1);
@@ -221,11 +264,11 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
     double za[L], zb[L];
     copy_arr<L>(Z0, za); copy_arr<L>(DZ, zb);
 #pragma unroll 1
-    for (int k = 0; k < L; k++) {
+    for (int k = 0; k < LR; k++) {
         const double sz0 = za[0], dsz = zb[0];
         shift_down<L>(za); shift_down<L>(zb);
 #pragma unroll
-        for (int j = 0; j < L; j++) {
+        for (int j = 0; j < LR; j++) {
             const double wyz = __builtin_fma(dsz, QY[j], sz0 * PY[j]);
             if (wyz == 0.0) continue;  // a plane outside both supports: only +-0 to add
 #pragma unroll
@@ -236,11 +279,11 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
     // Jy (particle.cpp:452-475)
     copy_arr<L>(Z0, za); copy_arr<L>(DZ, zb);
 #pragma unroll 1
-    for (int k = 0; k < L; k++) {
+    for (int k = 0; k < LR; k++) {
         const double sz0 = za[0], dsz = zb[0];
         shift_down<L>(za); shift_down<L>(zb);
 #pragma unroll
-        for (int i = 0; i < L; i++) {
+        for (int i = 0; i < LR; i++) {
             const double wxz = __builtin_fma(dsz, QX[i], sz0 * PX[i]);
             if (wxz == 0.0) continue;
 #pragma unroll
@@ -252,11 +295,11 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
     double ya[L], yb[L];
     copy_arr<L>(Y0, ya); copy_arr<L>(DY, yb);
 #pragma unroll 1
-    for (int j = 0; j < L; j++) {
+    for (int j = 0; j < LR; j++) {
         const double sy0 = ya[0], dsy = yb[0];
         shift_down<L>(ya); shift_down<L>(yb);
 #pragma unroll
-        for (int i = 0; i < L; i++) {
+        for (int i = 0; i < LR; i++) {
             const double wxy = __builtin_fma(dsy, QX[i], sy0 * PX[i]);
             if (wxy == 0.0) continue;
 #pragma unroll
@@ -285,13 +328,13 @@ __device__ __forceinline__ void esirkepov_s0(double qw, const double* Sx0, const
     }
 }
 
-template <int M, typename D>
+template <int M, typename D, bool STILL = false>
 __device__ __forceinline__ void esirkepov(double qw, double xo, double yo, double zo, double xn,
                                           double yn, double zn, const PushConsts& K, int m, D& dst) {
     const double org = (double)(1 - m);
     double Sx0[M + 1], Sy0[M + 1], Sz0[M + 1];
     const int i1 = shape_at<M>(xo - org, Sx0), j1 = shape_at<M>(yo - org, Sy0), k1 = shape_at<M>(zo - org, Sz0);
-    esirkepov_s0<M>(qw, Sx0, Sy0, Sz0, i1, j1, k1, xn, yn, zn, K, m, dst);
+    esirkepov_s0<M, D, STILL>(qw, Sx0, Sy0, Sz0, i1, j1, k1, xn, yn, zn, K, m, dst);
 }
 
 // deposit sinks ---------------------------------------------------------
@@ -609,12 +652,18 @@ __global__ __launch_bounds__(256, 2) void k_push64_tiled(P64 P, Fields64 F, Grid
 //   field reads fall on the LDS words its cell-mates read (one broadcast
 //   read, no bank conflict).  It writes the particle state back and, for
 //   every particle the deposit kernel is to deposit, its wrapped old
-//   position (dxo, dyo, dzo; NaN: nothing to deposit).  A particle outside
+//   position (dxo, dyo, dzo; NaN: nothing to deposit; dxo negative: the
+//   particle changed its nearest node along some axis).  A particle outside
 //   its tile takes the HBM path of k_push64_tiled, deposit included.
 //  k_push64_deposit: the Esirkepov deposit of those particles into the
-//   accumulator box alone in LDS.  Per pass of 256 slots, thread t takes
-//   slot spread_pick(t), so a wave's lanes sit in different cells, then
-//   the particles are dealt by the bank of their deposit base.
+//   accumulator box alone in LDS.  The chunk's particles are listed in two
+//   classes, those that kept their nearest node along every axis (a box
+//   one line shorter per axis, fixed at compile time) and the rest.  Each
+//   list is deposited in passes of 256 where thread t takes entry
+//   spread_pick(t), so a wave's lanes sit in different cells, then the
+//   particles are dealt by the bank of their deposit base.  A wave runs the
+//   union of its lanes' box lines, so a class of its own for the still
+//   particles saves the lines only moving neighbours need.
 // Each kernel holds half of the one-kernel form's LDS and fewer registers,
 // so more workgroups share a CU.  The particle arithmetic is the same code:
 // the deposit's old-position weights come from the same shape function of
@@ -692,8 +741,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             if (P.xo) { P.xo[p] = x; P.yo[p] = y; P.zo[p] = z; }
             if (SPIN) spin_bad += spin_step(P, p, ebq, ux, uy, uz, gam, K);
             // a non-finite momentum would put the deposit box outside the arrays
-            if (fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0) { ox = x; oy = y; oz = z; }
-            else oob++;
+            if (fabs(xn - x) < 1.0 && fabs(yn - y) < 1.0 && fabs(zn - z) < 1.0) {
+                // sign of dxo: the particle changed its nearest node (x >= 0 here)
+                const bool moved = i1 != (int)(xn - org + 0.5) || j1 != (int)(yn - org + 0.5) ||
+                                   k1 != (int)(zn - org + 0.5);
+                ox = moved ? -fabs(x) : fabs(x); oy = y; oz = z;
+            } else {
+                oob++;
+            }
         }
         dxo[p] = ox; dyo[p] = oy; dzo[p] = oz;
     }
@@ -715,8 +770,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int64_t* __restrict__ work, const int64_t* __restrict__ nwork, int chunk,
     const double* __restrict__ dxo, const double* __restrict__ dyo, const double* __restrict__ dzo) {
     using B = AccBox<M, TX, TY, TZ>;
+    static_assert(PIC_PUSH_CHUNK <= 65536, "chunk slots are listed as 16-bit offsets");
     extern __shared__ double lds64[];
     unsigned long long* acc = reinterpret_cast<unsigned long long*>(lds64);  // AVOL: J, rho
+    uint16_t* lst = reinterpret_cast<uint16_t*>(lds64 + B::AVOL);           // chunk slots by class
+    __shared__ int ncls[2];
     const int64_t wi = blockIdx.x;
     if (wi >= *nwork) return;
     const int64_t item = work[wi];
@@ -730,41 +788,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int T0x = txi * TX, T0y = tyi * TY, T0z = tzi * TZ;
     for (int e = threadIdx.x; e < B::AVOL; e += blockDim.x) acc[e] = 0ull;
     __shared__ unsigned rg_hist[256 / 64][8];  // per wave: bank-spreading counters
-    if (lane_id() < 8) rg_hist[threadIdx.x >> 6][lane_id()] = 0u;
+    const int lane = lane_id();
+    if (lane < 8) rg_hist[threadIdx.x >> 6][lane] = 0u;
+    if (threadIdx.x < 2) ncls[threadIdx.x] = 0;
+    __syncthreads();
+    const int len = (int)(end - beg);
+    // the two lists: still particles from the front, the others from the back
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int t = threadIdx.x; t - lane < len; t += blockDim.x) {
+        const double d = t < len ? dxo[beg + t] : __longlong_as_double(0x7ff8000000000000ll);
+        const bool some = d == d, mv = some && signbit(d);
+        const uint64_t m0 = __ballot(some && !mv), m1 = __ballot(mv);
+        int o0 = 0, o1 = 0;
+        if (lane == 0) {
+            o0 = m0 ? atomicAdd(&ncls[0], __popcll(m0)) : 0;
+            o1 = m1 ? atomicAdd(&ncls[1], __popcll(m1)) : 0;
+        }
+        o0 = __shfl(o0, 0);
+        o1 = __shfl(o1, 0);
+        if (some && !mv) lst[o0 + __popcll(m0 & below)] = (uint16_t)t;
+        if (mv) lst[len - 1 - (o1 + __popcll(m1 & below))] = (uint16_t)t;
+    }
     __syncthreads();
     const int ox = T0x + g.m - 2 - B::H, oy = T0y + g.m - 2 - B::H, oz = T0z + g.m - 2 - B::H;
     LdsSink<B> sink{acc, K.inv_unit, oz, oy, ox, 0};
-    const int len = (int)(end - beg);
     const int wv = threadIdx.x >> 6;
     const int pick = spread_pick(threadIdx.x);
     const double org = (double)(1 - g.m);
-    for (int t0 = 0; t0 < len; t0 += blockDim.x) {
-        const int t = t0 + pick;
-        double x0 = 0.0, y0 = 0.0, z0 = 0.0, xn = 0.0, yn = 0.0, zn = 0.0, qw = 0.0;
-        if (t < len) {
-            const int64_t p = beg + t;
-            x0 = dxo[p];
-            if (x0 == x0) {
-                y0 = dyo[p]; z0 = dzo[p];
+    auto run = [&](auto still) {
+        constexpr bool STILL = decltype(still)::value;
+        const int n = ncls[STILL ? 0 : 1];
+        for (int t0 = 0; t0 < n; t0 += blockDim.x) {
+            const int e = t0 + pick;
+            double x0 = 0.0, y0 = 0.0, z0 = 0.0, xn = 0.0, yn = 0.0, zn = 0.0, qw = 0.0;
+            if (e < n) {
+                const int64_t p = beg + lst[STILL ? e : len - 1 - e];
+                x0 = fabs(dxo[p]); y0 = dyo[p]; z0 = dzo[p];
                 xn = P.x[p]; yn = P.y[p]; zn = P.z[p];
                 qw = K.q * P.w[p];
             }
+            // deal by the bank of the deposit box base, the nearest node of
+            // the old position (q w == 0: nothing to add)
+            int r = 32;
+            if (qw != 0.0) {
+                const int ib = (int)(x0 - org + 0.5) - B::H, jb = (int)(y0 - org + 0.5) - B::H,
+                          kb = (int)(z0 - org + 0.5) - B::H;
+                r = (2 * ((kb - oz) * B::PXY + (jb - oy) * B::PX + (ib - ox))) & 31;
+            }
+            const int to = bank_spread_lane(r, rg_hist[wv]);
+            x0 = lane_send_d(to, x0); y0 = lane_send_d(to, y0); z0 = lane_send_d(to, z0);
+            xn = lane_send_d(to, xn); yn = lane_send_d(to, yn); zn = lane_send_d(to, zn);
+            qw = lane_send_d(to, qw);
+            // an in-tile particle that moved < 1 cell: its whole box lies in the LDS box
+            if (qw != 0.0) esirkepov<M, LdsSink<B>, STILL>(qw, x0, y0, z0, xn, yn, zn, K, g.m, sink);
         }
-        // deal by the bank of the deposit box base, the nearest node of the
-        // old position (q w == 0: nothing to add)
-        int r = 32;
-        if (qw != 0.0) {
-            const int ib = (int)(x0 - org + 0.5) - B::H, jb = (int)(y0 - org + 0.5) - B::H,
-                      kb = (int)(z0 - org + 0.5) - B::H;
-            r = (2 * ((kb - oz) * B::PXY + (jb - oy) * B::PX + (ib - ox))) & 31;
-        }
-        const int to = bank_spread_lane(r, rg_hist[wv]);
-        x0 = lane_send_d(to, x0); y0 = lane_send_d(to, y0); z0 = lane_send_d(to, z0);
-        xn = lane_send_d(to, xn); yn = lane_send_d(to, yn); zn = lane_send_d(to, zn);
-        qw = lane_send_d(to, qw);
-        // an in-tile particle that moved < 1 cell: its whole box lies in the LDS box
-        if (qw != 0.0) esirkepov<M>(qw, x0, y0, z0, xn, yn, zn, K, g.m, sink);
-    }
+    };
+    run(std::true_type{});
+    run(std::false_type{});
     __syncthreads();
     // flush: one global atomic per nonzero accumulator point
     double* jp[4] = {F.jx, F.jy, F.jz, F.rho};
@@ -831,7 +911,8 @@ static int launch_split64(pic_ctx* c, Species& S, const P64& P, const Fields64& 
                           int64_t max_items) {
     using TS = TileShape<0, M>;
     using B = AccBox<M, TS::X, TS::Y, TS::Z>;
-    constexpr size_t lds_g = (size_t)B::FVOL * sizeof(double), lds_d = (size_t)B::AVOL * sizeof(double);
+    constexpr size_t lds_g = (size_t)B::FVOL * sizeof(double),
+                     lds_d = (size_t)B::AVOL * sizeof(double) + (size_t)PIC_PUSH_CHUNK * sizeof(uint16_t);
     PIC_HIP(c, hipFuncSetAttribute((const void*)k_push64_gather<M, TS::X, TS::Y, TS::Z, SPIN>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_g));
     PIC_HIP(c, hipFuncSetAttribute((const void*)k_push64_deposit<M, TS::X, TS::Y, TS::Z>,
