@@ -809,15 +809,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     }
     __syncthreads();
     const int aox = T0x + m - 2 - B::H, aoy = T0y + m - 2 - B::H, aoz = T0z + m - 2 - B::H;
-    const int pick = spread_pick(threadIdx.x);
     auto run = [&](auto still) {
         constexpr bool STILL = decltype(still)::value;
         const int n = ncls[STILL ? 0 : 1];
         for (int t0 = 0; t0 < n; t0 += blockDim.x) {
-            const int e = t0 + pick;
+            const int pk = spread_pick_rem(threadIdx.x, n - t0), e = t0 + pk;
             int bd = -1;
             float a0[3] = {0.0f, 0.0f, 0.0f}, a1[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f;
-            if (e < n) {
+            if (pk >= 0) {
                 const int64_t p = beg + lst[STILL ? e : len - 1 - e];
                 const int rc = R.cell[p];
                 const int cell = STILL ? rc : -2 - rc;

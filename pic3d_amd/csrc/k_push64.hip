@@ -813,15 +813,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const int ox = T0x + g.m - 2 - B::H, oy = T0y + g.m - 2 - B::H, oz = T0z + g.m - 2 - B::H;
     LdsSink<B> sink{acc, K.inv_unit, oz, oy, ox, 0};
     const int wv = threadIdx.x >> 6;
-    const int pick = spread_pick(threadIdx.x);
     const double org = (double)(1 - g.m);
     auto run = [&](auto still) {
         constexpr bool STILL = decltype(still)::value;
         const int n = ncls[STILL ? 0 : 1];
         for (int t0 = 0; t0 < n; t0 += blockDim.x) {
-            const int e = t0 + pick;
+            const int pk = spread_pick_rem(threadIdx.x, n - t0), e = t0 + pk;
             double x0 = 0.0, y0 = 0.0, z0 = 0.0, xn = 0.0, yn = 0.0, zn = 0.0, qw = 0.0;
-            if (e < n) {
+            if (pk >= 0) {
                 const int64_t p = beg + lst[STILL ? e : len - 1 - e];
                 x0 = fabs(dxo[p]); y0 = dyo[p]; z0 = dzo[p];
                 xn = P.x[p]; yn = P.y[p]; zn = P.z[p];

@@ -81,6 +81,17 @@ __device__ __forceinline__ int spread_pick(int t) {
     return w + 4 * (l >> 4) + 16 * (l & 15);
 }
 
+// The same over a pass of rem < 256 remaining slots: only the first
+// W = ceil(rem / 64) waves take slots (w + W (l / 16) + 4 W (l % 16)), so
+// a short pass leaves whole waves idle instead of thinning every wave.
+// Returns -1 for a thread without a slot.
+__device__ __forceinline__ int spread_pick_rem(int t, int rem) {
+    if (rem >= 256) return spread_pick(t);
+    const int w = t >> 6, l = t & 63, W = (rem + 63) >> 6;
+    const int e = w + W * (l >> 4) + 4 * W * (l & 15);
+    return w < W && e < rem ? e : -1;
+}
+
 // value v of this lane sent to lane 'to' (a permutation of the wave)
 __device__ __forceinline__ int lane_send(int to, int v) { return __builtin_amdgcn_ds_permute(to << 2, v); }
 __device__ __forceinline__ float lane_send(int to, float v) {
