@@ -513,7 +513,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
-                          "design-only (see DESIGN.md)", "n_gpus": args.gpus}))
+                          "design-only", "n_gpus_supported": 1, "n_gpus": args.gpus}))
         sys.exit(2)
     precs = ["fp64", "fp32"] if args.precision == "both" else [args.precision]
     res = {pr: run_gpu(args, pr) for pr in precs}

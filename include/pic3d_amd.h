@@ -209,9 +209,10 @@ enum pic_option {
                                   order; default 8), and the tiled push gathers
                                   with the lanes of a wave in neighbouring cells
                                   and deposits with its lanes spread over cells
-                                  (FP64: two kernels, the deposit reading the
-                                  old positions the gather kernel wrote; FP32:
-                                  one kernel in two phases over an LDS stage);
+                                  (two kernels: gather, Boris and move, then the
+                                  deposit from the per-slot records the first
+                                  wrote: FP64 old positions, FP32 old cell and
+                                  offsets);
                                   0: tile bins (chunks stored transposed) and
                                   one pass with scattered lanes.  Results are
                                   the same up to the order of the fixed-point
