@@ -812,8 +812,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     auto run = [&](auto still) {
         constexpr bool STILL = decltype(still)::value;
         const int n = ncls[STILL ? 0 : 1];
-        for (int t0 = 0; t0 < n; t0 += blockDim.x) {
-            const int pk = spread_pick_rem(threadIdx.x, n - t0), e = t0 + pk;
+        for (int pass = 0; 4 * pass < (n + 63) >> 6; pass++) {
+            const int e = spread_row(threadIdx.x, pass, n), pk = e;
+            if ((threadIdx.x >> 6) + 4 * pass >= (n + 63) >> 6) break;  // this wave's rows are done
             int bd = -1;
             float a0[3] = {0.0f, 0.0f, 0.0f}, a1[3] = {0.0f, 0.0f, 0.0f}, q = 0.0f;
             if (pk >= 0) {

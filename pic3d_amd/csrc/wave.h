@@ -92,6 +92,18 @@ __device__ __forceinline__ int spread_pick_rem(int t, int rem) {
     return w < W && e < rem ? e : -1;
 }
 
+// Lanes of a 256-thread workgroup spread over a whole list of n
+// cell-ordered entries: the list is cut into S = ceil(n / 64) rows, lane l
+// of row r taking entry l S + r, and wave w takes row 4 p + w in pass p
+// (ceil(S / 4) passes).  A wave's 64 lanes then hold entries S apart, in
+// as many different cells as the list spans (64 at 16 particles per cell
+// and 1024 entries), where a pass of 256 consecutive entries holds 16 cells
+// with four lanes each.  Returns -1 for a lane without an entry.
+__device__ __forceinline__ int spread_row(int t, int pass, int n) {
+    const int S = (n + 63) >> 6, r = 4 * pass + (t >> 6), e = (t & 63) * S + r;
+    return r < S && e < n ? e : -1;
+}
+
 // value v of this lane sent to lane 'to' (a permutation of the wave)
 __device__ __forceinline__ int lane_send(int to, int v) { return __builtin_amdgcn_ds_permute(to << 2, v); }
 __device__ __forceinline__ float lane_send(int to, float v) {
