@@ -206,7 +206,7 @@ enum pic_option {
                                   accumulators: N >= 1 keeps each tile's
                                   particles in cell order (every N-th sort bins
                                   by cell, the tile sorts in between keep the
-                                  order; default 8), and the tiled push gathers
+                                  order; default 4), and the tiled push gathers
                                   with the lanes of a wave in neighbouring cells
                                   and deposits with its lanes spread over cells
                                   (two kernels: gather, Boris and move, then the

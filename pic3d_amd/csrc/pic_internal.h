@@ -139,7 +139,7 @@ struct pic_ctx {
     int keep_old = 0;
     int xmax_face = 0;            // PIC_OPT_XMAX_BOUNDARY
     int fix32 = 0;                // FP32: int32 fixed-point LDS accumulators
-    int cell_order = 8;           // PIC_OPT_CELL_ORDER: 0 off; N: cell bins every N-th sort
+    int cell_order = 4;           // PIC_OPT_CELL_ORDER: 0 off; N: cell bins every N-th sort
     void* jtmp = nullptr;         // J-sized scratch of the current filter (lazy)
     void* stage = nullptr;        // output staging buffer (lazy, k_stage.hip)
     size_t stage_bytes = 0;
