@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void k_scatter64_dev(Soa64 A, Soa64 B, const i
             base = (int64_t)atomicAdd((unsigned long long*)&tile_cur[key], (unsigned long long)count);
         base = __shfl(base, leader);
         if (valid) {
-            const int64_t q = placed_slot(base + rank, key, T.tl.ntiles, tile_off, scramble);
+            const int64_t q = placed_slot(base + rank, key, T.tl.ntiles, tile_off, scramble, PIC_PUSH_CHUNK);
             B.x[q] = x; B.y[q] = y; B.z[q] = z; B.w[q] = w;
             B.ux[q] = A.ux[p]; B.uy[q] = A.uy[p]; B.uz[q] = A.uz[p];
             B.id[q] = A.id[p];

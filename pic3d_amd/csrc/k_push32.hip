@@ -46,8 +46,9 @@ struct Consts32 {
     // in units ufix[c] = (bound on one particle's contribution) * 2^-18, so
     // one chunk (<= 4096 = 2^12 particles, one contribution each per node)
     // stays below 2^30 and cannot overflow; rounding error per deposit is
-    // <= 2^-19 of that bound.  (Float atomic adds to LDS are an order of
-    // magnitude slower than integer ones on this part.)
+    // <= 2^-19 of that bound.  The quadratic push's chunks of 2^13 use
+    // 2^-17 (rounding <= 2^-18 of the bound).  (Float atomic adds to LDS are
+    // an order of magnitude slower than integer ones on this part.)
     float ufix[4], inv_ufix[4];
     float qm;          // q/m * dt/2
     float q;
@@ -909,12 +910,20 @@ __global__ __launch_bounds__(256) void k_push32_direct(Soa32 P, Fields32 F, Grid
 // ---------------------------------------------------------------------
 // tile sort (fp32 layout): drops removed particles (cell < 0)
 // ---------------------------------------------------------------------
+// Push chunk of the FP32 pushes: PIC_PUSH_CHUNK_Q32 for the quadratic
+// one-kernel push with fixed-point accumulators outside differentiated
+// runs, else PIC_PUSH_CHUNK (the two-kernel deposit lists a chunk in LDS).
+static int push_chunk32(const pic_ctx* c) {
+    return (c->fix32 && c->g.m == 2 && !c->diff) ? PIC_PUSH_CHUNK_Q32 : PIC_PUSH_CHUNK;
+}
+
 struct TileArgs32 {
     int n[3];
     Tiling tl;
     CellSplit cs;
     FastDiv td[3];  // tile edges
     int cpt;        // bins per tile: 1, or its cells (cell order inside tiles)
+    int chunk;      // push chunk of the in-tile transpose
 };
 
 // sort bin of a cell: its tile, or (cpt > 1) the tile's first bin plus the
@@ -1034,7 +1043,8 @@ __global__ __launch_bounds__(256) void k_scatter32(Soa32 A, Soa32 Bo, const int6
         for (int r = 0; r < R; r++) {
             const int64_t p = b + wseg + r * 64;
             if (!valid[r]) continue;
-            const int64_t q = placed_slot(base[r] + (lane_id() - head[r]), key[r], T.tl.ntiles, tile_off, scramble);
+            const int64_t q = placed_slot(base[r] + (lane_id() - head[r]), key[r], T.tl.ntiles, tile_off, scramble,
+                                            T.chunk);
             Bo.cell[q] = cell[r];
             Bo.fx[q] = v[r][0]; Bo.fy[q] = v[r][1]; Bo.fz[q] = v[r][2];
             Bo.ux[q] = v[r][3]; Bo.uy[q] = v[r][4]; Bo.uz[q] = v[r][5];
@@ -1057,6 +1067,7 @@ int sort32(pic_ctx* c, int s, int32_t* perm) {
     const bool fine = cellmode && (!S.cellsorted || S.sorts_since_fine + 1 >= c->cell_order);
     TileArgs32 T;
     T.cpt = fine ? c->tl.cells() : 1;
+    T.chunk = push_chunk32(c);
     const int64_t nbins = c->tl.nbins() * T.cpt + (perm ? 1 : 0);
     int r = launch_fill(c, c->tile_cnt, nbins * 8);
     if (r) return r;
@@ -1211,8 +1222,12 @@ static Consts32 consts32(const pic_ctx* c, const Species& S) {
     K.spin = c->spin;
     const double qw = std::fabs(S.q) * (S.wmax > 0 ? S.wmax : 1.0);
     const double bound[4] = {qw * c->p.dx / c->p.dt, qw * c->p.dy / c->p.dt, qw * c->p.dz / c->p.dt, qw};
+    // 2^-18 of the bound for chunks of 2^12 particles, 2^-17 for 2^13: a
+    // chunk's sum stays below 2^30 units either way
+    const int e = push_chunk32(c) == PIC_PUSH_CHUNK ? -18 : -17;
+    static_assert(PIC_PUSH_CHUNK == 4096 && PIC_PUSH_CHUNK_Q32 == 8192, "fixed-point unit exponents");
     for (int k = 0; k < 4; k++) {
-        const double u = std::ldexp(bound[k], -18);
+        const double u = std::ldexp(bound[k], e);
         K.ufix[k] = (float)u;
         K.inv_ufix[k] = (float)(1.0 / u);
     }
@@ -1223,12 +1238,13 @@ template <int M, int KIND, typename ACC, bool SPIN>
 static int launch_tiled32_s(pic_ctx* c, Species& S, Fields32 Fd, Consts32 K) {
     constexpr int TX = TileShape<KIND, M>::X, TY = TileShape<KIND, M>::Y, TZ = TileShape<KIND, M>::Z;
     using B = Box32<M, TX, TY, TZ, ACC>;
-    const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / PIC_PUSH_CHUNK + 1;
+    const int chunk = push_chunk32(c);
+    const int64_t max_items = (int64_t)c->tl.ntiles + S.cap / chunk + 1;
     PIC_HIP(c, hipFuncSetAttribute((const void*)k_push32_tiled<M, TX, TY, TZ, ACC, SPIN>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)B::LDS));
     k_push32_tiled<M, TX, TY, TZ, ACC, SPIN><<<(unsigned)max_items, Push32Threads<M, ACC, SPIN>::N, B::LDS,
                                                c->stream>>>(
-        soa32(S, S.cur), Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, PIC_PUSH_CHUNK, S.d_cnt);
+        soa32(S, S.cur), Fd, c->g, K, c->tl, S.tile_off, c->work, c->nwork, chunk, S.d_cnt);
     PIC_LAUNCH_CHECK(c);
     return PIC_OK;
 }
@@ -1291,7 +1307,7 @@ int launch_push32(pic_ctx* c, int s) {
             int r = sort32(c, s, nullptr);
             if (r) return r;
         }
-        int r = build_work(c, s, PIC_PUSH_CHUNK);
+        int r = build_work(c, s, push_chunk32(c));
         if (r) return r;
     }
     S.pushes_since_sort++;

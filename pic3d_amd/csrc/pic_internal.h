@@ -223,6 +223,13 @@ struct PhaseTimer {
 };
 
 constexpr int PIC_PUSH_CHUNK = 4096;  // particles per tiled-push workgroup
+// Particles per workgroup of the FP32 quadratic one-kernel push (512
+// threads, fixed-point accumulators): a chunk's field staging and
+// accumulator flush then serve twice the particles, and the in-tile
+// transpose puts a wave's lanes 128 ranks apart (dense cells: 228 particles
+// at the ramp target).  Not in differentiated runs, whose adjoint work
+// lists are cut in PIC_PUSH_CHUNK.
+constexpr int PIC_PUSH_CHUNK_Q32 = 8192;
 // the tiled pushes track one bit per particle a thread visits (256 threads)
 static_assert(PIC_PUSH_CHUNK <= 32 * 256, "stray mask holds 32 iterations");
 
@@ -257,13 +264,14 @@ __host__ __device__ inline void to_cell_offset(double v, int n, int& c, float& f
 
 // Slot of the particle at sorted position q of tile 'tile' (the sort's
 // cursor value): in order, or (scramble) at its transposed slot within its
-// chunk.  Removed slots (bin ntiles) stay in order.
+// push chunk of 'chunk' particles.  Removed slots (bin ntiles) stay in order.
 __device__ __forceinline__ int64_t placed_slot(int64_t q, int64_t tile, int ntiles,
-                                               const int64_t* __restrict__ tile_off, int scramble) {
+                                               const int64_t* __restrict__ tile_off, int scramble,
+                                               int chunk) {
     if (!scramble || tile >= ntiles) return q;
     const int64_t t0 = tile_off[tile], r = q - t0;
-    const int64_t c0 = r / PIC_PUSH_CHUNK * PIC_PUSH_CHUNK;
-    const int len = (int)min((int64_t)PIC_PUSH_CHUNK, tile_off[tile + 1] - t0 - c0);
+    const int64_t c0 = r / chunk * chunk;
+    const int len = (int)min((int64_t)chunk, tile_off[tile + 1] - t0 - c0);
     return t0 + c0 + stored_slot((int)(r - c0), len);
 }
 
