@@ -1223,9 +1223,11 @@ static Consts32 consts32(const pic_ctx* c, const Species& S) {
     const double qw = std::fabs(S.q) * (S.wmax > 0 ? S.wmax : 1.0);
     const double bound[4] = {qw * c->p.dx / c->p.dt, qw * c->p.dy / c->p.dt, qw * c->p.dz / c->p.dt, qw};
     // 2^-18 of the bound for chunks of 2^12 particles, 2^-17 for 2^13: a
-    // chunk's sum stays below 2^30 units either way
-    const int e = push_chunk32(c) == PIC_PUSH_CHUNK ? -18 : -17;
-    static_assert(PIC_PUSH_CHUNK == 4096 && PIC_PUSH_CHUNK_Q32 == 8192, "fixed-point unit exponents");
+    // chunk's sum stays below 2^30 units
+    int e = -30;
+    for (int n = push_chunk32(c); n > 1; n >>= 1) e++;
+    static_assert((PIC_PUSH_CHUNK & (PIC_PUSH_CHUNK - 1)) == 0 && (PIC_PUSH_CHUNK_Q32 & (PIC_PUSH_CHUNK_Q32 - 1)) == 0,
+                  "chunks are powers of two");
     for (int k = 0; k < 4; k++) {
         const double u = std::ldexp(bound[k], e);
         K.ufix[k] = (float)u;
