@@ -42,10 +42,26 @@ def fp32(p):
 @pytest.mark.parametrize("mem", [2, 4])
 def test_fp32_single_push(mem, kernel, fix):
     print(f"fp32 push mem {mem} kernel {kernel} fixed-point {fix}", flush=True)
-    rng = np.random.default_rng(21 + mem)
     wl = W.laser_slab(nx=24, ny=20, nz=18, ppc=4, slab=(0.0, 24.0), seed=8, mem=mem)
     wl.params.periodic[0] = 1
     wl.species[0].loader.u_th = 0.4
+    single_push(wl, kernel, fix, 21 + mem)
+
+
+def test_fp32_single_push_dense_tiles():
+    """quadratic, fixed point, 40 particles per cell: an 8x8x8 tile holds
+    about 20000 particles, so the quadratic push's workgroups take several
+    chunks of 8192 per tile (fixed-point unit 2^-17 of the bound); same
+    bars as the single push"""
+    print("fp32 push dense tiles", flush=True)
+    wl = W.laser_slab(nx=16, ny=16, nz=16, ppc=40, slab=(0.0, 16.0), seed=12, mem=2)
+    wl.params.periodic[0] = 1
+    wl.species[0].loader.u_th = 0.4
+    single_push(wl, 2, 1, 31)
+
+
+def single_push(wl, kernel, fix, seed):
+    rng = np.random.default_rng(seed)
     p = wl.params
     parts = load_particles(p, wl.species[0].loader)
     F = {c: rng.normal(0, 0.3, field_shape(p, c)).astype(np.float32).astype(np.float64)
