@@ -187,7 +187,8 @@ enum pic_option {
     PIC_OPT_TIMING = 4,        /* 1: time every phase with HIP events              */
     PIC_OPT_FP32_ACCUM = 5,    /* FP32 only: 1 (default) int32 fixed-point deposit
                                   accumulators in LDS (each add rounded to 2^-19 of
-                                  one particle's largest contribution); 0 FP64
+                                  one particle's largest contribution, 2^-18 in
+                                  the quadratic push's 8192-particle chunks); 0 FP64
                                   accumulators in LDS (smaller tiles, slower) */
     PIC_OPT_ADJ_PUSH = 6,      /* differentiated runs: 0 (default) the LDS-tiled
                                   adjoint push (HBM kernel after a direct forward);
