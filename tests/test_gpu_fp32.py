@@ -3,11 +3,11 @@
 Bars, stated per test:
   * one push on identical fields: positions within 2e-6 cells, momenta
     within 2e-6 relative (of max |u|), J/rho within 1e-5 norm-wise relative
-    (FP64 LDS accumulators, the default) or 5e-4 (int32 fixed-point LDS
-    accumulators: each add is rounded to 2^-19 of the largest single-particle
-    contribution, 2^-18 in the quadratic push's chunks of 8192); cell indices
-    identical except for particles closer than
-    1e-5 cells to a cell face.
+    (FP64 LDS accumulators) or 5e-4 (int32 fixed-point LDS accumulators, the
+    default: each add is rounded to 2^-19 of the largest single-particle
+    contribution, 2^-18 in the quadratic push's chunks of 8192); cell
+    indices identical except for particles closer than 1e-5 cells to a cell
+    face.
   * boundary losses/wraps per face within one particle of the oracle.
   * 32^3 thermal plasma, 100 steps: total energy within 1e-4 relative,
     field energy within 1e-2 relative (field energy is ~1e-4 of the total,
